@@ -1,0 +1,313 @@
+/*
+ * nbots_amd.h -- C-ABI of libnbots_amd.so, the MI355X (gfx950) FEM solve path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in symbols with the reference's names and signatures
+ *     (vecn/nbots).  Each comment cites the reference declaration it
+ *     replaces; behaviour, argument meaning and status codes are the
+ *     reference's.  The heavy work (SpMV, Jacobi-PCG, assembly, boundary
+ *     conditions, strain) runs on the GPU.
+ *
+ *  2. nbx_* extensions that the reference lacks and the north-star workloads
+ *     need: a device-resident FEM system (per-element modulus, reassembly
+ *     without re-uploading the mesh), a relative tolerance, mixed precision
+ *     (FP32 CG inside FP64 iterative refinement), SIMP compliance and
+ *     sensitivities, the adjoint solve, density filter and OC update.
+ *     These never call exit(); they return an nbx status code and
+ *     nbx_last_error() describes the failure.
+ *
+ * Plain pointers and sizes only.  Host pointers unless a name says "dev".
+ */
+#ifndef NBOTS_AMD_H
+#define NBOTS_AMD_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* Graph (public struct of the reference: headers/nb/graph_bot/graph.h:25-31) */
+/* ======================================================================== */
+typedef struct {
+	uint32_t N;
+	uint32_t *N_adj;
+	uint32_t **adj;
+	double *wi;   /* NULL for equal weights on vertices */
+	double **wij; /* NULL for equal weights on edges */
+} nb_graph_t;
+
+/* headers/nb/graph_bot/graph.h:33-37 */
+uint32_t nb_graph_get_memsize(void);
+void nb_graph_init(nb_graph_t *graph);
+void nb_graph_finish(nb_graph_t *graph);
+void nb_graph_clear(nb_graph_t *graph);
+
+/* ======================================================================== */
+/* Sparse matrix (headers/nb/solver_bot/sparse/sparse.h:9-78)                */
+/* ======================================================================== */
+typedef struct nb_sparse_s nb_sparse_t;   /* opaque: sparse.h:19 */
+
+/* sparse.h:28-30 -- pattern from a graph, vars_per_node dofs per node.  */
+nb_sparse_t *nb_sparse_create(const nb_graph_t *const graph,
+			      const uint32_t *const perm,
+			      uint32_t vars_per_node);
+nb_sparse_t *nb_sparse_clone(nb_sparse_t *A);                  /* sparse.h:32 */
+void nb_sparse_save(const nb_sparse_t *const A, const char *filename); /* :33 */
+void nb_sparse_destroy(nb_sparse_t *A);                        /* sparse.h:34 */
+void nb_sparse_reset(nb_sparse_t *A);                          /* sparse.h:35 */
+void nb_sparse_set(nb_sparse_t *A, uint32_t i, uint32_t j, double value); /* :36 */
+void nb_sparse_set_identity_row(nb_sparse_t *A, uint32_t row); /* sparse.h:37 */
+void nb_sparse_make_diagonal(nb_sparse_t *A, double diag_val); /* sparse.h:38 */
+double nb_sparse_get(const nb_sparse_t *const A, uint32_t i, uint32_t j); /* :39 */
+double nb_sparse_get_and_set(nb_sparse_t *A, uint32_t i, uint32_t j,
+			     double value);                    /* sparse.h:40 */
+bool nb_sparse_is_non_zero(const nb_sparse_t *const A, uint32_t i,
+			   uint32_t j);                        /* sparse.h:41 */
+void nb_sparse_add(nb_sparse_t *A, uint32_t i, uint32_t j, double value); /* :43 */
+void nb_sparse_scale(nb_sparse_t *A, double factor);           /* sparse.h:44 */
+uint32_t nb_sparse_get_size(const nb_sparse_t *const A);       /* sparse.h:47 */
+uint32_t nb_sparse_get_nnz(const nb_sparse_t *const A);        /* sparse.h:48 */
+double nb_sparse_get_asym(const nb_sparse_t *const A);         /* sparse.h:49 */
+double nb_sparse_get_frobenius_norm(const nb_sparse_t *const A); /* sparse.h:50 */
+/* sparse.h:52-55 -- omp_parallel_threads is accepted and ignored (GPU).  */
+void nb_sparse_multiply_scalar(nb_sparse_t *A, double scalar,
+			       uint32_t omp_parallel_threads);
+void nb_sparse_multiply_vector(const nb_sparse_t *A, const double *in,
+			       double *out, uint32_t omp_parallel_threads);
+/* sparse.h:60-61 -- symmetric elimination of one dof.                    */
+void nb_sparse_set_Dirichlet_condition(nb_sparse_t *A, double *RHS,
+				       uint32_t idx, double value);
+
+/* headers/nb/solver_bot/sparse/solvers/cg_precond_jacobi.h:8-15
+ * Jacobi-PCG on the GPU: absolute tolerance on ||Ax-b||_2, lagged stop
+ * test, x is the initial guess.  Returns 0 converged, 1 not converged
+ * (-1: GPU failure, see nbx_last_error()).                              */
+int nb_sparse_solve_CG_precond_Jacobi(const nb_sparse_t *const A,
+				      const double *const b,
+				      double *_x,
+				      uint32_t max_iter, double tolerance,
+				      uint32_t *niter_performed,
+				      double *tolerance_reached,
+				      uint32_t omp_parallel_threads);
+
+/* ======================================================================== */
+/* FEM objects used by the driver                                          */
+/* ======================================================================== */
+typedef enum { NB_TRIAN, NB_QUAD, NB_POLY, NB_DISK } nb_mesh2D_type; /* mesh2D.h:10-12 */
+typedef struct nb_mesh2D_s nb_mesh2D_t;                              /* mesh2D.h:14 */
+
+typedef enum { NB_TRG_LINEAR, NB_QUAD_LINEAR } nb_elem_id;           /* element.h:10-13 */
+typedef struct nb_fem_elem_s nb_fem_elem_t;                          /* element.h:8 */
+
+typedef struct nb_material_s nb_material_t;                          /* material.h */
+
+typedef enum { NB_PLANE_STRESS, NB_PLANE_STRAIN,
+	       NB_SOLID_OF_REVOLUTION } nb_analysis2D_t;             /* analysis2D.h:4-8 */
+typedef struct {
+	double thickness;
+	double revolution_axe;
+	char revolution_coordinate; /* 0: x, 1: y */
+} nb_analysis2D_params;                                              /* analysis2D.h:10-14 */
+
+typedef enum { NB_DIRICHLET, NB_NEUMANN } nb_bcond_id;               /* bcond.h:7-10 */
+typedef enum { NB_BC_ON_POINT, NB_BC_ON_SEGMENT } nb_bcond_where;    /* bcond.h:12-15 */
+typedef struct nb_bcond_s nb_bcond_t;                                /* bcond.h:17 */
+
+/* headers/nb/geometric_bot/mesh/mesh2D.h:16-25 (triangle meshes only)    */
+nb_mesh2D_t *nb_mesh2D_create(nb_mesh2D_type type);
+void nb_mesh2D_destroy(nb_mesh2D_t *mesh);
+uint32_t nb_mesh2D_get_N_invtx(const nb_mesh2D_t *mesh);          /* mesh2D.h:30 */
+uint32_t nb_mesh2D_get_N_insgm(const nb_mesh2D_t *mesh);          /* mesh2D.h:31 */
+uint32_t nb_mesh2D_get_N_nodes(const nb_mesh2D_t *mesh);          /* mesh2D.h:32 */
+uint32_t nb_mesh2D_get_N_elems(const nb_mesh2D_t *mesh);          /* mesh2D.h:34 */
+double nb_mesh2D_node_get_x(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:36 */
+double nb_mesh2D_node_get_y(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:37 */
+uint32_t nb_mesh2D_elem_get_adj(const nb_mesh2D_t *mesh, uint32_t elem_id,
+				uint8_t node_id);                 /* mesh2D.h:72 */
+uint32_t nb_mesh2D_get_invtx(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:79 */
+uint32_t nb_mesh2D_insgm_get_N_nodes(const nb_mesh2D_t *mesh, uint32_t id); /* :80 */
+uint32_t nb_mesh2D_insgm_get_node(const nb_mesh2D_t *mesh, uint32_t sgm_id,
+				  uint32_t node_id);              /* mesh2D.h:82 */
+/* nbx extension: the reference fills triangle meshes only through its
+ * Delaunay tessellator (nb_mesh2D_load_from_tessellator2D, mesh2D.h:93);
+ * this sets one from arrays: nodes nod[2*N_nod], triangles adj[3*N_elem]
+ * (counter-clockwise), input vertices vtx[N_vtx] (model vertex -> node),
+ * input segments as node lists nod_x_sgm[sgm_ptr[s]..sgm_ptr[s+1]). */
+int nbx_mesh2D_set_trg(nb_mesh2D_t *mesh, uint32_t N_nod, const double *nod,
+		       uint32_t N_elem, const uint32_t *adj,
+		       uint32_t N_vtx, const uint32_t *vtx,
+		       uint32_t N_sgm, const uint32_t *sgm_ptr,
+		       const uint32_t *nod_x_sgm);
+
+/* headers/nb/pde_bot/finite_element/element.h:15-26 */
+nb_fem_elem_t *nb_fem_elem_create(nb_elem_id type);
+void nb_fem_elem_destroy(nb_fem_elem_t *elemtype);
+uint8_t nb_fem_elem_get_N_gpoints(const nb_fem_elem_t *const elemtype);
+uint8_t nb_fem_elem_get_N_nodes(const nb_fem_elem_t *const elemtype);
+
+/* headers/nb/pde_bot/material.h */
+nb_material_t *nb_material_create(void);
+void nb_material_destroy(nb_material_t *mat);
+void nb_material_set_poisson_module(nb_material_t *mat, double pmodule);
+void nb_material_set_elasticity_module(nb_material_t *mat, double emodule);
+void nb_material_set_density(nb_material_t *mat, double density);
+double nb_material_get_poisson_module(const nb_material_t *const mat);
+double nb_material_get_elasticity_module(const nb_material_t *const mat);
+double nb_material_get_density(const nb_material_t *const mat);
+
+/* headers/nb/pde_bot/boundary_conditions/bcond.h:19-45 (FIFO lists) */
+uint16_t nb_bcond_get_memsize(uint8_t N_dof);
+void nb_bcond_init(void *bcond_ptr, uint8_t N_dof);
+void nb_bcond_finish(void *bcond_ptr);
+void *nb_bcond_create(uint8_t N_dof);
+void nb_bcond_destroy(void *bcond_ptr);
+void nb_bcond_clear(void *bcond_ptr);
+uint8_t nb_bcond_get_N_dof(const nb_bcond_t *const bcond);
+void nb_bcond_push(nb_bcond_t *bcond, nb_bcond_id type_id,
+		   nb_bcond_where type_elem, uint32_t elem_id,
+		   const bool dof_mask[], const double value[]);
+void nb_bcond_push_function(nb_bcond_t *bcond, nb_bcond_id type_id,
+			    nb_bcond_where type_elem, uint32_t elem_id,
+			    const bool dof_mask[],
+			    void (*fval)(const double *x, double t, double *out));
+
+/* headers/nb/pde_bot/finite_element/solid_mechanics/static_elasticity2D.h:
+ * 13-24.  Returns 0 ok, 1 assembly failed (distorted element), 2 solver
+ * failed (unreachable, as in the reference: a non-converged CG counts as
+ * success, static_elasticity2D.c:91-96); -1 GPU failure. */
+int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t *const part,
+				      const nb_fem_elem_t *const elemtype,
+				      const nb_material_t *const material,
+				      const nb_bcond_t *const bcond,
+				      bool enable_self_weight,
+				      double gravity[2],
+				      nb_analysis2D_t analysis2D,
+				      nb_analysis2D_params *params2D,
+				      const bool *elements_enabled,
+				      double *displacement,
+				      double *strain);
+/* static_elasticity2D.h:26-33 */
+void nb_fem_compute_stress_from_strain(uint32_t N_elements,
+				       const nb_fem_elem_t *const elemtype,
+				       const nb_material_t *const material,
+				       nb_analysis2D_t analysis2D,
+				       double *strain,
+				       const bool *elements_enabled,
+				       double *stress);
+
+/* ======================================================================== */
+/* nbx extensions                                                           */
+/* ======================================================================== */
+enum {
+	NBX_OK = 0,
+	NBX_NOT_CONVERGED = 1,
+	NBX_ERR_DISTORTED = 2,
+	NBX_ERR_HIP = 3,
+	NBX_ERR_ARG = 4,
+	NBX_ERR_PATTERN = 5,
+	NBX_ERR_BREAKDOWN = 6
+};
+enum {
+	NBX_SOLVE_REFERENCE = 0, /* abs tol on ||Ax-b||, lagged test, x0 given */
+	NBX_SOLVE_FP64 = 1,      /* Jacobi-PCG, relative tolerance           */
+	NBX_SOLVE_MIXED = 2      /* FP32 CG inside FP64 refinement, rel. tol */
+};
+
+typedef struct {
+	uint32_t iters;       /* CG iterations (FP32 inner ones for MIXED) */
+	uint32_t outer;       /* refinement sweeps (MIXED)                  */
+	uint32_t fp64_iters;  /* FP64 PCG iterations (incl. a fallback)     */
+	int32_t status;       /* 0 converged, 1 not, else nbx error         */
+	double rel_res;       /* final ||b-Ax||/||b|| (FP64)                */
+	double tol_reached;   /* reference mode: the lagged ||g||           */
+	double ms;            /* device time of the solve                   */
+	int32_t fallback;     /* MIXED sweeps stalled, FP64 PCG finished    */
+	int32_t pad;
+} nbx_solve_info_t;
+
+const char *nbx_last_error(void);
+int nbx_device_info(char *name, uint32_t len, uint32_t *n_cu, double *hbm_gb);
+
+/* Solve with a drop-in matrix in any mode (no reference counterpart).    */
+int nbx_sparse_solve(const nb_sparse_t *A, const double *b, double *x,
+		     int mode, double tol, uint32_t max_iter,
+		     double inner_rtol, nbx_solve_info_t *info);
+/* y = A x with device pointers (no reference counterpart; benchmarking). */
+int nbx_sparse_spmv_dev(const nb_sparse_t *A, const double *x_dev,
+			double *y_dev);
+/* Bulk copies in CSR order (rowptr[N+1], col[nnz], val[nnz]; any may be
+ * NULL).  The reference only offers per-entry access (sparse.h:36-43).  */
+int nbx_sparse_export_csr(const nb_sparse_t *A, uint32_t *rowptr,
+			  uint32_t *col, double *val);
+int nbx_sparse_import_values(nb_sparse_t *A, const double *val);
+/* 1 when the device copy uses the 2x2-block format. */
+int nbx_sparse_is_blocked(const nb_sparse_t *A);
+
+/* ---- device-resident T3 plane-stress system ----------------------------- */
+typedef struct nbx_fem_s nbx_fem_t;
+
+/* Upload a triangle mesh and build the reference's 2-dof pattern
+ * (load_graph.c:230-307 + sparse.c:20-60) on the device.              */
+int nbx_fem_create(nbx_fem_t **out, uint32_t N_nod, const double *nod,
+		   uint32_t N_elem, const uint32_t *conn);
+void nbx_fem_destroy(nbx_fem_t *sys);
+/* N (dofs), nnz of the scalar pattern, stored 2x2 blocks incl. padding */
+int nbx_fem_sizes(const nbx_fem_t *sys, uint32_t *N, uint64_t *nnz,
+		  uint64_t *blocks_stored);
+/* K and body-force F (pipeline.c:42-264) on the device.  elem_E
+ * (nullable) replaces E per element; enabled (nullable) disables
+ * elements (D = 1e-6).  Returns NBX_ERR_DISTORTED for detJ < 0.        */
+int nbx_fem_assemble(nbx_fem_t *sys, double E, double nu, double thickness,
+		     const double *elem_E, const uint8_t *enabled,
+		     int self_weight, double density, const double *gravity);
+/* Boundary conditions in the reference order (set_bconditions.c:52-61):
+ * vtx[] maps model vertices to nodes, segments are node lists; four
+ * lists (Dirichlet vertex, Neumann vertex, Dirichlet segment, Neumann
+ * segment), each n, id[n], mask[2n], val[2n]; a Neumann segment with
+ * ns_fptr[c] >= 0 takes nodal samples ns_fval[2*(ns_fptr[c]+i)+j].     */
+int nbx_fem_set_bconditions(nbx_fem_t *sys, const uint32_t *vtx,
+			    const uint32_t *sgm_ptr, const uint32_t *sgm_nodes,
+			    uint32_t n_dv, const uint32_t *dv_id,
+			    const uint8_t *dv_mask, const double *dv_val,
+			    uint32_t n_nv, const uint32_t *nv_id,
+			    const uint8_t *nv_mask, const double *nv_val,
+			    uint32_t n_ds, const uint32_t *ds_id,
+			    const uint8_t *ds_mask, const double *ds_val,
+			    uint32_t n_ns, const uint32_t *ns_id,
+			    const uint8_t *ns_mask, const double *ns_val,
+			    const int32_t *ns_fptr, const double *ns_fval,
+			    double factor);
+/* Solve K u = F.  mode NBX_SOLVE_*, tol absolute (REFERENCE) or relative;
+ * u_init (nullable) is the initial guess, else 0.                      */
+int nbx_fem_solve(nbx_fem_t *sys, int mode, double tol, uint32_t max_iter,
+		  double inner_rtol, const double *u_init,
+		  nbx_solve_info_t *info);
+int nbx_fem_get_displacement(const nbx_fem_t *sys, double *u);
+/* K and F in the reference's scalar CSR layout (row-sorted columns).   */
+int nbx_fem_get_system(const nbx_fem_t *sys, uint32_t *rowptr, uint32_t *col,
+		       double *val, double *F);
+/* strain (pipeline.c:266-319) and stress (static_elasticity2D.c:99-127)
+ * of the current displacement, 3 values per element.                   */
+int nbx_fem_strain_stress(const nbx_fem_t *sys, double *strain,
+			  double *stress);
+/* SIMP compliance C = F.u and dC/drho_e = -p rho^(p-1) (E0-Emin) u_e'K0u_e */
+int nbx_fem_compliance(nbx_fem_t *sys, const double *rho, double penal,
+		       double E0, double Emin, double *compliance,
+		       double *dc, double *ce);
+/* Adjoint solve K lambda = rhs with the system's solver (any objective). */
+int nbx_fem_adjoint(nbx_fem_t *sys, const double *rhs, double *lambda,
+		    int mode, double rtol, uint32_t max_iter,
+		    nbx_solve_info_t *info);
+/* Device time of the last SpMV-dominated kernel sequence (ms). */
+int nbx_fem_spmv_bench(nbx_fem_t *sys, int precision, uint32_t reps,
+		       double *ms_per_spmv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

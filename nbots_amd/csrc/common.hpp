@@ -1,0 +1,254 @@
+// Shared definitions for the nbots MI355X (gfx950) solve path.
+//
+// Every translation unit of libnbots_amd.so is compiled with
+// -ffp-contract=off: the reference's arithmetic has no fused multiply-add
+// (its default build targets plain x86-64), and the assembly, SpMV, strain
+// and BC kernels reproduce its results bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace nbx {
+
+// ---- error handling --------------------------------------------------------
+// New entry points never exit(); they return one of these codes.
+enum Status : int {
+    NBX_OK = 0,
+    NBX_NOT_CONVERGED = 1,
+    NBX_ERR_DISTORTED = 2,      // an element with detJ < 0
+    NBX_ERR_HIP = 3,            // a HIP runtime call failed
+    NBX_ERR_ARG = 4,            // invalid argument / size
+    NBX_ERR_PATTERN = 5,        // a row has more blocks than the format holds
+    NBX_ERR_BREAKDOWN = 6,      // non-finite or non-positive curvature in CG
+};
+
+void set_last_error(const char* fmt, ...);
+const char* last_error();
+
+#define NBX_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::nbx::set_last_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, \
+                                  hipGetErrorString(e_));                      \
+            return ::nbx::NBX_ERR_HIP;                                         \
+        }                                                                      \
+    } while (0)
+
+#define NBX_TRY(call)                 \
+    do {                              \
+        int rc_ = (call);             \
+        if (rc_ != 0) return rc_;     \
+    } while (0)
+
+// ---- launch geometry -------------------------------------------------------
+constexpr int WAVE = 64;          // CDNA wavefront
+constexpr int BLOCK = 256;        // 4 waves per workgroup
+constexpr int SLICE = 64;         // sliced-ELL slice height = one wavefront
+constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, 32 CUs each
+constexpr int MAX_GRID = 2048;    // reductions: at most this many workgroups
+
+inline unsigned grid_for(size_t n, int per_block = BLOCK) {
+    size_t g = (n + per_block - 1) / per_block;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8).  Remapping b so that each XCD walks one contiguous eighth
+// of the rows keeps the neighbours' x entries of a sparse row in that XCD's
+// own L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    unsigned per = (nb + NUM_XCD - 1) / NUM_XCD;
+    unsigned xcd = b % NUM_XCD, pos = b / NUM_XCD;
+    unsigned full = nb % NUM_XCD;             // XCDs with `per` blocks
+    if (full == 0) return xcd * per + pos;
+    // the first `full` XCDs own `per` blocks, the rest `per-1`
+    if (xcd < full) return xcd * per + pos;
+    return full * per + (xcd - full) * (per - 1) + pos;
+}
+
+// ---- 2x2-block sliced ELL (node rows of a 2-dof matrix) -------------------
+// Node row v holds rowlen[v] blocks; block k of row v is stored at
+//   slice_ptr[v / 64] + k * 64 + (v % 64)
+// so that the 64 lanes of a wavefront read block k of 64 consecutive rows as
+// one contiguous run.  Block columns within a row ascend, exactly as the
+// scalar rows 2v, 2v+1 of the reference's CSR (columns 2w, 2w+1 per block).
+// Values of block (v,w): {a(2v,2w), a(2v,2w+1), a(2v+1,2w), a(2v+1,2w+1)}.
+struct Sell2 {
+    uint32_t n_nodes = 0;
+    uint32_t n_slices = 0;
+    size_t n_blocks = 0;           // stored blocks, padding included
+    size_t n_valid = 0;            // blocks in the pattern (nnz / 4)
+    uint32_t max_rowlen = 0;
+    uint32_t* slice_ptr = nullptr; // [n_slices + 1]
+    uint8_t* rowlen = nullptr;     // [n_nodes]
+    uint32_t* bcol = nullptr;      // [n_blocks]
+    double* val = nullptr;         // [4 * n_blocks]
+    float* val32 = nullptr;        // [4 * n_blocks], Jacobi-scaled FP32 copy
+};
+
+// Generic scalar CSR on the device (any vars_per_node).
+struct DevCsr {
+    uint32_t n = 0;
+    size_t nnz = 0;
+    uint32_t* rowptr = nullptr;
+    uint32_t* col = nullptr;
+    double* val = nullptr;
+};
+
+// ---- compensated FP32 accumulation (TwoSum, Knuth) -------------------------
+struct f32c {
+    float s, c;
+};
+__device__ __forceinline__ f32c two_sum(float a, float b) {
+    float s = a + b;
+    float bb = s - a;
+    float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+__device__ __forceinline__ f32c cadd(f32c x, float v) {
+    f32c t = two_sum(x.s, v);
+    return {t.s, x.c + t.c};
+}
+__device__ __forceinline__ f32c cadd(f32c x, f32c y) {
+    f32c t = two_sum(x.s, y.s);
+    return {t.s, (x.c + y.c) + t.c};
+}
+
+// ---- wavefront / workgroup reductions in a fixed order --------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, WAVE);
+    return v;
+}
+__device__ __forceinline__ f32c wave_sum(f32c v) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        f32c o{__shfl_down(v.s, off, WAVE), __shfl_down(v.c, off, WAVE)};
+        v = cadd(v, o);
+    }
+    return v;
+}
+
+// Sum of v over the workgroup; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*smem)[BLOCK / WAVE]) {
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = wave_sum(v[i]);
+        if (lane == 0) smem[i][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int w = 0; w < BLOCK / WAVE; ++w) s += smem[i][w];
+            v[i] = s;
+        }
+    }
+}
+template <int NV>
+__device__ __forceinline__ void block_sum(f32c (&v)[NV], f32c (*smem)[BLOCK / WAVE]) {
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        f32c s = wave_sum(v[i]);
+        if (lane == 0) smem[i][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            f32c s{0.f, 0.f};
+            for (int w = 0; w < BLOCK / WAVE; ++w) s = cadd(s, smem[i][w]);
+            v[i] = s;
+        }
+    }
+}
+
+// Last-workgroup-done grid reduction.  Each workgroup stores its partials;
+// the workgroup that takes the final ticket sums all partials in index
+// order, so the result does not depend on scheduling.  Returns true in
+// thread 0 of that last workgroup, with tot[] holding the sums.
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_agent(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NV>
+__device__ bool grid_sum_last(double (&v)[NV], double* partials, unsigned* ticket,
+                              double (&tot)[NV]) {
+    __shared__ bool s_last;
+    const unsigned nb = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) partials[(size_t)i * nb + blockIdx.x] = v[i];
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        s_last = (t == nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    // the last workgroup sums the partials with all its threads
+    __shared__ double red[NV][BLOCK / WAVE];
+    double loc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) s += ld_agent(&partials[(size_t)i * nb + b]);
+        loc[i] = s;
+    }
+    block_sum<NV>(loc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) tot[i] = loc[i];
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return threadIdx.x == 0;
+}
+
+template <int NV>
+__device__ bool grid_sum_last(f32c (&v)[NV], float* partials, unsigned* ticket,
+                              f32c (&tot)[NV]) {
+    __shared__ bool s_last;
+    const unsigned nb = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            partials[(size_t)(2 * i) * nb + blockIdx.x] = v[i].s;
+            partials[(size_t)(2 * i + 1) * nb + blockIdx.x] = v[i].c;
+        }
+        __threadfence();
+        unsigned t = atomicAdd(ticket, 1u);
+        s_last = (t == nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    __shared__ f32c red[NV][BLOCK / WAVE];
+    f32c loc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        f32c s{0.f, 0.f};
+        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) {
+            f32c o{ld_agent(&partials[(size_t)(2 * i) * nb + b]),
+                   ld_agent(&partials[(size_t)(2 * i + 1) * nb + b])};
+            s = cadd(s, o);
+        }
+        loc[i] = s;
+    }
+    block_sum<NV>(loc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) tot[i] = loc[i];
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return threadIdx.x == 0;
+}
+
+}  // namespace nbx

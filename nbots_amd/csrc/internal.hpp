@@ -1,0 +1,90 @@
+// Host-side interfaces between the C-ABI layer and the HIP kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace nbx {
+
+// Scalar state of a device-resident CG run; written only by the last
+// workgroup of a reduction kernel, read by the following kernels and polled
+// by the host.
+struct PcgState {
+    double rz;            // g.q (Jacobi-PCG) or r.r (scaled CG) of the current iterate
+    double gg_cur;        // ||g_k||^2
+    double gg_prev;       // ||g_{k-1}||^2, the value the reference's stop test sees
+    double pw;            // p.w of the current iteration
+    double alpha, beta;
+    double tol2;          // stop threshold on gg (absolute) or rz (relative, inner)
+    double tol_reached;
+    uint32_t k, max_iter;
+    uint32_t done, status;
+    uint32_t breakdown, pad;
+};
+
+// A device matrix: 2-dof blocked sliced ELL when the pattern allows it,
+// otherwise scalar CSR.
+struct DevMatrix {
+    bool blocked = false;
+    uint32_t n = 0;            // scalar rows
+    Sell2 s;
+    DevCsr c;
+    double* diag = nullptr;    // [n] diagonal A(i,i) (0.0 if not stored)
+    double* dis = nullptr;     // [n] 1/sqrt(diag) for the scaled FP32 copy
+    bool has_val32 = false;
+};
+
+// Per-matrix solver scratch (device).  Sized for n scalar rows.
+struct Workspace {
+    uint32_t n = 0;
+    double* v64[6] = {};       // FP64 vectors
+    float* v32[4] = {};        // FP32 vectors
+    double* partials = nullptr;
+    float* partials32 = nullptr;
+    unsigned* ticket = nullptr;
+    PcgState* st = nullptr;    // device state (2 slots: outer, inner)
+    PcgState* h_st = nullptr;  // pinned host mirror
+    hipEvent_t ev[4] = {};
+};
+
+struct SolveStats {
+    uint32_t iters = 0;        // CG iterations (inner FP32 iterations for mixed)
+    uint32_t outer = 0;        // refinement sweeps (mixed)
+    uint32_t fp64_iters = 0;   // FP64 PCG iterations (incl. fallback)
+    double rel_res = 0.0;      // final ||b - A x|| / ||b|| (FP64)
+    double ms = 0.0;           // device time of the solve
+    double ms_spmv_avg = 0.0;  // average duration of the dominant SpMV kernel
+    int fallback = 0;          // 1 if the FP32 sweeps stalled and FP64 PCG finished
+};
+
+int ws_alloc(Workspace& w, uint32_t n);
+void ws_free(Workspace& w);
+void mat_free(DevMatrix& m);
+
+// Upload a host CSR (row-sorted columns) into a device matrix, blocked when
+// possible.  Allocates on first use; `values_only` refreshes values.
+int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rowptr, const uint32_t* col,
+                   const double* val, bool values_only, hipStream_t s);
+int mat_extract_diag(DevMatrix& m, hipStream_t s);
+int mat_make_scaled32(DevMatrix& m, hipStream_t s);
+
+int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s);
+
+// Reference-semantics Jacobi PCG (absolute tolerance on ||Ax-b||, lagged stop
+// test); x holds the initial guess.  Returns 0 converged / 1 not, <0 error.
+int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter,
+               double tol, uint32_t* niter, double* tol_reached, Workspace& w,
+               hipStream_t s, SolveStats* stats);
+
+// FP32 inner CG on the Jacobi-scaled FP32 copy inside FP64 iterative
+// refinement; stops when ||b - A x||_2 <= rtol * ||b||_2.
+int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol,
+                double inner_rtol, uint32_t max_iter, Workspace& w, hipStream_t s,
+                SolveStats* stats);
+
+// FP64 Jacobi PCG with a relative tolerance (x0 = x on input).
+int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol,
+               uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats);
+
+double device_norm2(uint32_t n, const double* v, Workspace& w, hipStream_t s, int* err);
+
+}  // namespace nbx

@@ -1,0 +1,378 @@
+// Drop-in nb_sparse_t: the reference's sparse-matrix API
+// (headers/nb/solver_bot/sparse/sparse.h:28-78) with SpMV and Jacobi-PCG on
+// the GPU.
+//
+// Host storage is one flat CSR (rowptr, col, val) instead of the reference's
+// arrays of row pointers (solver_bot/sparse/sparse_struct.h:6-11); entry
+// access keeps the reference's semantics (binary search in a sorted row,
+// exit(1) on an entry outside the pattern, sparse.c:134-219).  The device
+// copy is refreshed lazily: any host-side write marks the values stale, and
+// the next GPU operation uploads them.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/nbots_amd.h"
+#include "internal.hpp"
+
+struct nb_sparse_s {
+    uint32_t N = 0;
+    std::vector<uint32_t> rowptr;
+    std::vector<uint32_t> col;
+    std::vector<double> val;
+    // device mirror
+    mutable nbx::DevMatrix dev;
+    mutable nbx::Workspace ws;
+    mutable hipStream_t stream = nullptr;
+    mutable bool dev_pattern = false;
+    mutable bool dev_values = false;
+};
+
+namespace {
+
+int64_t find_slot(const nb_sparse_t* A, uint32_t i, uint32_t j) {
+    if (i >= A->N) return -1;
+    const uint32_t* b = A->col.data() + A->rowptr[i];
+    const uint32_t* e = A->col.data() + A->rowptr[i + 1];
+    const uint32_t* it = std::lower_bound(b, e, j);
+    if (it == e || *it != j) return -1;
+    return it - A->col.data();
+}
+
+[[noreturn]] void not_allocated(const char* fn, const nb_sparse_t* A, uint32_t i, uint32_t j,
+                                double value) {
+    // same message and exit as the reference (sparse.c:140-144)
+    printf("ERROR: Entry of sparse matrix is not allocated.\n");
+    printf("    -> %s(*A=%p, i=%i, j=%i, value=%lf)\n", fn, (const void*)A, i, j, value);
+    exit(1);
+}
+
+// Bring the device copy up to date; returns an nbx status.
+int sync_device(const nb_sparse_t* A) {
+    if (!A->stream) {
+        hipError_t e = hipStreamCreate(&A->stream);
+        if (e != hipSuccess) {
+            nbx::set_last_error("hipStreamCreate: %s", hipGetErrorString(e));
+            return nbx::NBX_ERR_HIP;
+        }
+    }
+    if (A->dev_pattern && A->dev_values) return nbx::NBX_OK;
+    int rc = nbx::mat_upload_csr(A->dev, A->N, A->rowptr.data(), A->col.data(), A->val.data(),
+                                 A->dev_pattern, A->stream);
+    if (rc) return rc;
+    A->dev_pattern = true;
+    A->dev_values = true;
+    return nbx::ws_alloc(A->ws, A->N);
+}
+
+void stale(nb_sparse_t* A) { A->dev_values = false; }
+
+}  // namespace
+
+extern "C" {
+
+// ---- graph (graph.c:20-40) -------------------------------------------------
+uint32_t nb_graph_get_memsize(void) { return sizeof(nb_graph_t); }
+void nb_graph_init(nb_graph_t* g) { memset(g, 0, sizeof *g); }
+void nb_graph_clear(nb_graph_t* g) {
+    if (g->adj) {
+        for (uint32_t i = 0; i < g->N; ++i) free(g->adj[i]);
+        free(g->adj);
+    }
+    free(g->N_adj);
+    if (g->wij) {
+        for (uint32_t i = 0; i < g->N; ++i) free(g->wij[i]);
+        free(g->wij);
+    }
+    free(g->wi);
+    memset(g, 0, sizeof *g);
+}
+void nb_graph_finish(nb_graph_t* g) { nb_graph_clear(g); }
+
+// ---- creation --------------------------------------------------------------
+// sparse.c:20-60: row perm[i]*vpn+k1 (or i*vpn+k1) holds the columns
+// {adj(i), i} x vpn in ascending order; columns are not permuted.
+nb_sparse_t* nb_sparse_create(const nb_graph_t* const graph, const uint32_t* const perm,
+                              uint32_t vpn) {
+    nb_sparse_t* A = new nb_sparse_t;
+    const uint32_t N = graph->N * vpn;
+    A->N = N;
+    std::vector<uint32_t> size(N, 0);
+    for (uint32_t i = 0; i < graph->N; ++i)
+        for (uint32_t k1 = 0; k1 < vpn; ++k1) {
+            const uint32_t r = perm ? perm[i] * vpn + k1 : i * vpn + k1;
+            size[r] = (graph->N_adj[i] + 1) * vpn;
+        }
+    A->rowptr.assign(N + 1, 0);
+    for (uint32_t r = 0; r < N; ++r) A->rowptr[r + 1] = A->rowptr[r] + size[r];
+    A->col.assign(A->rowptr[N], 0);
+    A->val.assign(A->rowptr[N], 0.0);
+    for (uint32_t i = 0; i < graph->N; ++i)
+        for (uint32_t k1 = 0; k1 < vpn; ++k1) {
+            const uint32_t r = perm ? perm[i] * vpn + k1 : i * vpn + k1;
+            uint32_t* row = A->col.data() + A->rowptr[r];
+            for (uint32_t j = 0; j < graph->N_adj[i]; ++j)
+                for (uint32_t k2 = 0; k2 < vpn; ++k2) row[j * vpn + k2] = graph->adj[i][j] * vpn + k2;
+            for (uint32_t k2 = 0; k2 < vpn; ++k2) row[graph->N_adj[i] * vpn + k2] = i * vpn + k2;
+            std::sort(row, row + size[r]);
+        }
+    return A;
+}
+
+nb_sparse_t* nb_sparse_clone(nb_sparse_t* A) {
+    nb_sparse_t* B = new nb_sparse_t;
+    B->N = A->N;
+    B->rowptr = A->rowptr;
+    B->col = A->col;
+    B->val = A->val;
+    return B;
+}
+
+void nb_sparse_save(const nb_sparse_t* const A, const char* filename) {
+    // text format of sparse.c:97-112
+    FILE* fp = fopen(filename, "w");
+    if (!fp) {
+        printf("Error: Opening the file %s to save the matrix.\n", filename);
+        return;
+    }
+    fprintf(fp, "%i %i\n", A->N, A->N);
+    for (uint32_t i = 0; i < A->N; ++i)
+        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1]; ++s)
+            fprintf(fp, "%i %i %e \n", i, A->col[s], A->val[s]);
+    fclose(fp);
+}
+
+void nb_sparse_destroy(nb_sparse_t* A) {
+    if (!A) return;
+    nbx::mat_free(A->dev);
+    nbx::ws_free(A->ws);
+    if (A->stream) (void)hipStreamDestroy(A->stream);
+    delete A;
+}
+
+void nb_sparse_reset(nb_sparse_t* A) {
+    std::fill(A->val.begin(), A->val.end(), 0.0);
+    stale(A);
+}
+
+// ---- entry access (sparse.c:134-219) ----------------------------------------
+void nb_sparse_set(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
+    const int64_t s = find_slot(A, i, j);
+    if (s < 0) not_allocated("nb_sparse_set", A, i, j, value);
+    A->val[s] = value;
+    stale(A);
+}
+
+void nb_sparse_set_identity_row(nb_sparse_t* A, uint32_t row) {
+    std::fill(A->val.begin() + A->rowptr[row], A->val.begin() + A->rowptr[row + 1], 0.0);
+    nb_sparse_set(A, row, row, 1.0);
+}
+
+void nb_sparse_make_diagonal(nb_sparse_t* A, double diag_val) {
+    for (uint32_t i = 0; i < A->N; ++i) nb_sparse_set_identity_row(A, i), nb_sparse_set(A, i, i, diag_val);
+}
+
+double nb_sparse_get(const nb_sparse_t* const A, uint32_t i, uint32_t j) {
+    const int64_t s = find_slot(A, i, j);
+    return s < 0 ? 0.0 : A->val[s];
+}
+
+double nb_sparse_get_and_set(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
+    const int64_t s = find_slot(A, i, j);
+    if (s < 0) not_allocated("nb_sparse_get_and_set", A, i, j, value);
+    const double old = A->val[s];
+    A->val[s] = value;
+    stale(A);
+    return old;
+}
+
+bool nb_sparse_is_non_zero(const nb_sparse_t* const A, uint32_t i, uint32_t j) {
+    return find_slot(A, i, j) >= 0;
+}
+
+void nb_sparse_add(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
+    const int64_t s = find_slot(A, i, j);
+    if (s < 0) not_allocated("nb_sparse_add", A, i, j, value);
+    A->val[s] += value;
+    stale(A);
+}
+
+void nb_sparse_scale(nb_sparse_t* A, double factor) {
+    for (double& v : A->val) v *= factor;
+    stale(A);
+}
+
+void nb_sparse_multiply_scalar(nb_sparse_t* A, double scalar, uint32_t) {
+    nb_sparse_scale(A, scalar);
+}
+
+uint32_t nb_sparse_get_size(const nb_sparse_t* const A) { return A->N; }
+uint32_t nb_sparse_get_nnz(const nb_sparse_t* const A) { return A->rowptr[A->N]; }
+
+double nb_sparse_get_asym(const nb_sparse_t* const A) {
+    // sparse.c:362-379
+    double sum = 0;
+    for (uint32_t i = 0; i < A->N; ++i)
+        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1] && A->col[s] < i; ++s) {
+            const double aji = nb_sparse_get(A, A->col[s], i);
+            sum += (A->val[s] - aji) * (A->val[s] - aji);
+        }
+    return sqrt(sum);
+}
+
+double nb_sparse_get_frobenius_norm(const nb_sparse_t* const A) {
+    double sum = 0;
+    for (double v : A->val) sum += v * v;
+    return sqrt(sum);
+}
+
+// sparse.c:416-430: zero row idx and column idx, diagonal 1, RHS[idx] = value
+// and RHS[j] -= A[j][idx] * value for the other rows j of the column.
+void nb_sparse_set_Dirichlet_condition(nb_sparse_t* A, double* RHS, uint32_t idx, double value) {
+    for (uint32_t s = A->rowptr[idx]; s < A->rowptr[idx + 1]; ++s) {
+        const uint32_t jdx = A->col[s];
+        if (idx == jdx) {
+            A->val[s] = 1.0;
+            RHS[idx] = value;
+        } else {
+            A->val[s] = 0.0;
+            const double var = nb_sparse_get_and_set(A, jdx, idx, 0.0);
+            RHS[jdx] -= var * value;
+        }
+    }
+    stale(A);
+}
+
+// ---- GPU operations -----------------------------------------------------------
+// sparse.c:405-414, on the device (bit-identical row sums).
+void nb_sparse_multiply_vector(const nb_sparse_t* A, const double* in, double* out, uint32_t) {
+    if (A->N == 0) return;
+    int rc = sync_device(A);
+    double* dx = A->ws.v64[3];
+    double* dy = A->ws.v64[4];
+    if (rc == 0 && hipMemcpyAsync(dx, in, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream) != hipSuccess)
+        rc = nbx::NBX_ERR_HIP;
+    if (rc == 0) rc = nbx::spmv(A->dev, dx, dy, A->stream);
+    if (rc == 0 && (hipMemcpyAsync(out, dy, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream) != hipSuccess ||
+                    hipStreamSynchronize(A->stream) != hipSuccess))
+        rc = nbx::NBX_ERR_HIP;
+    if (rc) {
+        // the reference has no error path here; report and leave NaNs
+        fprintf(stderr, "nb_sparse_multiply_vector: GPU failure: %s\n", nbx::last_error());
+        for (uint32_t i = 0; i < A->N; ++i) out[i] = NAN;
+    }
+}
+
+int nb_sparse_solve_CG_precond_Jacobi(const nb_sparse_t* const A, const double* const b,
+                                      double* _x, uint32_t max_iter, double tolerance,
+                                      uint32_t* niter_performed, double* tolerance_reached,
+                                      uint32_t) {
+    nbx_solve_info_t info;
+    int rc = nbx_sparse_solve(A, b, _x, NBX_SOLVE_REFERENCE, tolerance, max_iter, 0.0, &info);
+    if (rc != NBX_OK && rc != NBX_NOT_CONVERGED) {
+        fprintf(stderr, "nb_sparse_solve_CG_precond_Jacobi: GPU failure: %s\n", nbx::last_error());
+        return -1;
+    }
+    if (niter_performed) *niter_performed = info.iters;
+    if (tolerance_reached) *tolerance_reached = info.tol_reached;
+    return rc;
+}
+
+const char* nbx_last_error(void) { return nbx::last_error(); }
+
+int nbx_device_info(char* name, uint32_t len, uint32_t* n_cu, double* hbm_gb) {
+    hipDeviceProp_t p;
+    NBX_HIP(hipGetDeviceProperties(&p, 0));
+    if (name && len) {
+        strncpy(name, p.gcnArchName, len - 1);
+        name[len - 1] = 0;
+    }
+    if (n_cu) *n_cu = (uint32_t)p.multiProcessorCount;
+    if (hbm_gb) *hbm_gb = (double)p.totalGlobalMem / 1e9;
+    return 0;
+}
+
+// Bulk copies in the matrix's CSR order (no reference counterpart: the
+// reference exposes entries one at a time, sparse.h:36-43).
+int nbx_sparse_export_csr(const nb_sparse_t* A, uint32_t* rowptr, uint32_t* col, double* val) {
+    if (rowptr) memcpy(rowptr, A->rowptr.data(), (A->N + 1) * sizeof(uint32_t));
+    if (col) memcpy(col, A->col.data(), A->col.size() * sizeof(uint32_t));
+    if (val) memcpy(val, A->val.data(), A->val.size() * sizeof(double));
+    return NBX_OK;
+}
+
+int nbx_sparse_import_values(nb_sparse_t* A, const double* val) {
+    memcpy(A->val.data(), val, A->val.size() * sizeof(double));
+    stale(A);
+    return NBX_OK;
+}
+
+int nbx_sparse_is_blocked(const nb_sparse_t* A) {
+    if (sync_device(A)) return -1;
+    return A->dev.blocked ? 1 : 0;
+}
+
+int nbx_sparse_spmv_dev(const nb_sparse_t* A, const double* x_dev, double* y_dev) {
+    int rc = sync_device(A);
+    if (rc) return rc;
+    return nbx::spmv(A->dev, x_dev, y_dev, A->stream);
+}
+
+int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
+                     uint32_t max_iter, double inner_rtol, nbx_solve_info_t* info) {
+    nbx_solve_info_t tmp;
+    if (!info) info = &tmp;
+    memset(info, 0, sizeof *info);
+    if (A->N == 0) return NBX_OK;
+    int rc = sync_device(A);
+    if (rc) return rc;
+    double* db = A->ws.v64[3];
+    double* dx = A->ws.v64[4];
+    NBX_HIP(hipMemcpyAsync(db, b, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));
+    NBX_HIP(hipMemcpyAsync(dx, x, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));
+    nbx::SolveStats st;
+    int status = 0;
+    if (mode == NBX_SOLVE_REFERENCE) {
+        uint32_t it = 0;
+        double tr = 0;
+        status = nbx::pcg_jacobi(A->dev, db, dx, max_iter, tol, &it, &tr, A->ws, A->stream, &st);
+        if (status < 0) return -status;
+        info->iters = it;
+        info->fp64_iters = it;
+        info->tol_reached = tr;
+    } else if (mode == NBX_SOLVE_FP64) {
+        status = nbx::solve_fp64(A->dev, db, dx, tol, max_iter, A->ws, A->stream, &st);
+        if (status > 1) return status;
+        info->iters = st.iters;
+        info->fp64_iters = st.fp64_iters;
+        info->rel_res = st.rel_res;
+    } else if (mode == NBX_SOLVE_MIXED) {
+        if (!A->dev.has_val32) {
+            rc = nbx::mat_make_scaled32(A->dev, A->stream);
+            if (rc) return rc;
+        }
+        status = nbx::solve_mixed(A->dev, db, dx, tol, inner_rtol > 0 ? inner_rtol : 1e-4, max_iter,
+                                  A->ws, A->stream, &st);
+        if (status > 1) return status;
+        info->iters = st.iters;
+        info->outer = st.outer;
+        info->fp64_iters = st.fp64_iters;
+        info->rel_res = st.rel_res;
+        info->fallback = st.fallback;
+    } else {
+        nbx::set_last_error("unknown solve mode %d", mode);
+        return NBX_ERR_ARG;
+    }
+    info->ms = st.ms;
+    info->status = status;
+    NBX_HIP(hipMemcpyAsync(x, dx, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream));
+    NBX_HIP(hipStreamSynchronize(A->stream));
+    return status;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""ctypes bindings of libnbots_amd.so (declared in include/nbots_amd.h).
+
+The library is built in-tree by ``python -m nbots_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbots_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "nbots_amd.h")
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class _Opt:
+    def __init__(self, base):
+        self.base = base
+
+    def from_param(self, obj):
+        return None if obj is None else self.base.from_param(obj)
+
+
+u32o, u8o, f64o, i32o = _Opt(u32p), _Opt(u8p), _Opt(f64p), _Opt(i32p)
+U32, U64, F64, I32, VP = C.c_uint32, C.c_uint64, C.c_double, C.c_int, C.c_void_p
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("outer", C.c_uint32), ("fp64_iters", C.c_uint32),
+                ("status", C.c_int32), ("rel_res", C.c_double), ("tol_reached", C.c_double),
+                ("ms", C.c_double), ("fallback", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Graph(C.Structure):
+    """nb_graph_t (headers/nb/graph_bot/graph.h:25-31)."""
+    _fields_ = [("N", C.c_uint32), ("N_adj", C.POINTER(C.c_uint32)),
+                ("adj", C.POINTER(C.POINTER(C.c_uint32))),
+                ("wi", C.POINTER(C.c_double)), ("wij", C.POINTER(C.POINTER(C.c_double)))]
+
+
+_LIB = None
+
+# name -> (restype, argtypes)
+SIGS = {
+    "nb_graph_get_memsize": (U32, []),
+    "nb_graph_init": (None, [VP]),
+    "nb_graph_finish": (None, [VP]),
+    "nb_graph_clear": (None, [VP]),
+    "nb_sparse_create": (VP, [VP, u32o, U32]),
+    "nb_sparse_clone": (VP, [VP]),
+    "nb_sparse_save": (None, [VP, C.c_char_p]),
+    "nb_sparse_destroy": (None, [VP]),
+    "nb_sparse_reset": (None, [VP]),
+    "nb_sparse_set": (None, [VP, U32, U32, F64]),
+    "nb_sparse_set_identity_row": (None, [VP, U32]),
+    "nb_sparse_make_diagonal": (None, [VP, F64]),
+    "nb_sparse_get": (F64, [VP, U32, U32]),
+    "nb_sparse_get_and_set": (F64, [VP, U32, U32, F64]),
+    "nb_sparse_is_non_zero": (C.c_bool, [VP, U32, U32]),
+    "nb_sparse_add": (None, [VP, U32, U32, F64]),
+    "nb_sparse_scale": (None, [VP, F64]),
+    "nb_sparse_get_size": (U32, [VP]),
+    "nb_sparse_get_nnz": (U32, [VP]),
+    "nb_sparse_get_asym": (F64, [VP]),
+    "nb_sparse_get_frobenius_norm": (F64, [VP]),
+    "nb_sparse_multiply_scalar": (None, [VP, F64, U32]),
+    "nb_sparse_multiply_vector": (None, [VP, f64p, f64p, U32]),
+    "nb_sparse_set_Dirichlet_condition": (None, [VP, f64p, U32, F64]),
+    "nb_sparse_solve_CG_precond_Jacobi": (I32, [VP, f64p, f64p, U32, F64, C.POINTER(U32),
+                                                C.POINTER(F64), U32]),
+    "nb_mesh2D_create": (VP, [I32]),
+    "nb_mesh2D_destroy": (None, [VP]),
+    "nb_mesh2D_get_N_invtx": (U32, [VP]),
+    "nb_mesh2D_get_N_insgm": (U32, [VP]),
+    "nb_mesh2D_get_N_nodes": (U32, [VP]),
+    "nb_mesh2D_get_N_elems": (U32, [VP]),
+    "nb_mesh2D_node_get_x": (F64, [VP, U32]),
+    "nb_mesh2D_node_get_y": (F64, [VP, U32]),
+    "nb_mesh2D_elem_get_adj": (U32, [VP, U32, C.c_uint8]),
+    "nb_mesh2D_get_invtx": (U32, [VP, U32]),
+    "nb_mesh2D_insgm_get_N_nodes": (U32, [VP, U32]),
+    "nb_mesh2D_insgm_get_node": (U32, [VP, U32, U32]),
+    "nbx_mesh2D_set_trg": (I32, [VP, U32, f64p, U32, u32p, U32, u32p, U32, u32p, u32p]),
+    "nb_fem_elem_create": (VP, [I32]),
+    "nb_fem_elem_destroy": (None, [VP]),
+    "nb_fem_elem_get_N_gpoints": (C.c_uint8, [VP]),
+    "nb_fem_elem_get_N_nodes": (C.c_uint8, [VP]),
+    "nb_material_create": (VP, []),
+    "nb_material_destroy": (None, [VP]),
+    "nb_material_set_poisson_module": (None, [VP, F64]),
+    "nb_material_set_elasticity_module": (None, [VP, F64]),
+    "nb_material_set_density": (None, [VP, F64]),
+    "nb_material_get_poisson_module": (F64, [VP]),
+    "nb_material_get_elasticity_module": (F64, [VP]),
+    "nb_material_get_density": (F64, [VP]),
+    "nb_bcond_get_memsize": (C.c_uint16, [C.c_uint8]),
+    "nb_bcond_init": (None, [VP, C.c_uint8]),
+    "nb_bcond_finish": (None, [VP]),
+    "nb_bcond_create": (VP, [C.c_uint8]),
+    "nb_bcond_destroy": (None, [VP]),
+    "nb_bcond_clear": (None, [VP]),
+    "nb_bcond_get_N_dof": (C.c_uint8, [VP]),
+    "nb_bcond_push": (None, [VP, I32, I32, U32, C.POINTER(C.c_bool), C.POINTER(C.c_double)]),
+    "nb_bcond_push_function": (None, [VP, I32, I32, U32, C.POINTER(C.c_bool), VP]),
+    "nb_fem_compute_2D_Solid_Mechanics": (I32, [VP, VP, VP, VP, C.c_bool, f64o, I32, VP, VP,
+                                                f64p, f64p]),
+    "nb_fem_compute_stress_from_strain": (None, [U32, VP, VP, I32, f64p, VP, f64p]),
+    "nbx_last_error": (C.c_char_p, []),
+    "nbx_device_info": (I32, [C.c_char_p, U32, C.POINTER(U32), C.POINTER(F64)]),
+    "nbx_sparse_solve": (I32, [VP, f64p, f64p, I32, F64, U32, F64, C.POINTER(SolveInfo)]),
+    "nbx_sparse_spmv_dev": (I32, [VP, VP, VP]),
+    "nbx_sparse_is_blocked": (I32, [VP]),
+    "nbx_sparse_export_csr": (I32, [VP, u32o, u32o, f64o]),
+    "nbx_sparse_import_values": (I32, [VP, f64p]),
+    "nbx_fem_create": (I32, [C.POINTER(VP), U32, f64p, U32, u32p]),
+    "nbx_fem_destroy": (None, [VP]),
+    "nbx_fem_sizes": (I32, [VP, C.POINTER(U32), C.POINTER(U64), C.POINTER(U64)]),
+    "nbx_fem_assemble": (I32, [VP, F64, F64, F64, f64o, u8o, I32, F64, f64o]),
+    "nbx_fem_set_bconditions": (I32, [VP, u32p, u32p, u32p]
+                                + [U32, u32p, u8p, f64p] * 4 + [i32o, f64o, F64]),
+    "nbx_fem_solve": (I32, [VP, I32, F64, U32, F64, f64o, C.POINTER(SolveInfo)]),
+    "nbx_fem_get_displacement": (I32, [VP, f64p]),
+    "nbx_fem_get_system": (I32, [VP, u32p, u32p, f64p, f64p]),
+    "nbx_fem_strain_stress": (I32, [VP, f64p, f64p]),
+    "nbx_fem_compliance": (I32, [VP, f64o, F64, F64, F64, C.POINTER(F64), f64o, f64o]),
+    "nbx_fem_adjoint": (I32, [VP, f64p, f64p, I32, F64, U32, C.POINTER(SolveInfo)]),
+    "nbx_fem_spmv_bench": (I32, [VP, I32, U32, C.POINTER(F64)]),
+}
+
+
+def header_symbols():
+    """Function names declared in include/nbots_amd.h."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(nbx?_[A-Za-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def load(path: str = LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m nbots_amd.build` "
+                           "(the HIP library is required; there is no CPU fallback)")
+    L = C.CDLL(path)
+    for name, (res, args) in SIGS.items():
+        if not hasattr(L, name):
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def last_error() -> str:
+    e = load().nbx_last_error()
+    return e.decode() if e else ""
+
+
+def check(rc: int, what: str, ok=(0,)):
+    if rc not in ok:
+        raise RuntimeError(f"{what} failed with status {rc}: {last_error()}")
+    return rc

@@ -1,0 +1,188 @@
+"""Writes the golden fixtures in tests/golden/ from the compiled reference.
+
+Runs only where the reference checkout exists (it builds
+oracle/_ref/libnbref.so from the reference's own C sources).  Each fixture
+holds inputs and the reference's outputs; large arrays are stored as SHA-256
+digests of their little-endian bytes plus a few summary values.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from nbots_amd import synth  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+REF_TESTS = "/root/reference/utest/sources/nb/pde_bot/static_elasticity2D_inputs"
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def problem_arrays(p):
+    d = dict(nx=p.nx, ny=p.ny, nod=p.nod, conn=p.conn, vtx=p.vtx, sgm_ptr=p.sgm_ptr,
+             sgm_nodes=p.sgm_nodes, E=p.E, nu=p.nu, thickness=p.thickness,
+             elem_E=np.zeros(0) if p.elem_E is None else p.elem_E)
+    for k in ("dv", "nv", "ds", "ns"):
+        ids, m, v = getattr(p, k)
+        d[k + "_id"], d[k + "_mask"], d[k + "_val"] = ids, m, v
+    return d
+
+
+def full_case(p, name, store_full=True, keep_x=True):
+    """Pattern, K/F before and after BCs, the reference driver's solve, and a
+    PCG run cut at 7 iterations (exercises the lagged stop test)."""
+    out = problem_arrays(p)
+    rp, col, val0, F0, st = oracle.ref_assemble(p, apply_bc=False)
+    assert st == 0
+    _, _, val, F, _ = oracle.ref_assemble(p, apply_bc=True)
+    tol = 1e-8 if p.elem_E is None else 1e-8 * float(np.linalg.norm(F))
+    x, it, tr, cst = oracle.ref_pcg(rp, col, val, F, tol=tol)
+    x7, it7, tr7, st7 = oracle.ref_pcg(rp, col, val, F, max_iter=7, tol=tol)
+    y = oracle.ref_spmv(rp, col, val, np.sin(np.arange(p.N, dtype=np.float64)))
+    out.update(pcg_tol=tol, niter=it, tol_reached=tr, pcg_status=cst,
+               niter7=it7, tol_reached7=tr7, status7=st7)
+    arrays = dict(rowptr=rp, col=col, val_nobc=val0, F_nobc=F0, val=val, F=F, x=x,
+                  x7=x7, spmv_sin=y)
+    if p.elem_E is None:
+        st2, disp, eps, sig = oracle.ref_fem_compute(p)
+        assert st2 == 0
+        arrays.update(disp=disp, strain=eps, stress=sig)
+    for k, a in arrays.items():
+        if store_full:
+            out[k] = a
+        else:
+            out[k + "_sha"] = sha(a)
+            out[k + "_absmax"] = float(np.abs(a).max()) if a.size else 0.0
+    if not store_full and keep_x:
+        out["x"] = x          # keep the solution itself for tolerance checks
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(f"{name}: N={p.N} nnz={col.size} niter={it} tol_reached={tr:.3e}")
+
+
+def tokens(path):
+    toks = []
+    with open(path) as f:
+        for line in f:
+            line = line.split("#")[0]
+            toks += line.split()
+    return toks
+
+
+def beam_case(fname, name, N_vtx=1000):
+    """The reference's own FEM unit-test problem (disabled suite), meshed by
+    the reference tessellator; the mesh, BCs and results become a fixture
+    that the oracle and the GPU path replay."""
+    R = oracle.ref()
+    sizes = np.zeros(5, np.uint32)
+    st = R.ref_testcase_run(os.path.join(REF_TESTS, fname).encode(), N_vtx, sizes)
+    assert st == 0, st
+    nn, ne, nv, ns, tot = (int(s) for s in sizes)
+    nod = np.zeros(2 * nn)
+    conn = np.zeros(3 * ne, np.uint32)
+    vtx = np.zeros(nv, np.uint32)
+    sp = np.zeros(ns + 1, np.uint32)
+    sn = np.zeros(tot, np.uint32)
+    disp = np.zeros(2 * nn)
+    eps = np.zeros(3 * ne)
+    sig = np.zeros(3 * ne)
+    mat = np.zeros(3)
+    R.ref_testcase_fetch(nod, conn, vtx, sp, sn, disp, eps, sig, mat)
+    R.ref_testcase_free()
+    # boundary conditions as the reference reads them (bcond_read.c:12-62)
+    t = tokens(os.path.join(REF_TESTS, fname))
+    pos = 0
+
+    def nxt():
+        nonlocal pos
+        pos += 1
+        return t[pos - 1]
+    nverts = int(nxt())
+    pos += 2 * nverts
+    nsegs = int(nxt())
+    pos += 2 * nsegs
+    nholes = int(nxt())
+    pos += 2 * nholes
+    lists = []
+    for _ in range(4):
+        n = int(nxt())
+        ids, masks, vals = [], [], []
+        for _ in range(n):
+            ids.append(int(nxt()))
+            m = [int(nxt()), int(nxt())]
+            v = [float(nxt()) if m[k] else 0.0 for k in range(2)]
+            masks += m
+            vals += v
+        lists.append((np.array(ids, np.uint32), np.array(masks, np.uint8),
+                      np.array(vals, np.float64)))
+    dv, nvl, ds, nsl = lists
+    out = dict(nod=nod, conn=conn, vtx=vtx, sgm_ptr=sp, sgm_nodes=sn, E=mat[0], nu=mat[1],
+               thickness=mat[2], disp=disp, strain=eps, stress=sig,
+               max_disp=float(np.sqrt((disp.reshape(-1, 2) ** 2).sum(1).max())))
+    for k, (ids, m, v) in zip(("dv", "nv", "ds", "ns"), (dv, nvl, ds, nsl)):
+        out[k + "_id"], out[k + "_mask"], out[k + "_val"] = ids, m, v
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+    print(f"{name}: nodes={nn} elems={ne} max|u|={out['max_disp']:.9e}")
+
+
+def degenerate_cases():
+    """Edge cases of the solver: exact one-step convergence followed by the
+    lagged extra iteration (0/0), an empty system, and a zero right-hand
+    side."""
+    cases = {}
+    rp = np.array([0, 1], np.uint32)
+    col = np.array([0], np.uint32)
+    val = np.array([4.0])
+    x, it, tr, st = oracle.ref_pcg(rp, col, val, np.array([2.0]), max_iter=10, tol=1e-8)
+    cases["one_by_one"] = dict(x=x.tolist(), niter=it, tol_reached=tr, status=st)
+    rp2 = np.array([0, 2, 4], np.uint32)
+    col2 = np.array([0, 1, 0, 1], np.uint32)
+    val2 = np.array([4.0, 1.0, 1.0, 3.0])
+    x, it, tr, st = oracle.ref_pcg(rp2, col2, val2, np.array([1.0, 2.0]), max_iter=1, tol=1e-12)
+    cases["two_by_two_max1"] = dict(x=x.tolist(), niter=it, tol_reached=tr, status=st)
+    x, it, tr, st = oracle.ref_pcg(rp2, col2, val2, np.zeros(2), max_iter=5, tol=1e-8)
+    cases["zero_rhs"] = dict(x=x.tolist(), niter=it, tol_reached=tr, status=st)
+    with open(os.path.join(OUT, "pcg_edge_cases.json"), "w") as f:
+        json.dump(cases, f, indent=1, default=lambda v: None if v != v else v)
+    print("pcg_edge_cases:", cases)
+
+
+def synth_pin():
+    pin = dict(
+        seed=synth.DEFAULT_SEED,
+        splitmix_s1=[int(v) for v in synth.splitmix64(synth.DEFAULT_SEED, 1, 4)],
+        uniform_s2=synth.uniform(synth.DEFAULT_SEED, 2, 4).tolist(),
+        normal_s2=synth.normal(synth.DEFAULT_SEED, 2, 4).tolist(),
+    )
+    with open(os.path.join(OUT, "synth_pin.json"), "w") as f:
+        json.dump(pin, f, indent=1)
+
+
+def main():
+    oracle.build(ref=True)
+    synth_pin()
+    full_case(synth.cantilever_point(2, 1, jitter=0.0), "cant_2x1")
+    full_case(synth.cantilever_point(4, 2, jitter=0.1), "cant_4x2_jit")
+    full_case(synth.cantilever_point(8, 4, jitter=0.1), "cant_8x4_jit")
+    full_case(synth.cantilever_traction(16, 8), "traction_16x8")
+    full_case(synth.mbb(16, 8), "mbb_16x8")
+    full_case(synth.cantilever_point(64, 32, jitter=0.1), "c1_cantilever_64x32", store_full=False)
+    full_case(synth.cantilever_traction(128, 64), "traction_128x64", store_full=False,
+              keep_x=False)
+    beam_case("beam_cantilever.txt", "beam_cantilever")
+    degenerate_cases()
+
+
+if __name__ == "__main__":
+    main()
