@@ -227,6 +227,7 @@ typedef struct {
 	double ms;            /* device time of the solve                   */
 	int32_t fallback;     /* MIXED sweeps stalled, FP64 PCG finished    */
 	int32_t pad;
+	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 
 const char *nbx_last_error(void);

@@ -1,0 +1,71 @@
+// Device-resident T3 plane-stress system (host-side view).
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+
+namespace nbx {
+
+struct ElemParams {
+    double E = 1.0, nu = 0.3, thickness = 1.0;
+    const double* elemE = nullptr;     // device, per-element modulus (nullable)
+    const uint8_t* enabled = nullptr;  // device, element switch (nullable)
+};
+
+// Boundary conditions reduced to what the device applies, in the
+// reference's order (set_bconditions.c:52-61):
+//  * Neumann: for each touched dof, its additions in order;
+//  * Dirichlet: for each constrained dof, the rank of its first
+//    elimination and its first and last prescribed values.
+struct BcPlan {
+    std::vector<uint32_t> n_dof, n_ptr;
+    std::vector<double> n_add;
+    std::vector<uint32_t> d_dof, d_rank;
+    std::vector<double> d_vfirst, d_vlast;
+};
+
+struct FemSys {
+    uint32_t nn = 0, ne = 0;
+    size_t nnz = 0;                    // scalar nonzeros of the pattern
+    hipStream_t s = nullptr;
+    unsigned* err = nullptr;
+    double2* nod = nullptr;            // [nn]
+    uint32_t* conn = nullptr;          // [3 ne]
+    uint32_t* n2e_ptr = nullptr;       // [nn + 1]
+    uint32_t* n2e = nullptr;           // [3 ne], ascending per node
+    DevMatrix K;
+    double* F = nullptr;               // [2 nn]
+    double* u = nullptr;               // [2 nn]
+    double* aux = nullptr;             // [2 nn] scratch for adjoint / bench
+    double* aux2 = nullptr;
+    uint32_t* bc_rank = nullptr;       // [2 nn], 0xffffffff = free
+    double* bc_vf = nullptr;
+    double* bc_vl = nullptr;
+    double* elemE = nullptr;
+    uint8_t* enabled = nullptr;
+    double* rho = nullptr;
+    double* dc = nullptr;
+    double* ce = nullptr;
+    ElemParams P;
+    bool bc_applied = false;
+    uint32_t n_constrained = 0;
+    Workspace ws;
+};
+
+int fem_create(FemSys& F, uint32_t nn, const double* nod, uint32_t ne, const uint32_t* conn);
+void fem_free(FemSys& F);
+int fem_assemble(FemSys& F, double E, double nu, double thickness, const double* elemE,
+                 const uint8_t* enabled, int self_weight, double density, const double* gravity);
+int fem_assemble_device(FemSys& F);
+int fem_body_force(FemSys& F, int self_weight, double density, const double* gravity);
+int fem_apply_bc(FemSys& F, const BcPlan& plan);
+int fem_zero_constrained(FemSys& F, double* x);
+int fem_strain_stress(FemSys& F, double* strain, double* stress);
+int fem_compliance(FemSys& F, const double* rho, double penal, double E0, double Emin, double* ce,
+                   double* dc, double* C_host);
+int stress_from_strain(uint32_t ne, double E, double nu, const uint8_t* enabled_host,
+                       const double* strain_host, double* stress_host);
+int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* val);
+
+}  // namespace nbx

@@ -44,6 +44,8 @@ struct Workspace {
     PcgState* st = nullptr;    // device state (2 slots: outer, inner)
     PcgState* h_st = nullptr;  // pinned host mirror
     hipEvent_t ev[4] = {};
+    static constexpr int NSAMP = 8;     // sampled SpMV launches per solve
+    hipEvent_t sev[2 * NSAMP] = {};
 };
 
 struct SolveStats {
@@ -84,6 +86,9 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol,
 // FP64 Jacobi PCG with a relative tolerance (x0 = x on input).
 int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol,
                uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats);
+
+int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStream_t s,
+               double* ms_avg);
 
 double device_norm2(uint32_t n, const double* v, Workspace& w, hipStream_t s, int* err);
 

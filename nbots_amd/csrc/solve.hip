@@ -344,6 +344,12 @@ __device__ __forceinline__ float2 row32(const Sell2& A, uint32_t v, const float2
     return make_float2(s0, s1);
 }
 
+__global__ __launch_bounds__(BLOCK) void k_spmv32(Sell2 A, const float2* __restrict__ x,
+                                                  float2* __restrict__ y, uint32_t n) {
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) y[i] = row32(A, (uint32_t)i, x);
+}
+
 // Outer step: r = b - K u (FP64), ||r||^2; r32 = fl32(dis * r); y32 = 0;
 // p32 = r32; inner rz = r32.r32.  st[0] = outer state, st[1] = inner.
 __global__ __launch_bounds__(BLOCK) void k_ir_residual(OpS2 op, const double2* __restrict__ b,
@@ -515,6 +521,7 @@ int ws_alloc(Workspace& w, uint32_t n) {
     NBX_HIP(hipMalloc(&w.st, 2 * sizeof(PcgState)));
     NBX_HIP(hipHostMalloc(&w.h_st, 2 * sizeof(PcgState), hipHostMallocDefault));
     for (auto& e : w.ev) NBX_HIP(hipEventCreate(&e));
+    for (auto& e : w.sev) NBX_HIP(hipEventCreate(&e));
     w.n = n;
     return NBX_OK;
 }
@@ -534,6 +541,10 @@ void ws_free(Workspace& w) {
     if (w.st) (void)hipFree(w.st);
     if (w.h_st) (void)hipHostFree(w.h_st);
     for (auto& e : w.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    for (auto& e : w.sev) {
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
@@ -713,6 +724,41 @@ int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s) {
     return NBX_OK;
 }
 
+// Average duration of `reps` back-to-back SpMV launches (ms), FP64 or the
+// scaled FP32 copy, on x = 1.
+int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStream_t s,
+               double* ms_avg) {
+    if (!m.blocked || m.n == 0) {
+        set_last_error("spmv_bench needs a 2-dof node matrix");
+        return NBX_ERR_ARG;
+    }
+    NBX_TRY(ws_alloc(w, m.n));
+    const uint32_t nu = m.s.n_nodes;
+    std::vector<double> ones(m.n, 1.0);
+    std::vector<float> onesf(m.n, 1.0f);
+    NBX_HIP(hipMemcpyAsync(w.v64[0], ones.data(), m.n * 8, hipMemcpyHostToDevice, s));
+    NBX_HIP(hipMemcpyAsync(w.v32[0], onesf.data(), m.n * 4, hipMemcpyHostToDevice, s));
+    const unsigned G = map_grid(nu);
+    auto launch = [&]() {
+        if (prec == 32)
+            hipLaunchKernelGGL(k_spmv32, dim3(G), dim3(BLOCK), 0, s, m.s, (const float2*)w.v32[0],
+                               (float2*)w.v32[1], nu);
+        else
+            hipLaunchKernelGGL(k_spmv<OpS2>, dim3(G), dim3(BLOCK), 0, s, OpS2{m.s},
+                               (const double2*)w.v64[0], (double2*)w.v64[1], nu);
+    };
+    launch();   // warm-up
+    NBX_HIP(hipEventRecord(w.ev[2], s));
+    for (uint32_t r = 0; r < reps; ++r) launch();
+    NBX_HIP(hipEventRecord(w.ev[3], s));
+    NBX_HIP(hipEventSynchronize(w.ev[3]));
+    NBX_HIP(hipGetLastError());
+    float ms = 0.f;
+    NBX_HIP(hipEventElapsedTime(&ms, w.ev[2], w.ev[3]));
+    *ms_avg = reps ? ms / reps : 0.0;
+    return NBX_OK;
+}
+
 double device_norm2(uint32_t n, const double* v, Workspace& w, hipStream_t s, int* err) {
     *err = NBX_OK;
     double* out = w.v64[5] + w.n;   // spare slot past the vector end
@@ -752,11 +798,14 @@ static int pcg_run(const Op& op, uint32_t nu, const DevMatrix& m, const typename
     hipLaunchKernelGGL(k_pcg_init<Op>, dim3(G), dim3(BLOCK), 0, s, op, b, x, g, p, diag, w.st,
                        w.partials, w.ticket, nu);
     NBX_HIP(hipGetLastError());
-    uint32_t launched = 0, batch = 4;
+    uint32_t launched = 0, batch = 4, nsamp = 0;
     for (;;) {
         for (uint32_t i = 0; i < batch; ++i) {
+            const bool samp = launched == 0 && nsamp < (uint32_t)Workspace::NSAMP;
+            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, p, wv, w.st, w.partials,
                                w.ticket, nu);
+            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
             hipLaunchKernelGGL(k_pcg_B<V>, dim3(G), dim3(BLOCK), 0, s, x, g, p, wv, diag, w.st,
                                w.partials, w.ticket, nu);
             hipLaunchKernelGGL(k_pcg_C<V>, dim3(G), dim3(BLOCK), 0, s, p, g, diag, w.st, nu);
@@ -779,6 +828,16 @@ static int pcg_run(const Op& op, uint32_t nu, const DevMatrix& m, const typename
         stats->ms = ms;
         stats->iters = w.h_st[0].k;
         stats->fp64_iters = w.h_st[0].k;
+        // only samples of iterations that ran (k > sample index) count
+        double tot = 0.0;
+        uint32_t cnt = 0;
+        for (uint32_t i = 0; i < nsamp && i < w.h_st[0].k; ++i) {
+            float t = 0.f;
+            NBX_HIP(hipEventElapsedTime(&t, w.sev[2 * i], w.sev[2 * i + 1]));
+            tot += t;
+            ++cnt;
+        }
+        stats->ms_spmv_avg = cnt ? tot / cnt : 0.0;
     }
     return NBX_OK;
 }
@@ -842,7 +901,7 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, dou
     float2* y32 = (float2*)w.v32[3];
     const unsigned G = red_grid(nu);
     OpS2 op{m.s};
-    uint32_t inner_total = 0, outer = 0;
+    uint32_t inner_total = 0, outer = 0, nsamp = 0, samp_valid = 0;
     double prev_res = -1.0;
     int stalled = 0;
     NBX_HIP(hipEventRecord(w.ev[0], s));
@@ -862,16 +921,22 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, dou
         }
         prev_res = res;
         uint32_t batch = 8;
+        bool first_batch = true;
         for (;;) {
             for (uint32_t i = 0; i < batch; ++i) {
+                const bool samp = outer == 0 && first_batch && nsamp < (uint32_t)Workspace::NSAMP;
+                if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
                 hipLaunchKernelGGL(k_in_A, dim3(G), dim3(BLOCK), 0, s, m.s, p32, w32, w.st,
                                    w.partials32, w.ticket, nu);
+                if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
                 hipLaunchKernelGGL(k_in_B, dim3(G), dim3(BLOCK), 0, s, y32, r32, p32, w32, w.st,
                                    w.partials32, w.ticket, nu);
                 hipLaunchKernelGGL(k_in_C, dim3(G), dim3(BLOCK), 0, s, p32, r32, w.st, nu);
             }
             NBX_HIP(hipGetLastError());
             NBX_TRY(poll_state(w, s));
+            if (outer == 0 && first_batch) samp_valid = w.h_st[1].k;
+            first_batch = false;
             if (w.h_st[1].done) break;
             batch = batch < 64 ? batch * 2 : 64;
         }
@@ -900,6 +965,15 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, dou
         ms += (float)fb.ms;
     }
     if (stats) {
+        double tot = 0.0;
+        uint32_t cnt = 0;
+        for (uint32_t i = 0; i < nsamp && i < samp_valid; ++i) {
+            float t = 0.f;
+            NBX_HIP(hipEventElapsedTime(&t, w.sev[2 * i], w.sev[2 * i + 1]));
+            tot += t;
+            ++cnt;
+        }
+        stats->ms_spmv_avg = cnt ? tot / cnt : 0.0;
         stats->iters = inner_total;
         stats->outer = outer;
         stats->fp64_iters = fb.fp64_iters;

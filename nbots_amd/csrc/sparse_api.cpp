@@ -369,6 +369,7 @@ int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode,
         return NBX_ERR_ARG;
     }
     info->ms = st.ms;
+    info->ms_spmv = st.ms_spmv_avg;
     info->status = status;
     NBX_HIP(hipMemcpyAsync(x, dx, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream));
     NBX_HIP(hipStreamSynchronize(A->stream));

@@ -1,0 +1,179 @@
+"""Python view of the device-resident FEM system (nbx_fem_* in
+include/nbots_amd.h) and of the drop-in driver nb_fem_compute_2D_Solid_Mechanics."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .lib import SolveInfo, check, last_error, load
+
+SOLVE_REFERENCE, SOLVE_FP64, SOLVE_MIXED = 0, 1, 2
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Fem:
+    """A T3 plane-stress system held on the GPU."""
+
+    def __init__(self, nod, conn):
+        self.L = load()
+        nod = _c(nod, np.float64)
+        conn = _c(conn, np.uint32)
+        h = C.c_void_p()
+        check(self.L.nbx_fem_create(C.byref(h), nod.size // 2, nod, conn.size // 3, conn),
+              "nbx_fem_create")
+        self.h = h.value
+        self.N_nod = nod.size // 2
+        self.N_elem = conn.size // 3
+
+    @classmethod
+    def from_problem(cls, p, assemble=True, bc=True):
+        f = cls(p.nod, p.conn)
+        if assemble:
+            f.assemble(p.E, p.nu, p.thickness, p.elem_E)
+        if bc:
+            f.set_bconditions(p)
+        return f
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            self.L.nbx_fem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def N(self):
+        return 2 * self.N_nod
+
+    def sizes(self):
+        n, nnz, blk = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        self.L.nbx_fem_sizes(self.h, C.byref(n), C.byref(nnz), C.byref(blk))
+        return n.value, nnz.value, blk.value
+
+    def assemble(self, E=1.0, nu=0.3, thickness=1.0, elem_E=None, enabled=None,
+                 self_weight=False, density=0.0, gravity=None):
+        rc = self.L.nbx_fem_assemble(self.h, E, nu, thickness,
+                                     None if elem_E is None else _c(elem_E, np.float64),
+                                     None if enabled is None else _c(enabled, np.uint8),
+                                     int(self_weight), density,
+                                     None if gravity is None else _c(gravity, np.float64))
+        return check(rc, "nbx_fem_assemble", ok=(0, 2))
+
+    def set_bconditions(self, p, factor=1.0, ns_fptr=None, ns_fval=None):
+        args = []
+        for k in ("dv", "nv", "ds", "ns"):
+            ids, m, v = getattr(p, k)
+            args += [ids.size, _c(ids, np.uint32), _c(m, np.uint8), _c(v, np.float64)]
+        rc = self.L.nbx_fem_set_bconditions(
+            self.h, _c(p.vtx, np.uint32), _c(p.sgm_ptr, np.uint32), _c(p.sgm_nodes, np.uint32),
+            *args, None if ns_fptr is None else _c(ns_fptr, np.int32),
+            None if ns_fval is None else _c(ns_fval, np.float64), factor)
+        check(rc, "nbx_fem_set_bconditions")
+
+    def solve(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=None, inner_rtol=1e-4, u0=None):
+        info = SolveInfo()
+        rc = self.L.nbx_fem_solve(self.h, mode, tol, 20 * self.N if max_iter is None else max_iter,
+                                  inner_rtol, None if u0 is None else _c(u0, np.float64),
+                                  C.byref(info))
+        check(rc, "nbx_fem_solve", ok=(0, 1))
+        return info.as_dict()
+
+    def displacement(self):
+        u = np.zeros(self.N)
+        check(self.L.nbx_fem_get_displacement(self.h, u), "nbx_fem_get_displacement")
+        return u
+
+    def system(self):
+        """K (scalar CSR, the reference's layout) and F."""
+        _, nnz, _ = self.sizes()
+        rp = np.zeros(self.N + 1, np.uint32)
+        col = np.zeros(nnz, np.uint32)
+        val = np.zeros(nnz)
+        F = np.zeros(self.N)
+        check(self.L.nbx_fem_get_system(self.h, rp, col, val, F), "nbx_fem_get_system")
+        return rp, col, val, F
+
+    def strain_stress(self):
+        e = np.zeros(3 * self.N_elem)
+        s = np.zeros(3 * self.N_elem)
+        check(self.L.nbx_fem_strain_stress(self.h, e, s), "nbx_fem_strain_stress")
+        return e, s
+
+    def compliance(self, rho=None, penal=3.0, E0=1.0, Emin=1e-9):
+        Cv = C.c_double()
+        dc = np.zeros(self.N_elem)
+        ce = np.zeros(self.N_elem)
+        check(self.L.nbx_fem_compliance(self.h, None if rho is None else _c(rho, np.float64),
+                                        penal, E0, Emin, C.byref(Cv), dc, ce),
+              "nbx_fem_compliance")
+        return Cv.value, dc, ce
+
+    def adjoint(self, rhs, mode=SOLVE_FP64, rtol=1e-10, max_iter=None):
+        lam = np.zeros(self.N)
+        info = SolveInfo()
+        rc = self.L.nbx_fem_adjoint(self.h, _c(rhs, np.float64), lam, mode, rtol,
+                                    20 * self.N if max_iter is None else max_iter, C.byref(info))
+        check(rc, "nbx_fem_adjoint", ok=(0, 1))
+        return lam, info.as_dict()
+
+    def spmv_ms(self, precision=64, reps=20):
+        ms = C.c_double()
+        check(self.L.nbx_fem_spmv_bench(self.h, precision, reps, C.byref(ms)), "nbx_fem_spmv_bench")
+        return ms.value
+
+
+def compute_2D_solid_mechanics(p, enabled=None, self_weight=False, gravity=(0.0, 0.0),
+                               density=0.0):
+    """Run the drop-in nb_fem_compute_2D_Solid_Mechanics on a Problem through
+    the reference's object API (mesh, element, material, bcond)."""
+    L = load()
+    mesh = L.nb_mesh2D_create(0)
+    check(L.nbx_mesh2D_set_trg(mesh, p.N_nod, _c(p.nod, np.float64), p.N_elem,
+                               _c(p.conn, np.uint32), p.vtx.size, _c(p.vtx, np.uint32),
+                               p.sgm_ptr.size - 1, _c(p.sgm_ptr, np.uint32),
+                               _c(p.sgm_nodes, np.uint32)), "nbx_mesh2D_set_trg")
+    elem = L.nb_fem_elem_create(0)
+    mat = L.nb_material_create()
+    L.nb_material_set_elasticity_module(mat, p.E)
+    L.nb_material_set_poisson_module(mat, p.nu)
+    L.nb_material_set_density(mat, density)
+    bc = L.nb_bcond_create(2)
+    for k, (t, w) in (("dv", (0, 0)), ("nv", (1, 0)), ("ds", (0, 1)), ("ns", (1, 1))):
+        ids, m, v = getattr(p, k)
+        for c in range(ids.size):
+            mask = (C.c_bool * 2)(bool(m[2 * c]), bool(m[2 * c + 1]))
+            val = (C.c_double * 2)(v[2 * c], v[2 * c + 1])
+            L.nb_bcond_push(bc, t, w, int(ids[c]), mask, val)
+
+    class Params(C.Structure):
+        _fields_ = [("thickness", C.c_double), ("revolution_axe", C.c_double),
+                    ("revolution_coordinate", C.c_char)]
+    params = Params(p.thickness, 0.0, b"\0")
+    en = None
+    if enabled is not None:
+        en = (C.c_bool * p.N_elem)(*[bool(x) for x in enabled])
+    disp = np.zeros(p.N)
+    strain = np.zeros(3 * p.N_elem)
+    en_p = None if en is None else C.cast(en, C.c_void_p)
+    st = L.nb_fem_compute_2D_Solid_Mechanics(mesh, elem, mat, bc, bool(self_weight),
+                                             _c(gravity, np.float64), 0,
+                                             C.cast(C.pointer(params), C.c_void_p),
+                                             en_p, disp, strain)
+    stress = np.zeros(3 * p.N_elem)
+    if st == 0:
+        L.nb_fem_compute_stress_from_strain(p.N_elem, elem, mat, 0, strain, en_p, stress)
+    L.nb_bcond_destroy(bc)
+    L.nb_material_destroy(mat)
+    L.nb_fem_elem_destroy(elem)
+    L.nb_mesh2D_destroy(mesh)
+    if st < 0:
+        raise RuntimeError(f"nb_fem_compute_2D_Solid_Mechanics: {last_error()}")
+    return st, disp, strain, stress

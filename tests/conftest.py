@@ -29,7 +29,7 @@ def load_json(name):
 def problem_from_golden(g):
     """Rebuild an nbots_amd.synth.Problem from a golden fixture's inputs."""
     from nbots_amd.synth import Problem
-    elem_E = g["elem_E"] if g["elem_E"].size else None
+    elem_E = g["elem_E"] if "elem_E" in g and g["elem_E"].size else None
     p = Problem(int(g.get("nx", 0)), int(g.get("ny", 0)), g["nod"], g["conn"], g["vtx"],
                 g["sgm_ptr"], g["sgm_nodes"], E=float(g["E"]), nu=float(g["nu"]),
                 thickness=float(g["thickness"]), elem_E=elem_E)

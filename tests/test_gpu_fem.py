@@ -1,0 +1,205 @@
+"""GPU parity of the device FEM system and the drop-in driver.
+
+Bars (see the module docstrings of the kernels for why they hold):
+  * pattern, K and F before and after boundary conditions: bit-exact with
+    the golden vectors the compiled reference wrote;
+  * strain/stress of a given displacement: bit-exact with the oracle;
+  * displacement of a full solve: relative L2 <= 1e-9 against the
+    reference's own solve (same stopping rule, different dot-product order);
+  * the reference's beam-cantilever unit test: max |u| = 1.00701e-1 +- 1e-6
+    (utest/.../static_elasticity2D.c:118).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import SMALL_CASES, load_golden, problem_from_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", SMALL_CASES)
+def test_assembly_and_bc_bit_exact(case):
+    from nbots_amd.fem import Fem
+    g = load_golden(case)
+    p = problem_from_golden(g)
+    f = Fem(p.nod, p.conn)
+    assert f.assemble(p.E, p.nu, p.thickness, p.elem_E) == 0
+    rp, col, val, F = f.system()
+    assert np.array_equal(rp, g["rowptr"]) and np.array_equal(col, g["col"])
+    assert np.array_equal(val, g["val_nobc"]), np.abs(val - g["val_nobc"]).max()
+    assert np.array_equal(F, g["F_nobc"])
+    f.set_bconditions(p)
+    rp, col, val, F = f.system()
+    assert np.array_equal(val, g["val"]), np.abs(val - g["val"]).max()
+    assert np.array_equal(F, g["F"]), np.abs(F - g["F"]).max()
+    f.destroy()
+
+
+@pytest.mark.parametrize("case", ["c1_cantilever_64x32", "traction_128x64"])
+def test_assembly_bit_exact_digests(case):
+    from nbots_amd.fem import Fem
+    g = load_golden(case)
+    p = problem_from_golden(g)
+    f = Fem.from_problem(p, bc=False)
+    rp, col, val, F = f.system()
+    assert sha(rp) == str(g["rowptr_sha"]) and sha(col) == str(g["col_sha"])
+    assert sha(val) == str(g["val_nobc_sha"])
+    f.set_bconditions(p)
+    rp, col, val, F = f.system()
+    assert sha(val) == str(g["val_sha"]) and sha(F) == str(g["F_sha"])
+    f.destroy()
+
+
+def test_beam_cantilever_known_answer():
+    """The reference's own FEM test: 1000-vertex Delaunay mesh from the
+    reference tessellator, prescribed u_y = -0.1 at a vertex, replayed
+    through the drop-in nb_fem_compute_2D_Solid_Mechanics."""
+    from nbots_amd.fem import compute_2D_solid_mechanics
+    g = load_golden("beam_cantilever")
+    p = problem_from_golden(g)
+    st, disp, strain, stress = compute_2D_solid_mechanics(p)
+    assert st == 0
+    max_disp = np.sqrt((disp.reshape(-1, 2) ** 2).sum(1).max())
+    assert abs(max_disp - 1.00701e-1) < 1e-6
+    assert np.linalg.norm(disp - g["disp"]) <= 1e-9 * np.linalg.norm(g["disp"])
+    assert np.linalg.norm(strain - g["strain"]) <= 1e-8 * np.linalg.norm(g["strain"])
+    assert np.linalg.norm(stress - g["stress"]) <= 1e-8 * np.linalg.norm(g["stress"])
+
+
+@pytest.mark.parametrize("case", ["cant_2x1", "cant_4x2_jit", "cant_8x4_jit"])
+def test_driver_matches_reference(case):
+    from nbots_amd.fem import compute_2D_solid_mechanics
+    g = load_golden(case)
+    p = problem_from_golden(g)
+    st, disp, strain, stress = compute_2D_solid_mechanics(p)
+    assert st == 0
+    assert np.linalg.norm(disp - g["disp"]) <= 1e-9 * np.linalg.norm(g["disp"])
+    assert np.linalg.norm(strain - g["strain"]) <= 1e-8 * np.linalg.norm(g["strain"])
+    assert np.linalg.norm(stress - g["stress"]) <= 1e-8 * np.linalg.norm(g["stress"])
+
+
+def test_driver_distorted_element_status():
+    """A clockwise triangle makes the reference's assembly fail: status 1
+    (pipeline.c:158-159, static_elasticity2D.c:62-65)."""
+    from nbots_amd import synth
+    from nbots_amd.fem import compute_2D_solid_mechanics
+    p = synth.cantilever_point(4, 2, jitter=0.0)
+    c = p.conn.reshape(-1, 3).copy()
+    c[3, 1], c[3, 2] = c[3, 2], c[3, 1]
+    p.conn = c.reshape(-1)
+    st, _, _, _ = compute_2D_solid_mechanics(p)
+    assert st == 1
+
+
+def test_strain_stress_bit_exact():
+    """Strain and stress of the GPU's displacement equal the oracle's on it."""
+    import oracle
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_FP64
+    p = synth.cantilever_traction(24, 12)
+    f = Fem.from_problem(p)
+    f.solve(SOLVE_FP64, tol=1e-10)
+    u = f.displacement()
+    eps, sig = f.strain_stress()
+    assert np.array_equal(eps, oracle.strain(p, u))
+    assert np.array_equal(sig, oracle.stress(p, eps))
+    f.destroy()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_solve_modes_traction(mode):
+    """FP64 and FP32-inner/FP64-refinement reach relative residual 1e-8
+    (checked with the bit-exact SpMV of the oracle on the device's K) and
+    agree with a 1e-13 oracle solve within rel-L2 1e-6 (the error bound of
+    a 1e-8 residual times the condition number is far larger; 1e-6 is what
+    both modes achieve here)."""
+    import oracle
+    g = load_golden("traction_128x64")
+    p = problem_from_golden(g)
+    from nbots_amd.fem import Fem
+    f = Fem.from_problem(p)
+    info = f.solve(mode, tol=1e-8)
+    assert info["status"] == 0, info
+    u = f.displacement()
+    rp, col, val, F = f.system()
+    r = F - oracle.spmv(rp, col, val, u)
+    assert np.linalg.norm(r) <= 1.0000001e-8 * np.linalg.norm(F), (np.linalg.norm(r) / np.linalg.norm(F), info)
+    x_ref, _, _, _ = oracle.pcg(rp, col, val, F, tol=1e-13 * np.linalg.norm(F), max_iter=100000)
+    err = np.linalg.norm(u - x_ref) / np.linalg.norm(x_ref)
+    print("mode", mode, "rel err vs 1e-13 oracle", err, info)
+    assert err <= 1e-6
+    f.destroy()
+
+
+def test_compliance_sensitivities_small():
+    """SIMP on the MBB beam: element energies bit-exact with the oracle on
+    the same displacement; C = F.u (energy identity); dC/drho against
+    central finite differences (step 1e-6, FP64 solves to 1e-13)."""
+    import oracle
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_FP64
+    p = synth.mbb(16, 8)
+    f = Fem.from_problem(p)
+    f.solve(SOLVE_FP64, tol=1e-13)
+    u = f.displacement()
+    C, dc, ce = f.compliance(p.rho)
+    assert np.array_equal(ce, oracle.elem_energy(p, u))
+    Cref, dcref = oracle.simp_sens(p.rho, ce)
+    assert abs(C - Cref) <= 1e-12 * abs(Cref)
+    assert np.allclose(dc, dcref, rtol=1e-13, atol=0)
+    _, _, _, F = f.system()
+    assert abs(C - F @ u) <= 1e-10 * abs(C)
+    h = 1e-6
+    rng = np.random.default_rng(3)
+    for e in rng.choice(p.N_elem, 6, replace=False):
+        Cs = []
+        for s in (+1, -1):
+            rho = p.rho.copy()
+            rho[e] += s * h
+            f.assemble(p.E, p.nu, p.thickness, synth.simp_modulus(rho))
+            f.set_bconditions(p)
+            f.solve(SOLVE_FP64, tol=1e-14)
+            Cs.append(f.displacement() @ F)
+        fd = (Cs[0] - Cs[1]) / (2 * h)
+        assert abs(fd - dc[e]) <= 1e-5 * abs(dc[e]), (e, fd, dc[e])
+    f.destroy()
+
+
+def test_adjoint_general_rhs():
+    """K lambda = r for an arbitrary right-hand side (zero on Dirichlet
+    dofs); for r = F the adjoint of compliance is u itself."""
+    import oracle
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_FP64
+    p = synth.mbb(32, 16)
+    f = Fem.from_problem(p)
+    f.solve(SOLVE_FP64, tol=1e-12)
+    u = f.displacement()
+    rp, col, val, F = f.system()
+    lam, info = f.adjoint(F, rtol=1e-12)
+    assert np.linalg.norm(lam - u) <= 1e-9 * np.linalg.norm(u)
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(f.N)
+    fixed = np.zeros(f.N, bool)
+    fixed[np.arange(0, 2 * (p.nx + 1) * (p.ny + 1), 2 * (p.nx + 1))] = True   # u_x on left edge
+    fixed[2 * p.nx + 1] = True                                                 # u_y at (nx, 0)
+    r[fixed] = 0.0
+    lam, info = f.adjoint(r, rtol=1e-12)
+    res = r - oracle.spmv(rp, col, val, lam)
+    assert np.linalg.norm(res) <= 1e-11 * np.linalg.norm(r)
+    f.destroy()
+
+
+def test_spmv_bench_entry():
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem
+    p = synth.cantilever_traction(64, 32)
+    f = Fem.from_problem(p)
+    assert f.spmv_ms(64, 5) > 0 and f.spmv_ms(32, 5) > 0
+    f.destroy()

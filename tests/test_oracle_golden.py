@@ -1,0 +1,104 @@
+"""The CPU oracle reproduces the golden vectors that the compiled reference
+wrote (tests/golden/make_golden.py), bit for bit."""
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import SMALL_CASES, load_golden, load_json, problem_from_golden
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", SMALL_CASES)
+def test_pattern_assembly_bc_pcg(case):
+    g = load_golden(case)
+    p = problem_from_golden(g)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    assert np.array_equal(rp, g["rowptr"]) and np.array_equal(col, g["col"])
+    val, F, st = oracle.assemble(p, rp, col)
+    assert st == 0
+    assert np.array_equal(val, g["val_nobc"]) and np.array_equal(F, g["F_nobc"])
+    oracle.set_bconditions(p, rp, col, val, F)
+    assert np.array_equal(val, g["val"]) and np.array_equal(F, g["F"])
+    assert np.array_equal(oracle.spmv(rp, col, val, np.sin(np.arange(p.N, dtype=float))),
+                          g["spmv_sin"])
+    tol = float(g["pcg_tol"])
+    x, it, tr, cst = oracle.pcg(rp, col, val, F, tol=tol)
+    assert np.array_equal(x, g["x"]) and it == int(g["niter"])
+    assert tr == float(g["tol_reached"]) and cst == int(g["pcg_status"])
+    x7, it7, tr7, st7 = oracle.pcg(rp, col, val, F, tol=tol, max_iter=7)
+    assert np.array_equal(x7, g["x7"])
+    assert (it7, tr7, st7) == (int(g["niter7"]), float(g["tol_reached7"]), int(g["status7"]))
+    if "disp" in g:
+        res = oracle.fem_solve(p)
+        assert np.array_equal(res["u"], g["disp"])
+        assert np.array_equal(res["strain"], g["strain"])
+        assert np.array_equal(res["stress"], g["stress"])
+
+
+@pytest.mark.parametrize("case", ["c1_cantilever_64x32", "traction_128x64"])
+def test_digests(case):
+    g = load_golden(case)
+    p = problem_from_golden(g)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    assert sha(rp) == str(g["rowptr_sha"]) and sha(col) == str(g["col_sha"])
+    val, F, _ = oracle.assemble(p, rp, col)
+    assert sha(val) == str(g["val_nobc_sha"]) and sha(F) == str(g["F_nobc_sha"])
+    oracle.set_bconditions(p, rp, col, val, F)
+    assert sha(val) == str(g["val_sha"]) and sha(F) == str(g["F_sha"])
+    x, it, tr, st = oracle.pcg(rp, col, val, F, tol=float(g["pcg_tol"]))
+    assert sha(x) == str(g["x_sha"]) and it == int(g["niter"])
+    if "disp_sha" in g:
+        res = oracle.fem_solve(p)
+        assert sha(res["u"]) == str(g["disp_sha"])
+        assert sha(res["strain"]) == str(g["strain_sha"])
+        assert sha(res["stress"]) == str(g["stress_sha"])
+
+
+def test_c1_matches_survey_sizes():
+    """64x32 grid: 2,145 nodes, 4,096 T3, 4,290 DOF, nnz = 4*(nodes + 2*edges) = 58,500."""
+    g = load_golden("c1_cantilever_64x32")
+    p = problem_from_golden(g)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    assert (p.N_nod, p.N_elem, p.N, col.size) == (2145, 4096, 4290, 58500)
+
+
+def test_beam_cantilever_replay():
+    """The reference's unit-test mesh replayed through the oracle gives the
+    reference's displacement bit for bit, and its known answer."""
+    g = load_golden("beam_cantilever")
+    p = problem_from_golden(g)
+    res = oracle.fem_solve(p)
+    assert np.array_equal(res["u"], g["disp"])
+    assert np.array_equal(res["strain"], g["strain"])
+    assert np.array_equal(res["stress"], g["stress"])
+    max_disp = np.sqrt((res["u"].reshape(-1, 2) ** 2).sum(1).max())
+    assert abs(max_disp - 1.00701e-1) < 1e-6
+
+
+def test_pcg_edge_cases():
+    cases = load_json("pcg_edge_cases.json")
+    x, it, tr, st = oracle.pcg(np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                               np.array([4.0]), np.array([2.0]), max_iter=10, tol=1e-8)
+    c = cases["one_by_one"]
+    assert math.isnan(x[0]) and (it, tr, st) == (c["niter"], c["tol_reached"], c["status"])
+    rp = np.array([0, 2, 4], np.uint32)
+    col = np.array([0, 1, 0, 1], np.uint32)
+    val = np.array([4.0, 1.0, 1.0, 3.0])
+    x, it, tr, st = oracle.pcg(rp, col, val, np.array([1.0, 2.0]), max_iter=1, tol=1e-12)
+    c = cases["two_by_two_max1"]
+    assert x.tolist() == c["x"] and (it, tr, st) == (c["niter"], c["tol_reached"], c["status"])
+
+
+def test_synth_generator_pinned():
+    from nbots_amd import synth
+    pin = load_json("synth_pin.json")
+    s = pin["seed"]
+    assert [int(v) for v in synth.splitmix64(s, 1, 4)] == pin["splitmix_s1"]
+    assert synth.uniform(s, 2, 4).tolist() == pin["uniform_s2"]
+    assert np.allclose(synth.normal(s, 2, 4), pin["normal_s2"], rtol=1e-15, atol=0)

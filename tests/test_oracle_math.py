@@ -1,0 +1,222 @@
+"""Pins of the oracle that do not depend on the reference: closed forms,
+invariants, brute force and dense solves.  Each is chosen so that a
+plausible slip (a dropped term, a swapped index or sign, a transposed
+block) fails it."""
+import numpy as np
+import pytest
+
+import oracle
+from nbots_amd import synth
+
+
+def cst_closed_form(x, y, E, nu, t):
+    """Textbook constant-strain triangle: Ke = t A B^T D B with
+    b_i = y_j - y_k, c_i = x_k - x_j (cyclic), B over 2A."""
+    A2 = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0])
+    b = np.array([y[1] - y[2], y[2] - y[0], y[0] - y[1]])
+    c = np.array([x[2] - x[1], x[0] - x[2], x[1] - x[0]])
+    B = np.zeros((3, 6))
+    B[0, 0::2] = b
+    B[1, 1::2] = c
+    B[2, 0::2] = c
+    B[2, 1::2] = b
+    B /= A2
+    D = E / (1 - nu * nu) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    return t * (A2 / 2) * B.T @ D @ B
+
+
+def ke(x, y, E, nu, t):
+    L = oracle.lib()
+    D = np.zeros(4)
+    L.orc_plane_stress_D(E, nu, D)
+    detJ = oracle.F64()
+    dx, dy, K = np.zeros(3), np.zeros(3), np.zeros(36)
+    st = L.orc_t3_derivatives(np.asarray(x, float), np.asarray(y, float), oracle.C.byref(detJ), dx, dy)
+    assert st == 0
+    L.orc_t3_ke(D, t, detJ.value, dx, dy, K)
+    return K.reshape(6, 6), detJ.value
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_element_matrix_closed_form_and_invariants(seed):
+    rng = np.random.default_rng(seed)
+    x = np.array([0.0, 1.0, 0.2]) + 0.2 * rng.standard_normal(3)
+    y = np.array([0.0, 0.1, 1.0]) + 0.2 * rng.standard_normal(3)
+    E, nu, t = 1.0 + rng.random(), 0.1 + 0.35 * rng.random(), 0.1 + rng.random()
+    K, detJ = ke(x, y, E, nu, t)
+    assert detJ > 0
+    Kc = cst_closed_form(x, y, E, nu, t)
+    assert np.allclose(K, Kc, rtol=1e-12, atol=1e-12 * np.abs(Kc).max())
+    assert np.array_equal(K, K.T)                      # symmetric bit for bit
+    w = np.linalg.eigvalsh(K)
+    assert w.min() > -1e-12 * w.max() and np.sum(w > 1e-10 * w.max()) == 3   # PSD, rank 3
+    for mode in (np.tile([1.0, 0.0], 3), np.tile([0.0, 1.0], 3),
+                 np.ravel(np.column_stack([-y, x]))):  # rigid translations + rotation
+        assert np.abs(K @ mode).max() <= 1e-12 * np.abs(K).max() * (1 + np.abs(mode).max())
+    K2, _ = ke(x, y, 2.0 * E, nu, t)                    # linear in E
+    assert np.allclose(K2, 2.0 * K, rtol=1e-15, atol=0)
+
+
+def test_distorted_element_detected():
+    L = oracle.lib()
+    detJ = oracle.F64()
+    st = L.orc_t3_derivatives(np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0]),
+                              oracle.C.byref(detJ), np.zeros(3), np.zeros(3))
+    assert st == 1 and detJ.value < 0
+
+
+@pytest.mark.parametrize("nx,ny,jit", [(3, 2, 0.0), (5, 4, 0.1)])
+def test_pattern_brute_force(nx, ny, jit):
+    p = synth.cantilever_point(nx, ny, jitter=jit)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    nbr = [set([v]) for v in range(p.N_nod)]
+    for tri in p.conn.reshape(-1, 3):
+        for a in tri:
+            nbr[a].update(int(b) for b in tri)
+    rows = []
+    for v in range(p.N_nod):
+        cols = sorted(2 * w + k for w in nbr[v] for k in (0, 1))
+        rows += [cols, cols]
+    assert np.array_equal(rp, np.cumsum([0] + [len(r) for r in rows]))
+    assert np.array_equal(col, np.concatenate(rows))
+    edges = {tuple(sorted((int(t[i]), int(t[(i + 1) % 3])))) for t in p.conn.reshape(-1, 3) for i in range(3)}
+    assert col.size == 4 * (p.N_nod + 2 * len(edges))
+
+
+def dense(rp, col, val):
+    N = rp.size - 1
+    A = np.zeros((N, N))
+    for i in range(N):
+        A[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    return A
+
+
+def test_assembly_brute_force():
+    p = synth.cantilever_traction(6, 4, seed=11)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, st = oracle.assemble(p, rp, col)
+    Kd = np.zeros((p.N, p.N))
+    nod = p.nod.reshape(-1, 2)
+    for e, tri in enumerate(p.conn.reshape(-1, 3)):
+        Ke = cst_closed_form(nod[tri, 0], nod[tri, 1], p.elem_E[e], p.nu, p.thickness)
+        d = np.ravel(np.column_stack([2 * tri, 2 * tri + 1]))
+        Kd[np.ix_(d, d)] += Ke
+    assert np.allclose(dense(rp, col, val), Kd, rtol=0, atol=1e-12 * np.abs(Kd).max())
+    assert not F.any()
+
+
+def test_dirichlet_elimination_identity():
+    """After symmetric elimination, K_bc u = F_bc has the solution of the
+    reduced system K_ff u_f = F_f - K_fc u_c with u_c prescribed."""
+    p = synth.cantilever_point(5, 3, jitter=0.1)
+    p.dv = (np.array([2], np.uint32), np.array([0, 1], np.uint8), np.array([0.0, -0.25]))
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    K0 = dense(rp, col, val)
+    F0 = F.copy()
+    oracle.set_bconditions(p, rp, col, val, F)
+    Kb = dense(rp, col, val)
+    assert np.array_equal(Kb, Kb.T)
+    u = np.linalg.solve(Kb, F)
+    fixed = np.zeros(p.N, bool)
+    left = p.sgm_nodes[p.sgm_ptr[3]:p.sgm_ptr[4]]
+    fixed[2 * left] = fixed[2 * left + 1] = True
+    fixed[2 * p.vtx[2] + 1] = True
+    uc = np.zeros(p.N)
+    uc[2 * p.vtx[2] + 1] = -0.25
+    Fn = F0.copy()
+    Fn[2 * p.vtx[4] + 1] += -1.0          # the Neumann point load
+    fr = ~fixed
+    uf = np.linalg.solve(K0[np.ix_(fr, fr)], Fn[fr] - K0[np.ix_(fr, fixed)] @ uc[fixed])
+    assert np.allclose(u[fr], uf, rtol=1e-10, atol=1e-12)
+    assert np.array_equal(u[fixed], uc[fixed])
+
+
+def test_pcg_against_dense_solve():
+    p = synth.cantilever_traction(12, 6, seed=3)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    x, it, tr, st = oracle.pcg(rp, col, val, F, tol=1e-13 * np.linalg.norm(F), max_iter=10000)
+    xd = np.linalg.solve(dense(rp, col, val), F)
+    assert st == 0 and np.allclose(x, xd, rtol=1e-9, atol=1e-12 * np.abs(xd).max())
+
+
+def test_pcg_lagged_stop_hand_trace():
+    """1x1 system a x = b: iteration 0 solves it exactly (g = 0); the stop
+    test still sees ||g_0||, so iteration 1 runs with p = 0 and divides
+    0/0; it then stops with niter = 2, tol_reached = ||g_1|| = 0, x = NaN."""
+    x, it, tr, st = oracle.pcg(np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                               np.array([4.0]), np.array([2.0]), max_iter=10, tol=1e-8)
+    assert np.isnan(x[0]) and it == 2 and tr == 0.0 and st == 0
+    # max_iter = 1 on a 2x2 SPD system: one step of Jacobi-PCG from x = 0
+    A = np.array([[4.0, 1.0], [1.0, 3.0]])
+    b = np.array([1.0, 2.0])
+    q = -b / np.diag(A)                    # g0 = -b
+    p0 = -q
+    alpha = ((-b) @ q) / (p0 @ (A @ p0))
+    x1 = alpha * p0
+    x, it, tr, st = oracle.pcg(np.array([0, 2, 4], np.uint32), np.array([0, 1, 0, 1], np.uint32),
+                               A.ravel(), b, max_iter=1, tol=1e-12)
+    assert np.allclose(x, x1, rtol=1e-15) and it == 1 and st == 1
+    assert tr == np.linalg.norm(b)         # the lagged norm is ||g_0|| = ||b||
+
+
+def test_strain_patch_test():
+    """A linear displacement field gives the exact constant strain on any
+    (jittered) mesh, and stress = D strain."""
+    p = synth.cantilever_point(6, 4, jitter=0.1)
+    nod = p.nod.reshape(-1, 2)
+    G = np.array([[0.01, -0.02], [0.03, 0.015]])
+    u = (nod @ G.T + np.array([0.1, -0.2])).ravel()
+    eps = oracle.strain(p, u).reshape(-1, 3)
+    want = np.array([G[0, 0], G[1, 1], G[0, 1] + G[1, 0]])
+    assert np.allclose(eps, want, rtol=1e-12, atol=1e-14)
+    sig = oracle.stress(p, eps.ravel()).reshape(-1, 3)
+    D = p.E / (1 - p.nu ** 2) * np.array([[1, p.nu, 0], [p.nu, 1, 0], [0, 0, (1 - p.nu) / 2]])
+    assert np.allclose(sig, eps @ D.T, rtol=1e-12, atol=1e-15)
+
+
+def test_energy_identity_and_fd_sensitivity():
+    """Compliance C = F.u = sum_e E_e u_e'K0_e u_e; dC/drho against central
+    finite differences of the full FP64 solve."""
+    p = synth.mbb(8, 4)
+    res = oracle.fem_solve(p, tol=1e-14 * 1.0, max_iter=100000)
+    u = res["u"]
+    ce = oracle.elem_energy(p, u)
+    C, dc = oracle.simp_sens(p.rho, ce)
+    assert abs(C - res["F"] @ u) <= 1e-10 * C
+    h = 1e-6
+    for e in (0, 7, 31, 50):
+        Cs = []
+        for s in (1, -1):
+            q = synth.mbb(8, 4, rho=p.rho.copy())
+            q.rho[e] += s * h
+            q.elem_E = synth.simp_modulus(q.rho)
+            r = oracle.fem_solve(q, tol=1e-14, max_iter=100000)
+            Cs.append(r["F"] @ r["u"])
+        assert abs((Cs[0] - Cs[1]) / (2 * h) - dc[e]) <= 1e-5 * abs(dc[e])
+
+
+def test_filter_and_oc_invariants():
+    p = synth.mbb(16, 8, rho=0.5)
+    cx, cy = synth.elem_centroids(p)
+    H = oracle.filter_build(cx, cy, 1.5)
+    Hptr, Hidx, Hval, Hs = H
+    n = cx.size
+    Hd = np.zeros((n, n))
+    for e in range(n):
+        Hd[e, Hidx[Hptr[e]:Hptr[e + 1]]] = Hval[Hptr[e]:Hptr[e + 1]]
+    d = np.hypot(cx[:, None] - cx[None, :], cy[:, None] - cy[None, :])
+    assert np.allclose(Hd, np.maximum(0.0, 1.5 - d), rtol=0, atol=1e-15)   # brute force
+    assert np.array_equal(Hd, Hd.T) and (Hs > 0).all()
+    assert np.allclose(oracle.filter_apply(H, np.full(n, 0.37)), 0.37, rtol=1e-14)
+    rng = np.random.default_rng(1)
+    dc = -rng.random(n) - 0.1
+    area = synth.elem_areas(p)
+    dv = np.ones(n)
+    xnew, xphys, ch = oracle.oc_update(H, area, np.full(n, 0.5), oracle.filter_sens(H, dc),
+                                       oracle.filter_sens(H, dv), 0.5, 0.2)
+    assert (xnew >= 0.3 - 1e-15).all() and (xnew <= 0.7 + 1e-15).all()   # move limit
+    vol = (area * xphys).sum() / area.sum()
+    assert abs(vol - 0.5) < 2e-3                      # bisection tolerance 1e-3 on lambda
