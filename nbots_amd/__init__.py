@@ -127,7 +127,7 @@ class Sparse:
             raise RuntimeError(f"GPU solve failed: {last_error()}")
         return x, st, it.value, tr.value
 
-    def solve(self, b, mode=SOLVE_FP64, tol=1e-8, max_iter=None, x0=None, inner_rtol=1e-4):
+    def solve(self, b, mode=SOLVE_FP64, tol=1e-8, max_iter=None, x0=None, inner_rtol=0.0):
         x = np.zeros(self.N) if x0 is None else np.array(x0, np.float64)
         info = SolveInfo()
         rc = self.L.nbx_sparse_solve(self.h, np.ascontiguousarray(b, np.float64), x, mode, tol,

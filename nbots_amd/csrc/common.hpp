@@ -48,7 +48,7 @@ constexpr int WAVE = 64;          // CDNA wavefront
 constexpr int BLOCK = 256;        // 4 waves per workgroup
 constexpr int SLICE = 64;         // sliced-ELL slice height = one wavefront
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, 32 CUs each
-constexpr int MAX_GRID = 2048;    // reductions: at most this many workgroups
+constexpr int MAX_GRID = 65535;   // workgroups of a reduction kernel (partials)
 
 inline unsigned grid_for(size_t n, int per_block = BLOCK) {
     size_t g = (n + per_block - 1) / per_block;
@@ -75,7 +75,10 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 // so that the 64 lanes of a wavefront read block k of 64 consecutive rows as
 // one contiguous run.  Block columns within a row ascend, exactly as the
 // scalar rows 2v, 2v+1 of the reference's CSR (columns 2w, 2w+1 per block).
-// Values of block (v,w): {a(2v,2w), a(2v,2w+1), a(2v+1,2w), a(2v+1,2w+1)}.
+// FP64 values sit in two planes so that each wavefront load is one
+// contiguous run: plane a holds {a(2v,2w), a(2v,2w+1)} of every stored
+// block, plane b = val + 2*n_blocks holds {a(2v+1,2w), a(2v+1,2w+1)}.
+// The FP32 copy keeps the four values of a block together (16 bytes).
 struct Sell2 {
     uint32_t n_nodes = 0;
     uint32_t n_slices = 0;
@@ -88,6 +91,11 @@ struct Sell2 {
     double* val = nullptr;         // [4 * n_blocks]
     float* val32 = nullptr;        // [4 * n_blocks], Jacobi-scaled FP32 copy
 };
+
+// index of entry (i, j) of stored block idx in Sell2::val
+__host__ __device__ __forceinline__ size_t ent(size_t n_blocks, size_t idx, int i, int j) {
+    return (i ? 2 * n_blocks : 0) + 2 * idx + j;
+}
 
 // Generic scalar CSR on the device (any vars_per_node).
 struct DevCsr {
@@ -168,87 +176,77 @@ __device__ __forceinline__ void block_sum(f32c (&v)[NV], f32c (*smem)[BLOCK / WA
     }
 }
 
-// Last-workgroup-done grid reduction.  Each workgroup stores its partials;
-// the workgroup that takes the final ticket sums all partials in index
-// order, so the result does not depend on scheduling.  Returns true in
-// thread 0 of that last workgroup, with tot[] holding the sums.
-__device__ __forceinline__ double ld_agent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float ld_agent(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// Grid reductions without fences or atomics: a producing kernel writes one
+// partial per workgroup (thread 0, after block_sum); a one-workgroup
+// finalize kernel launched next on the same stream sums the partials in
+// index order.  Kernel boundaries order the writes, so no device-scope
+// fence is needed (on a multi-XCD part such a fence writes back and
+// invalidates L2, which costs more than the SpMV itself).
 template <int NV>
-__device__ bool grid_sum_last(double (&v)[NV], double* partials, unsigned* ticket,
-                              double (&tot)[NV]) {
-    __shared__ bool s_last;
-    const unsigned nb = gridDim.x;
+__device__ __forceinline__ void put_partials(const double (&v)[NV], double* partials) {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) partials[(size_t)i * nb + blockIdx.x] = v[i];
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        s_last = (t == nb - 1);
+        for (int i = 0; i < NV; ++i) partials[(size_t)i * gridDim.x + blockIdx.x] = v[i];
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    // the last workgroup sums the partials with all its threads
-    __shared__ double red[NV][BLOCK / WAVE];
-    double loc[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double s = 0.0;
-        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) s += ld_agent(&partials[(size_t)i * nb + b]);
-        loc[i] = s;
-    }
-    block_sum<NV>(loc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) tot[i] = loc[i];
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return threadIdx.x == 0;
 }
-
 template <int NV>
-__device__ bool grid_sum_last(f32c (&v)[NV], float* partials, unsigned* ticket,
-                              f32c (&tot)[NV]) {
-    __shared__ bool s_last;
-    const unsigned nb = gridDim.x;
+__device__ __forceinline__ void put_partials(const f32c (&v)[NV], float* partials) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-            partials[(size_t)(2 * i) * nb + blockIdx.x] = v[i].s;
-            partials[(size_t)(2 * i + 1) * nb + blockIdx.x] = v[i].c;
+            partials[(size_t)(2 * i) * gridDim.x + blockIdx.x] = v[i].s;
+            partials[(size_t)(2 * i + 1) * gridDim.x + blockIdx.x] = v[i].c;
         }
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        s_last = (t == nb - 1);
+    }
+}
+
+constexpr int FIN = 1024;   // threads of a finalize kernel
+
+// Sum of np partials per value, in a fixed order; valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void fin_sum(const double* partials, unsigned np, double (&tot)[NV]) {
+    __shared__ double red[NV][FIN / WAVE];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+        for (unsigned b = threadIdx.x; b < np; b += FIN) s += partials[(size_t)i * np + b];
+        s = wave_sum(s);
+        if (lane == 0) red[i][wid] = s;
     }
     __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    __shared__ f32c red[NV][BLOCK / WAVE];
-    f32c loc[NV];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int w = 0; w < FIN / WAVE; ++w) s += red[i][w];
+            tot[i] = s;
+        }
+    }
+}
+template <int NV>
+__device__ __forceinline__ void fin_sum(const float* partials, unsigned np, f32c (&tot)[NV]) {
+    __shared__ f32c red[NV][FIN / WAVE];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         f32c s{0.f, 0.f};
-        for (unsigned b = threadIdx.x; b < nb; b += BLOCK) {
-            f32c o{ld_agent(&partials[(size_t)(2 * i) * nb + b]),
-                   ld_agent(&partials[(size_t)(2 * i + 1) * nb + b])};
+        for (unsigned b = threadIdx.x; b < np; b += FIN) {
+            f32c o{partials[(size_t)(2 * i) * np + b], partials[(size_t)(2 * i + 1) * np + b]};
             s = cadd(s, o);
         }
-        loc[i] = s;
+        s = wave_sum(s);
+        if (lane == 0) red[i][wid] = s;
     }
-    block_sum<NV>(loc, red);
+    __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) tot[i] = loc[i];
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = 0; i < NV; ++i) {
+            f32c s{0.f, 0.f};
+            for (int w = 0; w < FIN / WAVE; ++w) s = cadd(s, red[i][w]);
+            tot[i] = s;
+        }
     }
-    return threadIdx.x == 0;
 }
 
 }  // namespace nbx

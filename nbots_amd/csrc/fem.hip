@@ -324,9 +324,8 @@ __global__ __launch_bounds__(BLOCK) void k_assemble(Sell2 K, const double2* __re
             acc[3] += kb[3];
         }
     }
-    double2* out = reinterpret_cast<double2*>(K.val) + 2 * idx;
-    out[0] = make_double2(acc[0], acc[1]);
-    out[1] = make_double2(acc[2], acc[3]);
+    reinterpret_cast<double2*>(K.val + ent(K.n_blocks, idx, 0, 0))[0] = make_double2(acc[0], acc[1]);
+    reinterpret_cast<double2*>(K.val + ent(K.n_blocks, idx, 1, 0))[0] = make_double2(acc[2], acc[3]);
 }
 
 // F from self weight: Fe (pipeline.c:224-228) gathered per node in element
@@ -409,7 +408,7 @@ __global__ void k_bc_rhs(Sell2 K, const uint32_t* __restrict__ rank, const doubl
                     const uint32_t c = 2 * w + j, rk = rank[c];
                     if (rk != FREE && (int64_t)rk > last && (best == FREE || rk < best)) {
                         best = rk;
-                        a = K.val[4 * idx + 2 * i + j];
+                        a = K.val[ent(K.n_blocks, idx, i, j)];
                         val = vf[c];
                     }
                 }
@@ -432,7 +431,7 @@ __global__ void k_bc_matrix(Sell2 K, const uint32_t* __restrict__ rank) {
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) {
                 const uint32_t r = 2 * v + i, c = 2 * w + j;
-                double* a = K.val + 4 * idx + 2 * i + j;
+                double* a = K.val + ent(K.n_blocks, idx, i, j);
                 if (rank[r] != FREE) *a = (r == c) ? 1.0 : 0.0;
                 else if (rank[c] != FREE) *a = 0.0;
             }
@@ -486,8 +485,7 @@ __global__ __launch_bounds__(BLOCK) void k_compliance(const double2* __restrict_
                                                       double penal, double E0, double Emin,
                                                       const double* __restrict__ elemE, uint32_t ne,
                                                       double* __restrict__ ce, double* __restrict__ dc,
-                                                      double* partials, unsigned* ticket,
-                                                      double* out) {
+                                                      double* partials) {
     double acc[1] = {0.0};
     double D[4];
     plane_stress(1.0, nu, D);
@@ -525,8 +523,13 @@ __global__ __launch_bounds__(BLOCK) void k_compliance(const double2* __restrict_
     }
     __shared__ double red[1][BLOCK / WAVE];
     block_sum<1>(acc, red);
+    put_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(FIN) void k_fin_sum1(const double* partials, unsigned np, double* out) {
     double tot[1];
-    if (grid_sum_last<1>(acc, partials, ticket, tot)) *out = tot[0];
+    fin_sum<1>(partials, np, tot);
+    if (threadIdx.x == 0) *out = tot[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -776,8 +779,8 @@ int fem_compliance(FemSys& F, const double* rho, double penal, double E0, double
     unsigned g = grid_for(F.ne);
     if (g > (unsigned)MAX_GRID) g = MAX_GRID;
     hipLaunchKernelGGL(k_compliance, dim3(g), dim3(BLOCK), 0, F.s, F.nod, F.conn, F.u, F.P.nu,
-                       F.P.thickness, rho, penal, E0, Emin, F.P.elemE, F.ne, ce, dc, F.ws.partials,
-                       F.ws.ticket, out);
+                       F.P.thickness, rho, penal, E0, Emin, F.P.elemE, F.ne, ce, dc, F.ws.partials);
+    hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(FIN), 0, F.s, F.ws.partials, g, out);
     NBX_HIP(hipGetLastError());
     NBX_HIP(hipMemcpyAsync(C_host, out, sizeof(double), hipMemcpyDeviceToHost, F.s));
     NBX_HIP(hipStreamSynchronize(F.s));
@@ -847,8 +850,8 @@ int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* v
                 const uint32_t o = rowptr[2 * v + i] + 2 * k;
                 col[o] = 2 * bc[idx];
                 col[o + 1] = 2 * bc[idx] + 1;
-                val[o] = hv[4 * idx + 2 * i];
-                val[o + 1] = hv[4 * idx + 2 * i + 1];
+                val[o] = hv[ent(S.n_blocks, idx, i, 0)];
+                val[o + 1] = hv[ent(S.n_blocks, idx, i, 1)];
             }
         }
     }

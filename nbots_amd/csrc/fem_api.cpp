@@ -35,6 +35,13 @@ struct nb_material_s {
 
 namespace {
 
+nbx::MixedOpts mixed_opts(double inner_rtol) {
+    nbx::MixedOpts o;
+    o.restart = inner_rtol > 0.0;
+    if (o.restart) o.inner_rtol = inner_rtol;
+    return o;
+}
+
 struct BcAtom {
     uint32_t id;
     bool mask[2];
@@ -213,7 +220,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, double in
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
         if (!S.K.has_val32) NBX_TRY(nbx::mat_make_scaled32(S.K, S.s));
-        status = nbx::solve_mixed(S.K, S.F, S.u, tol, inner_rtol > 0 ? inner_rtol : 1e-4, max_iter,
+        status = nbx::solve_mixed(S.K, S.F, S.u, tol, mixed_opts(inner_rtol), max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
         info->iters = st.iters;

@@ -18,7 +18,10 @@ struct PcgState {
     double tol_reached;
     uint32_t k, max_iter;
     uint32_t done, status;
-    uint32_t breakdown, pad;
+    uint32_t breakdown, hold;      // hold: paused for a reliable update
+    double maxrr;                  // largest r.r since the last reliable update
+    double conv_rr;                // r.r at which the true residual should meet tol
+    uint32_t updates, pad;
 };
 
 // A device matrix: 2-dof blocked sliced ELL when the pattern allows it,
@@ -58,6 +61,15 @@ struct SolveStats {
     int fallback = 0;          // 1 if the FP32 sweeps stalled and FP64 PCG finished
 };
 
+// Mixed-precision settings: FP32 CG with FP64 reliable updates when
+// restart == false; restarted iterative refinement (inner relative
+// tolerance inner_rtol per sweep) when restart == true.
+struct MixedOpts {
+    bool restart = false;
+    double inner_rtol = 1e-4;
+    double delta = 0.1;        // reliable update when r.r < delta^2 max r.r
+};
+
 int ws_alloc(Workspace& w, uint32_t n);
 void ws_free(Workspace& w);
 void mat_free(DevMatrix& m);
@@ -80,7 +92,7 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
 // FP32 inner CG on the Jacobi-scaled FP32 copy inside FP64 iterative
 // refinement; stops when ||b - A x||_2 <= rtol * ||b||_2.
 int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol,
-                double inner_rtol, uint32_t max_iter, Workspace& w, hipStream_t s,
+                const MixedOpts& opt, uint32_t max_iter, Workspace& w, hipStream_t s,
                 SolveStats* stats);
 
 // FP64 Jacobi PCG with a relative tolerance (x0 = x on input).

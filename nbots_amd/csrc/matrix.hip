@@ -1,0 +1,398 @@
+// Device matrices for gfx950: upload of a host CSR into the 2x2-block
+// sliced-ELL format (or scalar CSR when the pattern is not 2-dof blocked),
+// diagonal extraction, the Jacobi-scaled FP32 copy, SpMV and its benchmark,
+// and solver workspaces.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "rows.hpp"
+
+namespace nbx {
+
+static char g_err[512];
+void set_last_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+const char* last_error() { return g_err; }
+
+
+// y = A x, one node (or scalar row) per thread
+template <class Op>
+__global__ __launch_bounds__(BLOCK) void k_spmv(Op op, const typename Op::V* __restrict__ x,
+                                                typename Op::V* __restrict__ y, uint32_t n) {
+    const Chunk c = my_chunk(n);
+    for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) y[u] = op.row((uint32_t)u, x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_spmv32(Sell2 A, const float2* __restrict__ x,
+                                                  float2* __restrict__ y, uint32_t n) {
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) y[i] = row32(A, (uint32_t)i, x);
+}
+
+__global__ void k_norm2(const double* __restrict__ v, size_t n, double* partials) {
+    double acc[1] = {0.0};
+    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLOCK)
+        acc[0] += v[i] * v[i];
+    __shared__ double red[1][BLOCK / WAVE];
+    block_sum<1>(acc, red);
+    put_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(FIN) void k_fin_out(const double* partials, unsigned np, double* out) {
+    double tot[1];
+    fin_sum<1>(partials, np, tot);
+    if (threadIdx.x == 0) *out = tot[0];
+}
+
+__global__ void k_diag_sell2(Sell2 A, double* __restrict__ diag) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    double d0 = 0.0, d1 = 0.0;
+    for (int k = 0; k < A.rowlen[v]; ++k, idx += SLICE) {
+        if (A.bcol[idx] == v) {
+            d0 = A.val[ent(A.n_blocks, idx, 0, 0)];
+            d1 = A.val[ent(A.n_blocks, idx, 1, 1)];
+        }
+    }
+    diag[2 * v] = d0;
+    diag[2 * v + 1] = d1;
+}
+
+// nb_sparse_get(A, i, i): 0.0 when (i,i) is not stored (sparse.c:164-171)
+__global__ void k_diag_csr(DevCsr A, double* __restrict__ diag) {
+    const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r >= A.n) return;
+    int64_t lo = A.rowptr[r], hi = (int64_t)A.rowptr[r + 1] - 1;
+    double d = 0.0;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) / 2;
+        uint32_t cm = A.col[mid];
+        if (cm == r) {
+            d = A.val[mid];
+            break;
+        }
+        if (cm < r) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    diag[r] = d;
+}
+
+// dis = 1/sqrt(diag); val32 = fl32(a_ij * dis_i * dis_j), computed in FP64.
+__global__ void k_dis(const double* __restrict__ diag, double* __restrict__ dis, uint32_t n,
+                      unsigned* bad) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i];
+    if (!(d > 0.0)) {
+        atomicAdd(bad, 1u);
+        dis[i] = 1.0;
+    } else {
+        dis[i] = 1.0 / sqrt(d);
+    }
+}
+
+__global__ void k_scale32(Sell2 A, const double* __restrict__ dis) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    const double r0 = dis[2 * v], r1 = dis[2 * v + 1];
+    for (int k = 0; k < len; ++k, idx += SLICE) {
+        const uint32_t w = A.bcol[idx];
+        const double c0 = dis[2 * w], c1 = dis[2 * w + 1];
+        const double* a = A.val + ent(A.n_blocks, idx, 0, 0);
+        const double* b = A.val + ent(A.n_blocks, idx, 1, 0);
+        float4 o;
+        o.x = (float)(a[0] * r0 * c0);
+        o.y = (float)(a[1] * r0 * c1);
+        o.z = (float)(b[0] * r1 * c0);
+        o.w = (float)(b[1] * r1 * c1);
+        reinterpret_cast<float4*>(A.val32)[idx] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static unsigned map_grid(uint32_t n_units) {
+    unsigned g = grid_for(n_units);
+    return g > 65535u ? 65535u : g;
+}
+
+int ws_alloc(Workspace& w, uint32_t n) {
+    if (w.n >= n && w.st) return NBX_OK;
+    ws_free(w);
+    const size_t nn = (size_t)(n ? n : 1) + 2;
+    for (auto& p : w.v64) NBX_HIP(hipMalloc(&p, nn * sizeof(double)));
+    for (auto& p : w.v32) NBX_HIP(hipMalloc(&p, nn * sizeof(float)));
+    NBX_HIP(hipMalloc(&w.partials, 4 * MAX_GRID * sizeof(double)));
+    NBX_HIP(hipMalloc(&w.partials32, 4 * MAX_GRID * sizeof(float)));
+    NBX_HIP(hipMalloc(&w.ticket, 8 * sizeof(unsigned)));
+    NBX_HIP(hipMemset(w.ticket, 0, 8 * sizeof(unsigned)));
+    NBX_HIP(hipMalloc(&w.st, 2 * sizeof(PcgState)));
+    NBX_HIP(hipHostMalloc(&w.h_st, 2 * sizeof(PcgState), hipHostMallocDefault));
+    for (auto& e : w.ev) NBX_HIP(hipEventCreate(&e));
+    for (auto& e : w.sev) NBX_HIP(hipEventCreate(&e));
+    w.n = n;
+    return NBX_OK;
+}
+
+void ws_free(Workspace& w) {
+    for (auto& p : w.v64) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    for (auto& p : w.v32) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    if (w.partials) (void)hipFree(w.partials);
+    if (w.partials32) (void)hipFree(w.partials32);
+    if (w.ticket) (void)hipFree(w.ticket);
+    if (w.st) (void)hipFree(w.st);
+    if (w.h_st) (void)hipHostFree(w.h_st);
+    for (auto& e : w.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    for (auto& e : w.sev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    w.partials = nullptr;
+    w.partials32 = nullptr;
+    w.ticket = nullptr;
+    w.st = nullptr;
+    w.h_st = nullptr;
+    w.n = 0;
+}
+
+void mat_free(DevMatrix& m) {
+    void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32,
+                  m.c.rowptr,    m.c.col,    m.c.val,  m.diag,  m.dis};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    m = DevMatrix();
+}
+
+// Is the CSR a 2-dof node matrix (rows 2v, 2v+1 share their columns, which
+// come in pairs 2w, 2w+1)?
+static bool csr_blockable(uint32_t n, const uint32_t* rp, const uint32_t* col) {
+    if (n == 0 || (n & 1u)) return false;
+    for (uint32_t v = 0; v < n / 2; ++v) {
+        const uint32_t a = rp[2 * v], b = rp[2 * v + 1], e = rp[2 * v + 2];
+        const uint32_t la = b - a, lb = e - b;
+        if (la != lb || (la & 1u) || la / 2 > 255u) return false;
+        for (uint32_t k = 0; k < la; k += 2) {
+            const uint32_t c0 = col[a + k], c1 = col[a + k + 1];
+            if ((c0 & 1u) || c1 != c0 + 1 || col[b + k] != c0 || col[b + k + 1] != c1) return false;
+        }
+    }
+    return true;
+}
+
+static void sell2_values_host(uint32_t nn, const uint32_t* rp, const double* val,
+                              const std::vector<uint32_t>& sptr, std::vector<double>& hv) {
+    const size_t nb = sptr.back();
+    for (uint32_t v = 0; v < nn; ++v) {
+        const uint32_t a = rp[2 * v], b = rp[2 * v + 1], len = (b - a) / 2;
+        size_t idx = (size_t)sptr[v >> 6] + (v & 63);
+        for (uint32_t k = 0; k < len; ++k, idx += SLICE) {
+            hv[ent(nb, idx, 0, 0)] = val[a + 2 * k];
+            hv[ent(nb, idx, 0, 1)] = val[a + 2 * k + 1];
+            hv[ent(nb, idx, 1, 0)] = val[b + 2 * k];
+            hv[ent(nb, idx, 1, 1)] = val[b + 2 * k + 1];
+        }
+    }
+}
+
+int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rp, const uint32_t* col,
+                   const double* val, bool values_only, hipStream_t s) {
+    if (!values_only) {
+        mat_free(m);
+        m.n = n;
+        m.blocked = csr_blockable(n, rp, col);
+        NBX_HIP(hipMalloc(&m.diag, (size_t)(n ? n : 1) * sizeof(double)));
+    }
+    if (m.blocked) {
+        const uint32_t nn = n / 2;
+        const uint32_t ns = (nn + SLICE - 1) / SLICE;
+        std::vector<uint32_t> sptr(ns + 1, 0);
+        std::vector<uint8_t> len(nn);
+        uint32_t maxlen = 0;
+        for (uint32_t v = 0; v < nn; ++v) {
+            len[v] = (uint8_t)((rp[2 * v + 1] - rp[2 * v]) / 2);
+            if (len[v] > maxlen) maxlen = len[v];
+        }
+        for (uint32_t sl = 0; sl < ns; ++sl) {
+            uint32_t wdt = 0;
+            for (uint32_t v = sl * SLICE; v < nn && v < (sl + 1) * SLICE; ++v)
+                if (len[v] > wdt) wdt = len[v];
+            sptr[sl + 1] = sptr[sl] + wdt * SLICE;
+        }
+        const size_t nb = sptr[ns];
+        std::vector<double> hv(4 * nb + 4, 0.0);
+        sell2_values_host(nn, rp, val, sptr, hv);
+        if (!values_only) {
+            std::vector<uint32_t> hc(nb + 1);
+            for (uint32_t v = 0; v < nn; ++v) {
+                const uint32_t a = rp[2 * v];
+                size_t idx = (size_t)sptr[v >> 6] + (v & 63);
+                const uint32_t wdt = (sptr[(v >> 6) + 1] - sptr[v >> 6]) / SLICE;
+                for (uint32_t k = 0; k < wdt; ++k, idx += SLICE)
+                    hc[idx] = k < len[v] ? col[a + 2 * k] / 2 : v;
+            }
+            Sell2& S = m.s;
+            S.n_nodes = nn;
+            S.n_slices = ns;
+            S.n_blocks = nb;
+            S.n_valid = rp[n] / 4;
+            S.max_rowlen = maxlen;
+            NBX_HIP(hipMalloc(&S.slice_ptr, (ns + 1) * sizeof(uint32_t)));
+            NBX_HIP(hipMalloc(&S.rowlen, (nn + 1) * sizeof(uint8_t)));
+            NBX_HIP(hipMalloc(&S.bcol, (nb + 1) * sizeof(uint32_t)));
+            NBX_HIP(hipMalloc(&S.val, (4 * nb + 4) * sizeof(double)));
+            NBX_HIP(hipMemcpyAsync(S.slice_ptr, sptr.data(), (ns + 1) * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+            NBX_HIP(hipMemcpyAsync(S.rowlen, len.data(), nn, hipMemcpyHostToDevice, s));
+            NBX_HIP(hipMemcpyAsync(S.bcol, hc.data(), (nb + 1) * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+            NBX_HIP(hipMemcpyAsync(S.val, hv.data(), (4 * nb + 4) * sizeof(double),
+                                   hipMemcpyHostToDevice, s));
+            NBX_HIP(hipStreamSynchronize(s));
+        } else {
+            NBX_HIP(hipMemcpyAsync(m.s.val, hv.data(), (4 * nb + 4) * sizeof(double),
+                                   hipMemcpyHostToDevice, s));
+            NBX_HIP(hipStreamSynchronize(s));
+        }
+        m.has_val32 = false;
+    } else {
+        DevCsr& Cm = m.c;
+        const size_t nnz = n ? rp[n] : 0;
+        if (!values_only) {
+            Cm.n = n;
+            Cm.nnz = nnz;
+            NBX_HIP(hipMalloc(&Cm.rowptr, (size_t)(n + 1) * sizeof(uint32_t)));
+            NBX_HIP(hipMalloc(&Cm.col, (nnz + 1) * sizeof(uint32_t)));
+            NBX_HIP(hipMalloc(&Cm.val, (nnz + 1) * sizeof(double)));
+            NBX_HIP(hipMemcpyAsync(Cm.rowptr, rp, (size_t)(n + 1) * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, s));
+            if (nnz)
+                NBX_HIP(hipMemcpyAsync(Cm.col, col, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        }
+        if (nnz) NBX_HIP(hipMemcpyAsync(Cm.val, val, nnz * sizeof(double), hipMemcpyHostToDevice, s));
+        NBX_HIP(hipStreamSynchronize(s));
+    }
+    return mat_extract_diag(m, s);
+}
+
+int mat_extract_diag(DevMatrix& m, hipStream_t s) {
+    if (m.n == 0) return NBX_OK;
+    if (m.blocked)
+        hipLaunchKernelGGL(k_diag_sell2, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.diag);
+    else
+        hipLaunchKernelGGL(k_diag_csr, dim3(grid_for(m.c.n)), dim3(BLOCK), 0, s, m.c, m.diag);
+    NBX_HIP(hipGetLastError());
+    m.has_val32 = false;
+    return NBX_OK;
+}
+
+int mat_make_scaled32(DevMatrix& m, hipStream_t s) {
+    if (!m.blocked) {
+        set_last_error("mixed precision needs a 2-dof node matrix");
+        return NBX_ERR_ARG;
+    }
+    if (!m.dis) NBX_HIP(hipMalloc(&m.dis, (size_t)m.n * sizeof(double)));
+    if (!m.s.val32) NBX_HIP(hipMalloc(&m.s.val32, (4 * m.s.n_blocks + 4) * sizeof(float)));
+    unsigned* bad = nullptr;
+    NBX_HIP(hipMalloc(&bad, sizeof(unsigned)));
+    NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
+    hipLaunchKernelGGL(k_scale32, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
+    NBX_HIP(hipGetLastError());
+    unsigned hbad = 0;
+    NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    (void)hipFree(bad);
+    if (hbad) {
+        set_last_error("matrix diagonal has %u non-positive entries", hbad);
+        return NBX_ERR_ARG;
+    }
+    m.has_val32 = true;
+    return NBX_OK;
+}
+
+int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s) {
+    if (m.n == 0) return NBX_OK;
+    if (m.blocked) {
+        OpS2 op{m.s};
+        hipLaunchKernelGGL(k_spmv<OpS2>, dim3(map_grid(m.s.n_nodes)), dim3(BLOCK), 0, s, op,
+                           (const double2*)x, (double2*)y, m.s.n_nodes);
+    } else {
+        OpCsr op{m.c};
+        hipLaunchKernelGGL(k_spmv<OpCsr>, dim3(map_grid(m.c.n)), dim3(BLOCK), 0, s, op, x, y, m.c.n);
+    }
+    NBX_HIP(hipGetLastError());
+    return NBX_OK;
+}
+
+// Average duration of `reps` back-to-back SpMV launches (ms), FP64 or the
+// scaled FP32 copy, on x = 1.
+int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStream_t s,
+               double* ms_avg) {
+    if (!m.blocked || m.n == 0) {
+        set_last_error("spmv_bench needs a 2-dof node matrix");
+        return NBX_ERR_ARG;
+    }
+    NBX_TRY(ws_alloc(w, m.n));
+    const uint32_t nu = m.s.n_nodes;
+    std::vector<double> ones(m.n, 1.0);
+    std::vector<float> onesf(m.n, 1.0f);
+    NBX_HIP(hipMemcpyAsync(w.v64[0], ones.data(), m.n * 8, hipMemcpyHostToDevice, s));
+    NBX_HIP(hipMemcpyAsync(w.v32[0], onesf.data(), m.n * 4, hipMemcpyHostToDevice, s));
+    const unsigned G = map_grid(nu);
+    auto launch = [&]() {
+        if (prec == 32)
+            hipLaunchKernelGGL(k_spmv32, dim3(G), dim3(BLOCK), 0, s, m.s, (const float2*)w.v32[0],
+                               (float2*)w.v32[1], nu);
+        else
+            hipLaunchKernelGGL(k_spmv<OpS2>, dim3(G), dim3(BLOCK), 0, s, OpS2{m.s},
+                               (const double2*)w.v64[0], (double2*)w.v64[1], nu);
+    };
+    launch();   // warm-up
+    NBX_HIP(hipEventRecord(w.ev[2], s));
+    for (uint32_t r = 0; r < reps; ++r) launch();
+    NBX_HIP(hipEventRecord(w.ev[3], s));
+    NBX_HIP(hipEventSynchronize(w.ev[3]));
+    NBX_HIP(hipGetLastError());
+    float ms = 0.f;
+    NBX_HIP(hipEventElapsedTime(&ms, w.ev[2], w.ev[3]));
+    *ms_avg = reps ? ms / reps : 0.0;
+    return NBX_OK;
+}
+
+double device_norm2(uint32_t n, const double* v, Workspace& w, hipStream_t s, int* err) {
+    *err = NBX_OK;
+    double* out = w.v64[5] + w.n;   // spare slot past the vector end
+    unsigned g = red_grid(n);
+    hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BLOCK), 0, s, v, (size_t)n, w.partials);
+    hipLaunchKernelGGL(k_fin_out, dim3(1), dim3(FIN), 0, s, w.partials, g, out);
+    double h = 0.0;
+    if (hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        *err = NBX_ERR_HIP;
+        return 0.0;
+    }
+    return h;
+}
+
+}  // namespace nbx

@@ -35,6 +35,13 @@ struct nb_sparse_s {
 
 namespace {
 
+nbx::MixedOpts mixed_opts(double inner_rtol) {
+    nbx::MixedOpts o;
+    o.restart = inner_rtol > 0.0;
+    if (o.restart) o.inner_rtol = inner_rtol;
+    return o;
+}
+
 int64_t find_slot(const nb_sparse_t* A, uint32_t i, uint32_t j) {
     if (i >= A->N) return -1;
     const uint32_t* b = A->col.data() + A->rowptr[i];
@@ -356,7 +363,7 @@ int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode,
             rc = nbx::mat_make_scaled32(A->dev, A->stream);
             if (rc) return rc;
         }
-        status = nbx::solve_mixed(A->dev, db, dx, tol, inner_rtol > 0 ? inner_rtol : 1e-4, max_iter,
+        status = nbx::solve_mixed(A->dev, db, dx, tol, mixed_opts(inner_rtol), max_iter,
                                   A->ws, A->stream, &st);
         if (status > 1) return status;
         info->iters = st.iters;

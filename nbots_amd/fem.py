@@ -78,7 +78,7 @@ class Fem:
             None if ns_fval is None else _c(ns_fval, np.float64), factor)
         check(rc, "nbx_fem_set_bconditions")
 
-    def solve(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=None, inner_rtol=1e-4, u0=None):
+    def solve(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=None, inner_rtol=0.0, u0=None):
         info = SolveInfo()
         rc = self.L.nbx_fem_solve(self.h, mode, tol, 20 * self.N if max_iter is None else max_iter,
                                   inner_rtol, None if u0 is None else _c(u0, np.float64),
