@@ -3,17 +3,18 @@
 Workload (default): 2048x1024 cantilever, 4,194,304 T3 elements,
 4,200,450 DOF, lognormal(0,1) per-element modulus, clamped left edge,
 uniform shear traction on the right edge.  K, F and all solver vectors are
-resident in HBM before the timed region.  One step = one solve from u = 0
-to relative residual ||F - K u|| / ||F|| <= 1e-8 with FP32 inner CG
-(tolerance 1e-4 per sweep) inside FP64 iterative refinement.
+resident in HBM before the timed region.  One step = one FP64
+Jacobi-PCG solve (the reference's algorithm) from u = 0 to relative residual
+||F - K u|| / ||F|| <= 1e-8.
 
-The JSON line reports DOF * CG-iteration / s of that solve (whole job), the
-time to 1e-8 of the mixed and of plain FP64 Jacobi-PCG on the same GPU, the
-HBM roofline of the dominant kernel (the FP32 SpMV fused with its dot
-product, timed with HIP events on the solver's stream), and the reference's
-own CPU Jacobi-PCG timed on this host over a bounded number of iterations.
+The JSON line reports DOF * CG-iteration / s over the timed steps (whole
+job), the time to 1e-8 of FP64 PCG and of the mixed-precision solver
+(FP32 inner work, FP64 residual and update) on the same GPU, the HBM
+roofline of the dominant kernel (the FP64 SpMV with its p.w partials, timed
+with HIP events on the solver's stream), and the reference's own CPU
+Jacobi-PCG timed on this host over a bounded number of iterations.
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu] [--mixed cheb]
 """
 from __future__ import annotations
 
@@ -113,9 +114,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--mixed", default="reliable", choices=["reliable", "restart"])
+    ap.add_argument("--mixed", default="cheb", choices=["cheb", "restart", "reliable"])
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cheb-degree", type=int, default=8)
+    ap.add_argument("--cheb-ratio", type=float, default=100.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args()
@@ -125,7 +128,8 @@ def main():
         sys.exit(2)
 
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, SOLVE_FP64, SOLVE_MIXED
+    from nbots_amd.fem import (Fem, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART,
+                               MIXED_CG_RELIABLE, solve_opts)
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
@@ -144,78 +148,92 @@ def main():
     log(f"[bench] pattern {t_pattern:.2f}s, assembly+BC {t_assemble:.2f}s, nnz {nnz}, "
         f"stored blocks {nblk} (valid {n_valid})")
 
-    # FP64 Jacobi-PCG to 1e-8 (the comparison baseline on the same GPU)
-    fp64 = None
-    if not args.no_fp64:
-        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000)
-        u64 = f.displacement()
-        fp64 = info
-        log(f"[bench] FP64 PCG: {info['iters']} it, {info['ms']/1e3:.3f}s, rel_res {info['rel_res']:.2e}, "
-            f"spmv {info['ms_spmv']*1e3:.1f}us")
-
-    inner = 1e-4 if args.mixed == "restart" else 0.0
+    # Steps: FP64 Jacobi-PCG solves to 1e-8 (the reference's algorithm,
+    # device resident); value = DOF * CG iterations / s over the timed steps.
     for w in range(args.warmup):
-        info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, inner_rtol=inner)
-        log(f"[bench] warmup {w}: {info['iters']} it / {info['outer']} updates, {info['ms']/1e3:.3f}s, "
-            f"fallback {info['fallback']}")
+        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000)
+        log(f"[bench] warmup {w}: FP64 PCG {info['iters']} it, {info['ms']/1e3:.3f}s")
     times, iters, infos = [], [], []
     t_wall0 = time.perf_counter()
     for s in range(args.steps):
         t0 = time.perf_counter()
-        info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, inner_rtol=inner)
+        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000)
         dt = time.perf_counter() - t0
         times.append(dt)
         iters.append(info["iters"])
         infos.append(info)
-        log(f"[bench] step {s}: {info['iters']} it / {info['outer']} sweeps, {dt:.3f}s wall, "
-            f"{info['ms']/1e3:.3f}s device, rel_res {info['rel_res']:.2e}, fallback {info['fallback']}")
+        log(f"[bench] step {s}: FP64 PCG {info['iters']} it, {dt:.3f}s wall, "
+            f"{info['ms']/1e3:.3f}s device, rel_res {info['rel_res']:.2e}, "
+            f"spmv {info['ms_spmv']*1e3:.1f}us")
     t_wall = time.perf_counter() - t_wall0
-    u32 = f.displacement()
-    total_iters = sum(iters)
-    value = N * total_iters / t_wall
+    u64 = f.displacement()
+    value = N * sum(iters) / t_wall
     ms_step = 1e3 * t_wall / args.steps
 
-    # dominant kernel: FP32 SpMV + dot, sampled on the solver's stream
+    # dominant kernel: the FP64 SpMV (+ p.w partials), sampled with HIP
+    # events on the solver's stream during the timed steps
     spmv_ms = float(np.mean([i["ms_spmv"] for i in infos if i["ms_spmv"] > 0]))
-    b32 = spmv_bytes(n_nodes, n_valid, 32)
-    achieved = b32 / (spmv_ms * 1e-3) / 1e9
+    b64 = spmv_bytes(n_nodes, n_valid, 64)
+    achieved = b64 / (spmv_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_in_A (FP32 2x2-block SpMV + p.w)", bytes_per_launch=b32,
-                    us_per_launch=round(spmv_ms * 1e3, 2))
+                    kernel="k_pcg_A<OpS2> (FP64 2x2-block sliced-ELL SpMV + p.w partials)",
+                    bytes_per_launch=b64, us_per_launch=round(spmv_ms * 1e3, 2))
+
+    # mixed precision, timed separately: time to 1e-8 against the FP64 steps
     extra = {}
-    if fp64 is not None:
-        b64 = spmv_bytes(n_nodes, n_valid, 64)
-        err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    if not args.no_mixed:
+        variant = dict(cheb=MIXED_CHEB_FCG, restart=MIXED_CG_RESTART,
+                       reliable=MIXED_CG_RELIABLE)[args.mixed]
+        opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio)
+        f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, opts=opts)
+        mtimes, minfos = [], []
+        for s in range(2):
+            t0 = time.perf_counter()
+            info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, opts=opts)
+            mtimes.append(time.perf_counter() - t0)
+            minfos.append(info)
+            log(f"[bench] mixed ({args.mixed}) {s}: {info['iters']} it, {info['outer']} inner, "
+                f"{mtimes[-1]:.3f}s, rel_res {info['rel_res']:.2e}, fallback {info['fallback']}")
+        u32 = f.displacement()
+        t_mixed = float(np.median(mtimes))
+        t_fp64 = float(np.median(times))
+        ms32 = f.spmv_ms(32, 20)
+        ms64 = f.spmv_ms(64, 20)
         extra = dict(
-            time_to_1e8_mixed_s=round(float(np.median(times)), 4),
-            time_to_1e8_fp64_s=round(fp64["ms"] / 1e3, 4),
-            mixed_speedup_vs_fp64=round(fp64["ms"] / 1e3 / float(np.median(times)), 3),
-            fp64_iters=fp64["iters"], fp64_rel_res=fp64["rel_res"],
-            fp64_dof_iter_per_s=N * fp64["iters"] / (fp64["ms"] / 1e3),
-            fp64_spmv_us=round(fp64["ms_spmv"] * 1e3, 2),
-            fp64_spmv_gbs=round(b64 / (fp64["ms_spmv"] * 1e-3) / 1e9, 1),
-            mixed_vs_fp64_disp_rel_l2=err)
+            time_to_1e8_fp64_s=round(t_fp64, 4),
+            time_to_1e8_mixed_s=round(t_mixed, 4),
+            mixed_speedup_vs_fp64=round(t_fp64 / t_mixed, 3),
+            mixed_variant=args.mixed,
+            mixed_solver=("FP64 flexible CG, FP32 Chebyshev-Jacobi preconditioner "
+                          f"(degree {opts.cheb_degree}, [lmax/{opts.cheb_ratio:g}, lmax])"
+                          if variant == MIXED_CHEB_FCG else
+                          "FP32 CG sweeps (inner rtol 1e-4) in FP64 iterative refinement"
+                          if variant == MIXED_CG_RESTART else
+                          "FP32 CG with FP64 reliable updates"),
+            mixed_iters=minfos[-1]["iters"], mixed_inner=minfos[-1]["outer"],
+            mixed_rel_res=minfos[-1]["rel_res"], mixed_fallback=minfos[-1]["fallback"],
+            mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            spmv64_us=round(ms64 * 1e3, 2),
+            spmv64_gbs=round(b64 / (ms64 * 1e-3) / 1e9, 1),
+            spmv32_us=round(ms32 * 1e3, 2),
+            spmv32_gbs=round(spmv_bytes(n_nodes, n_valid, 32) / (ms32 * 1e-3) / 1e9, 1),
+            fp64_iter_us=round(1e6 * t_fp64 / iters[-1], 1),
+            fp64_iter_gbs=round(pcg_iter_bytes(n_nodes, n_valid, 64) * iters[-1] / t_fp64 / 1e9, 1))
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: ~{args.cpu_seconds}s of iterations, {args.cpu_threads} threads")
         cpu = cpu_baseline(f, p, args.cpu_seconds, args.cpu_threads)
         log(f"[bench] CPU: {cpu['value']:.3e} DOF*it/s ({cpu['kind']}, {cpu['seconds']:.1f}s)")
-        if fp64 is not None:
-            cpu["time_to_1e8_fp64_extrapolated_s"] = round(fp64["iters"] * cpu["seconds"] / cpu["iters"], 1)
+        cpu["time_to_1e8_fp64_extrapolated_s"] = round(iters[-1] * cpu["seconds"] / cpu["iters"], 1)
     line = dict(
         metric=METRIC, value=value, unit="DOF*CG-iter/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True,
-        scaling="weak", vs_baseline=None, dtype="f32+f64", data="synthetic",
+        scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
         config=dict(workload=cfg["name"], dof=N, elements=p.N_elem, nnz=nnz,
                     stored_blocks=nblk,
-                    solver=("FP32 Jacobi-scaled CG with FP64 reliable updates (residual and "
-                            "solution in FP64)" if inner == 0.0 else
-                            "restarted FP32 CG sweeps (inner rtol 1e-4) in FP64 refinement")
-                    + ", stop at ||F-Ku||/||F|| <= 1e-8",
-                    iters_per_step=iters, fp64_updates_per_step=[i["outer"] for i in infos],
-                    fallback=[i["fallback"] for i in infos],
-                    rel_res=[i["rel_res"] for i in infos]),
+                    solver="FP64 Jacobi-PCG from u = 0 to ||F-Ku||/||F|| <= 1e-8",
+                    iters_per_step=iters, rel_res=[i["rel_res"] for i in infos]),
         roofline=roofline, cpu_baseline=cpu, setup_s=dict(pattern=round(t_pattern, 3),
                                                           assembly_bc=round(t_assemble, 3)),
         **extra)

@@ -214,18 +214,39 @@ enum {
 enum {
 	NBX_SOLVE_REFERENCE = 0, /* abs tol on ||Ax-b||, lagged test, x0 given */
 	NBX_SOLVE_FP64 = 1,      /* Jacobi-PCG, relative tolerance           */
-	NBX_SOLVE_MIXED = 2      /* FP32 CG inside FP64 refinement, rel. tol */
+	NBX_SOLVE_MIXED = 2      /* FP32 inner work, FP64 residual and update,
+				    relative tol; variant in nbx_solve_opts_t */
 };
+enum {
+	NBX_MIXED_CG_RESTART = 0,   /* FP32 CG sweeps to inner_rtol inside FP64
+				       iterative refinement                   */
+	NBX_MIXED_CG_RELIABLE = 1,  /* FP32 CG, FP64 reliable updates          */
+	NBX_MIXED_CHEB_FCG = 2      /* FP64 flexible CG, FP32 Chebyshev-Jacobi
+				       preconditioner (default)               */
+};
+typedef struct {
+	int32_t mixed_variant;   /* NBX_MIXED_*                               */
+	uint32_t cheb_degree;    /* SpMVs per preconditioner application (8)  */
+	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (100) */
+	double inner_rtol;       /* CG_RESTART inner tolerance (1e-4)         */
+	double reliable_delta;   /* CG_RELIABLE update threshold (0.1)        */
+} nbx_solve_opts_t;
+/* Fills *opts with the defaults above. */
+void nbx_solve_opts_default(nbx_solve_opts_t *opts);
 
 typedef struct {
-	uint32_t iters;       /* CG iterations (FP32 inner ones for MIXED) */
-	uint32_t outer;       /* refinement sweeps (MIXED)                  */
+	uint32_t iters;       /* CG iterations: FP64 PCG, the FP32 ones of the
+				 MIXED CG variants, the FP64 flexible-CG ones
+				 of MIXED_CHEB_FCG                          */
+	uint32_t outer;       /* MIXED: FP64 residual updates (CG variants) or
+				 FP32 SpMVs in the preconditioner (CHEB)    */
 	uint32_t fp64_iters;  /* FP64 PCG iterations (incl. a fallback)     */
 	int32_t status;       /* 0 converged, 1 not, else nbx error         */
 	double rel_res;       /* final ||b-Ax||/||b|| (FP64)                */
 	double tol_reached;   /* reference mode: the lagged ||g||           */
 	double ms;            /* device time of the solve                   */
-	int32_t fallback;     /* MIXED sweeps stalled, FP64 PCG finished    */
+	int32_t fallback;     /* MIXED CG: sweeps stalled and FP64 PCG
+				 finished; CHEB: direction restarts         */
 	int32_t pad;
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
@@ -233,10 +254,15 @@ typedef struct {
 const char *nbx_last_error(void);
 int nbx_device_info(char *name, uint32_t len, uint32_t *n_cu, double *hbm_gb);
 
-/* Solve with a drop-in matrix in any mode (no reference counterpart).    */
+/* Solve with a drop-in matrix in any mode (no reference counterpart).
+ * inner_rtol selects the mixed variant: 0 default (CHEB_FCG), > 0
+ * CG_RESTART with that inner tolerance, < 0 CG_RELIABLE.               */
 int nbx_sparse_solve(const nb_sparse_t *A, const double *b, double *x,
 		     int mode, double tol, uint32_t max_iter,
 		     double inner_rtol, nbx_solve_info_t *info);
+int nbx_sparse_solve_ex(const nb_sparse_t *A, const double *b, double *x,
+			int mode, double tol, uint32_t max_iter,
+			const nbx_solve_opts_t *opts, nbx_solve_info_t *info);
 /* y = A x with device pointers (no reference counterpart; benchmarking). */
 int nbx_sparse_spmv_dev(const nb_sparse_t *A, const double *x_dev,
 			double *y_dev);
@@ -283,10 +309,14 @@ int nbx_fem_set_bconditions(nbx_fem_t *sys, const uint32_t *vtx,
 			    const int32_t *ns_fptr, const double *ns_fval,
 			    double factor);
 /* Solve K u = F.  mode NBX_SOLVE_*, tol absolute (REFERENCE) or relative;
- * u_init (nullable) is the initial guess, else 0.                      */
+ * u_init (nullable) is the initial guess, else 0; inner_rtol as in
+ * nbx_sparse_solve.                                                    */
 int nbx_fem_solve(nbx_fem_t *sys, int mode, double tol, uint32_t max_iter,
 		  double inner_rtol, const double *u_init,
 		  nbx_solve_info_t *info);
+int nbx_fem_solve_ex(nbx_fem_t *sys, int mode, double tol, uint32_t max_iter,
+		     const nbx_solve_opts_t *opts, const double *u_init,
+		     nbx_solve_info_t *info);
 int nbx_fem_get_displacement(const nbx_fem_t *sys, double *u);
 /* K and F in the reference's scalar CSR layout (row-sorted columns).   */
 int nbx_fem_get_system(const nbx_fem_t *sys, uint32_t *rowptr, uint32_t *col,

@@ -37,8 +37,23 @@ namespace {
 
 nbx::MixedOpts mixed_opts(double inner_rtol) {
     nbx::MixedOpts o;
-    o.restart = inner_rtol > 0.0;
-    if (o.restart) o.inner_rtol = inner_rtol;
+    if (inner_rtol > 0.0) {
+        o.variant = nbx::CG_RESTART;
+        o.inner_rtol = inner_rtol;
+    } else if (inner_rtol < 0.0) {
+        o.variant = nbx::CG_RELIABLE;
+    }
+    return o;
+}
+
+nbx::MixedOpts mixed_opts(const nbx_solve_opts_t* p) {
+    nbx::MixedOpts o;
+    if (!p) return o;
+    o.variant = (nbx::MixedVariant)p->mixed_variant;
+    if (p->cheb_degree) o.cheb_degree = p->cheb_degree;
+    if (p->cheb_ratio > 1.0) o.cheb_ratio = p->cheb_ratio;
+    if (p->inner_rtol > 0.0) o.inner_rtol = p->inner_rtol;
+    if (p->reliable_delta > 0.0) o.delta = p->reliable_delta;
     return o;
 }
 
@@ -199,7 +214,7 @@ std::vector<BcAtom> atoms(uint32_t n, const uint32_t* id, const uint8_t* mask, c
     return v;
 }
 
-int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, double inner_rtol,
+int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx::MixedOpts& mo,
               nbx_solve_info_t* info) {
     nbx::SolveStats st;
     int status = 0;
@@ -220,7 +235,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, double in
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
         if (!S.K.has_val32) NBX_TRY(nbx::mat_make_scaled32(S.K, S.s));
-        status = nbx::solve_mixed(S.K, S.F, S.u, tol, mixed_opts(inner_rtol), max_iter,
+        status = nbx::solve_mixed(S.K, S.F, S.u, tol, mo, max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
         info->iters = st.iters;
@@ -316,8 +331,8 @@ int nbx_fem_set_bconditions(nbx_fem_t* f, const uint32_t* vtx, const uint32_t* s
     return nbx::fem_apply_bc(S, plan);
 }
 
-int nbx_fem_solve(nbx_fem_t* f, int mode, double tol, uint32_t max_iter, double inner_rtol,
-                  const double* u_init, nbx_solve_info_t* info) {
+static int fem_solve_entry(nbx_fem_t* f, int mode, double tol, uint32_t max_iter,
+                           const nbx::MixedOpts& mo, const double* u_init, nbx_solve_info_t* info) {
     nbx::FemSys& S = f->sys;
     nbx_solve_info_t tmp;
     if (!info) info = &tmp;
@@ -325,7 +340,26 @@ int nbx_fem_solve(nbx_fem_t* f, int mode, double tol, uint32_t max_iter, double 
         NBX_HIP(hipMemcpyAsync(S.u, u_init, 2 * (size_t)S.nn * 8, hipMemcpyHostToDevice, S.s));
     else
         NBX_HIP(hipMemsetAsync(S.u, 0, 2 * (size_t)S.nn * 8, S.s));
-    return fem_solve(S, mode, tol, max_iter, inner_rtol, info);
+    return fem_solve(S, mode, tol, max_iter, mo, info);
+}
+
+void nbx_solve_opts_default(nbx_solve_opts_t* o) {
+    const nbx::MixedOpts d;
+    o->mixed_variant = (int32_t)d.variant;
+    o->cheb_degree = d.cheb_degree;
+    o->cheb_ratio = d.cheb_ratio;
+    o->inner_rtol = d.inner_rtol;
+    o->reliable_delta = d.delta;
+}
+
+int nbx_fem_solve(nbx_fem_t* f, int mode, double tol, uint32_t max_iter, double inner_rtol,
+                  const double* u_init, nbx_solve_info_t* info) {
+    return fem_solve_entry(f, mode, tol, max_iter, mixed_opts(inner_rtol), u_init, info);
+}
+
+int nbx_fem_solve_ex(nbx_fem_t* f, int mode, double tol, uint32_t max_iter,
+                     const nbx_solve_opts_t* opts, const double* u_init, nbx_solve_info_t* info) {
+    return fem_solve_entry(f, mode, tol, max_iter, mixed_opts(opts), u_init, info);
 }
 
 int nbx_fem_get_displacement(const nbx_fem_t* f, double* u) {
@@ -387,7 +421,7 @@ int nbx_fem_adjoint(nbx_fem_t* f, const double* rhs, double* lambda, int mode, d
     std::swap(S.F, S.aux);
     std::swap(S.u, S.aux2);
     nbx_solve_info_t tmp;
-    int rc = fem_solve(S, mode, rtol, max_iter, 1e-4, info ? info : &tmp);
+    int rc = fem_solve(S, mode, rtol, max_iter, nbx::MixedOpts(), info ? info : &tmp);
     std::swap(S.F, S.aux);
     std::swap(S.u, S.aux2);
     if (rc > 1) return rc;

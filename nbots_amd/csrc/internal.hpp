@@ -34,6 +34,7 @@ struct DevMatrix {
     double* diag = nullptr;    // [n] diagonal A(i,i) (0.0 if not stored)
     double* dis = nullptr;     // [n] 1/sqrt(diag) for the scaled FP32 copy
     bool has_val32 = false;
+    double lambda_max = 0.0;   // Gershgorin bound of the scaled matrix
 };
 
 // Per-matrix solver scratch (device).  Sized for n scalar rows.
@@ -61,13 +62,18 @@ struct SolveStats {
     int fallback = 0;          // 1 if the FP32 sweeps stalled and FP64 PCG finished
 };
 
-// Mixed-precision settings: FP32 CG with FP64 reliable updates when
-// restart == false; restarted iterative refinement (inner relative
-// tolerance inner_rtol per sweep) when restart == true.
+// Mixed-precision settings.
+//  CG_RESTART: FP32 CG sweeps to inner_rtol inside FP64 iterative refinement
+//  CG_RELIABLE: FP32 CG with FP64 reliable updates (no restarts)
+//  CHEB_FCG: FP64 flexible CG preconditioned by a fixed-degree Chebyshev
+//            iteration on the FP32 Jacobi-scaled matrix
+enum MixedVariant { CG_RESTART = 0, CG_RELIABLE = 1, CHEB_FCG = 2 };
 struct MixedOpts {
-    bool restart = false;
+    MixedVariant variant = CHEB_FCG;
     double inner_rtol = 1e-4;
     double delta = 0.1;        // reliable update when r.r < delta^2 max r.r
+    uint32_t cheb_degree = 8;  // SpMVs per preconditioner application
+    double cheb_ratio = 100.0; // Chebyshev interval [lambda_max/ratio, lambda_max]
 };
 
 int ws_alloc(Workspace& w, uint32_t n);

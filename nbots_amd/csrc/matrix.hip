@@ -120,6 +120,43 @@ __global__ void k_scale32(Sell2 A, const double* __restrict__ dis) {
     }
 }
 
+// Gershgorin bound of the scaled FP32 matrix: max over rows of sum |a_ij|.
+__global__ void k_gersh(Sell2 A, double* partials) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    double m = 0.0;
+    if (v < A.n_nodes) {
+        size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = 0; k < A.rowlen[v]; ++k, idx += SLICE) {
+            const float4 a = reinterpret_cast<const float4*>(A.val32)[idx];
+            s0 += fabs((double)a.x) + fabs((double)a.y);
+            s1 += fabs((double)a.z) + fabs((double)a.w);
+        }
+        m = fmax(s0, s1);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, WAVE));
+    __shared__ double red[BLOCK / WAVE];
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / WAVE; ++w) m = fmax(m, red[w]);
+        partials[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(FIN) void k_fin_max(const double* partials, unsigned np, double* out) {
+    double m = 0.0;
+    for (unsigned b = threadIdx.x; b < np; b += FIN) m = fmax(m, partials[b]);
+    for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, WAVE));
+    __shared__ double red[FIN / WAVE];
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < FIN / WAVE; ++w) m = fmax(m, red[w]);
+        *out = m;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -318,11 +355,21 @@ int mat_make_scaled32(DevMatrix& m, hipStream_t s) {
     NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
     hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
     hipLaunchKernelGGL(k_scale32, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
+    // Gershgorin bound of the scaled matrix (Chebyshev preconditioner)
+    const unsigned gg = grid_for(m.s.n_nodes);
+    double *part = nullptr, *dmax = nullptr;
+    NBX_HIP(hipMalloc(&part, (size_t)gg * sizeof(double)));
+    NBX_HIP(hipMalloc(&dmax, sizeof(double)));
+    hipLaunchKernelGGL(k_gersh, dim3(gg), dim3(BLOCK), 0, s, m.s, part);
+    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(FIN), 0, s, part, gg, dmax);
     NBX_HIP(hipGetLastError());
     unsigned hbad = 0;
     NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(&m.lambda_max, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
     NBX_HIP(hipStreamSynchronize(s));
     (void)hipFree(bad);
+    (void)hipFree(part);
+    (void)hipFree(dmax);
     if (hbad) {
         set_last_error("matrix diagonal has %u non-positive entries", hbad);
         return NBX_ERR_ARG;

@@ -335,6 +335,158 @@ __global__ __launch_bounds__(BLOCK) void k_mix_update(double2* __restrict__ u,
 }
 
 // ---------------------------------------------------------------------------
+// mixed precision: FP64 flexible CG, FP32 Chebyshev-Jacobi preconditioner
+// ---------------------------------------------------------------------------
+// z = D^-1/2 c(Kt) D^-1/2 r, where c(Kt) s is m steps of the Chebyshev
+// iteration for Kt y = s from y = 0 on [lmin, lmax] (Saad, Iterative Methods
+// for Sparse Linear Systems, 2nd ed., Alg. 12.1), Kt = D^-1/2 K D^-1/2 in
+// FP32.  The outer iteration is CG with the Polak-Ribiere beta (Notay 2000,
+// flexible CG), all in FP64 with the FP64 matrix, so FP32 rounding only
+// weakens the preconditioner and never the attainable residual.
+
+// first: r = b - K x; otherwise x += alpha p, r -= alpha w.  Then the
+// preconditioner's start: s = fl32(dis r), res = s, d = z = s / theta.
+__global__ __launch_bounds__(BLOCK) void k_fcg_update(Sell2 K, const double2* __restrict__ b,
+                                                      double2* __restrict__ x,
+                                                      double2* __restrict__ r,
+                                                      const double2* __restrict__ p,
+                                                      const double2* __restrict__ w,
+                                                      const double2* __restrict__ dis,
+                                                      float2* __restrict__ res,
+                                                      float2* __restrict__ d,
+                                                      float2* __restrict__ z, PcgState* st,
+                                                      double* partials, uint32_t n, int first,
+                                                      float inv_theta) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double acc[1] = {0.0};
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        double2 rv;
+        if (first) {
+            const double2 ku = row64(K, (uint32_t)i, x);
+            const double2 bi = b[i];
+            rv = make_double2(bi.x - ku.x, bi.y - ku.y);
+        } else {
+            x[i] = axpyu(x[i], alpha, p[i]);
+            const double2 ro = r[i], wv = w[i];
+            rv = make_double2(ro.x - alpha * wv.x, ro.y - alpha * wv.y);
+        }
+        r[i] = rv;
+        const double2 dd = dis[i];
+        const float2 sv = make_float2((float)(dd.x * rv.x), (float)(dd.y * rv.y));
+        res[i] = sv;
+        const float2 dv = make_float2(sv.x * inv_theta, sv.y * inv_theta);
+        d[i] = dv;
+        z[i] = dv;
+        acc[0] += rv.x * rv.x + rv.y * rv.y;
+    }
+    __shared__ double red[1][BLOCK / WAVE];
+    block_sum<1>(acc, red);
+    put_partials<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(FIN) void k_fin_fcg_update(PcgState* st, const double* partials,
+                                                        unsigned np, int first) {
+    if (st->done) return;
+    double tot[1];
+    fin_sum<1>(partials, np, tot);
+    if (threadIdx.x == 0) {
+        const double gg = tot[0];
+        if (!first) st->k += 1;
+        st->gg_cur = gg;
+        st->tol_reached = sqrt(gg);
+        if (!(gg > st->tol2)) {
+            st->done = 1;
+            st->status = 0;
+        } else if (st->k >= st->max_iter || !isfinite(gg)) {
+            st->done = 1;
+            st->status = 1;
+        }
+    }
+}
+
+// one Chebyshev step: res -= Kt d; d' = c1 d + c2 res; z += d'
+__global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __restrict__ din,
+                                                     float2* __restrict__ dout,
+                                                     float2* __restrict__ res,
+                                                     float2* __restrict__ z, PcgState* st,
+                                                     uint32_t n, float c1, float c2) {
+    if (st->done) return;
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const float2 t = row32(K, (uint32_t)i, din);
+        float2 rs = res[i];
+        rs.x = rs.x - t.x;
+        rs.y = rs.y - t.y;
+        const float2 di = din[i];
+        const float2 dn = make_float2(c1 * di.x + c2 * rs.x, c1 * di.y + c2 * rs.y);
+        float2 zv = z[i];
+        zv.x = zv.x + dn.x;
+        zv.y = zv.y + dn.y;
+        res[i] = rs;
+        dout[i] = dn;
+        z[i] = zv;
+    }
+}
+
+// z64 = dis z; partials z64.w (Polak-Ribiere numerator) and r.z64
+__global__ __launch_bounds__(BLOCK) void k_fcg_dots(const float2* __restrict__ z,
+                                                    const double2* __restrict__ dis,
+                                                    const double2* __restrict__ w,
+                                                    const double2* __restrict__ r, PcgState* st,
+                                                    double* partials, uint32_t n, int first) {
+    if (st->done) return;
+    double acc[2] = {0.0, 0.0};
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 dd = dis[i];
+        const float2 zv = z[i];
+        const double2 zz = make_double2(dd.x * (double)zv.x, dd.y * (double)zv.y);
+        if (!first) acc[0] += dotu(zz, w[i]);
+        acc[1] += dotu(r[i], zz);
+    }
+    __shared__ double red[2][BLOCK / WAVE];
+    block_sum<2>(acc, red);
+    put_partials<2>(acc, partials);
+}
+
+__global__ __launch_bounds__(FIN) void k_fin_fcg_dots(PcgState* st, const double* partials,
+                                                      unsigned np, int first) {
+    if (st->done) return;
+    double tot[2];
+    fin_sum<2>(partials, np, tot);
+    if (threadIdx.x == 0) {
+        const double zw = tot[0], rz = tot[1];
+        double beta = first ? 0.0 : -st->alpha * zw / st->rz;
+        if (!(rz > 0.0) || !isfinite(rz) || !isfinite(beta)) {
+            // the preconditioned residual lost positivity: restart the
+            // direction (p = z) and count the event
+            beta = 0.0;
+            st->breakdown += 1;
+        }
+        st->beta = beta;
+        st->rz = rz;
+    }
+}
+
+// p = z64 + beta p  (p = z64 on the first step)
+__global__ __launch_bounds__(BLOCK) void k_fcg_p(const float2* __restrict__ z,
+                                                 const double2* __restrict__ dis,
+                                                 double2* __restrict__ p, PcgState* st,
+                                                 uint32_t n, int first) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 dd = dis[i];
+        const float2 zv = z[i];
+        const double2 zz = make_double2(dd.x * (double)zv.x, dd.y * (double)zv.y);
+        p[i] = first ? zz : axpyu(zz, beta, p[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static int poll_state(Workspace& w, hipStream_t s) {
@@ -447,14 +599,10 @@ int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol, uint
     return st == 0 ? NBX_OK : NBX_NOT_CONVERGED;
 }
 
-int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, const MixedOpts& opt,
-                uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats) {
-    if (!m.blocked || !m.has_val32) {
-        set_last_error("solve_mixed: matrix has no scaled FP32 copy");
-        return NBX_ERR_ARG;
-    }
-    int rc = ws_alloc(w, m.n);
-    if (rc) return rc;
+static int solve_mixed_cg(const DevMatrix& m, const double* b, double* x, double rtol,
+                          const MixedOpts& opt, uint32_t max_iter, Workspace& w, hipStream_t s,
+                          SolveStats* stats) {
+    int rc = NBX_OK;
     const uint32_t nu = m.s.n_nodes;
     const double bb = device_norm2(m.n, b, w, s, &rc);
     if (rc) return rc;
@@ -469,7 +617,8 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, con
     const double2* dis = (const double2*)m.dis;
     const unsigned G = red_grid(nu);
     const double delta2 = opt.delta * opt.delta;
-    const double inner_stop = opt.restart ? opt.inner_rtol * opt.inner_rtol : 0.0;
+    const bool restart = opt.variant == CG_RESTART;
+    const double inner_stop = restart ? opt.inner_rtol * opt.inner_rtol : 0.0;
     uint32_t nsamp = 0, samp_valid = 0, updates = 0;
     double prev_res = -1.0;
     int stalled = 0;
@@ -508,7 +657,7 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, con
             }
             // reliable update (or the end of a restarted sweep): fold y into
             // u and recompute the true residual with the FP64 matrix
-            const int first = opt.restart ? 1 : 0;
+            const int first = restart ? 1 : 0;
             hipLaunchKernelGGL(k_mix_update, dim3(G), dim3(BLOCK), 0, s, (double2*)x, dis, y32, w.st,
                                nu);
             hipLaunchKernelGGL(k_mix_residual, dim3(G), dim3(BLOCK), 0, s, m.s, (const double2*)b,
@@ -520,7 +669,7 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, con
             ++updates;
             NBX_TRY(poll_state(w, s));
             const double res = w.h_st->tol_reached;
-            if (!w.h_st->done && opt.restart && res > 0.5 * prev_res) {
+            if (!w.h_st->done && restart && res > 0.5 * prev_res) {
                 stalled = 1;   // a sweep that no longer reduces the FP64 residual
                 break;
             }
@@ -559,6 +708,116 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, con
         stats->fallback = stalled;
     }
     return status;
+}
+
+
+// FP64 flexible CG with the FP32 Chebyshev-Jacobi preconditioner.
+static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double rtol,
+                          const MixedOpts& opt, uint32_t max_iter, Workspace& w, hipStream_t s,
+                          SolveStats* stats) {
+    const uint32_t nu = m.s.n_nodes;
+    int rc = NBX_OK;
+    const double bb = device_norm2(m.n, b, w, s, &rc);
+    if (rc) return rc;
+    PcgState h{};
+    h.tol2 = rtol * rtol * bb;
+    h.max_iter = max_iter;
+    NBX_HIP(hipMemcpyAsync(w.st, &h, sizeof h, hipMemcpyHostToDevice, s));
+    // Chebyshev coefficients on [lmax/ratio, lmax] (Saad, Alg. 12.1)
+    const uint32_t deg = opt.cheb_degree < 1 ? 1 : opt.cheb_degree;
+    const double lmax = m.lambda_max, lmin = lmax / opt.cheb_ratio;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    std::vector<float> c1(deg), c2(deg);
+    double rho = 1.0 / sigma;
+    for (uint32_t k = 1; k < deg; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        c1[k] = (float)(rn * rho);
+        c2[k] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+    const float inv_theta = (float)(1.0 / theta);
+    double2* r = (double2*)w.v64[0];
+    double2* p = (double2*)w.v64[1];
+    double2* wv = (double2*)w.v64[2];
+    float2* res = (float2*)w.v32[0];
+    float2* da = (float2*)w.v32[1];
+    float2* db = (float2*)w.v32[2];
+    float2* z = (float2*)w.v32[3];
+    const double2* dis = (const double2*)m.dis;
+    const double2* bv = (const double2*)b;
+    double2* xv = (double2*)x;
+    const unsigned G = red_grid(nu);
+    OpS2 op{m.s};
+    auto precondition = [&](int first) {
+        hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv, dis, res,
+                           da, z, w.st, w.partials, nu, first, inv_theta);
+        hipLaunchKernelGGL(k_fin_fcg_update, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        float2* din = da;
+        float2* dout = db;
+        for (uint32_t k = 1; k < deg; ++k) {
+            hipLaunchKernelGGL(k_cheb_step, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z, w.st, nu,
+                               c1[k], c2[k]);
+            float2* t = din;
+            din = dout;
+            dout = t;
+        }
+        hipLaunchKernelGGL(k_fcg_dots, dim3(G), dim3(BLOCK), 0, s, z, dis, wv, r, w.st, w.partials, nu,
+                           first);
+        hipLaunchKernelGGL(k_fin_fcg_dots, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        hipLaunchKernelGGL(k_fcg_p, dim3(G), dim3(BLOCK), 0, s, z, dis, p, w.st, nu, first);
+    };
+    NBX_HIP(hipEventRecord(w.ev[0], s));
+    precondition(1);
+    NBX_HIP(hipGetLastError());
+    NBX_TRY(poll_state(w, s));
+    uint32_t batch = 4, nsamp = 0, launched = 0;
+    bool first_batch = true;
+    while (!w.h_st->done) {
+        for (uint32_t i = 0; i < batch; ++i) {
+            const bool samp = first_batch && nsamp < (uint32_t)Workspace::NSAMP;
+            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
+            hipLaunchKernelGGL(k_pcg_A<OpS2>, dim3(G), dim3(BLOCK), 0, s, op, (const double2*)p, wv,
+                               w.st, w.partials, nu);
+            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
+            hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+            precondition(0);
+        }
+        NBX_HIP(hipGetLastError());
+        launched += batch;
+        NBX_TRY(poll_state(w, s));
+        first_batch = false;
+        if (launched > max_iter + 8u && !w.h_st->done) {
+            set_last_error("flexible CG did not report completion");
+            return NBX_ERR_BREAKDOWN;
+        }
+        batch = batch < 32 ? batch * 2 : 32;
+    }
+    NBX_HIP(hipEventRecord(w.ev[1], s));
+    NBX_HIP(hipEventSynchronize(w.ev[1]));
+    if (stats) {
+        float ms = 0.f;
+        NBX_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+        stats->ms = ms;
+        stats->iters = w.h_st->k;
+        stats->outer = w.h_st->k * (deg - 1);    // FP32 SpMVs in the preconditioner
+        stats->fp64_iters = w.h_st->k;
+        stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
+        stats->fallback = (int)w.h_st->breakdown;
+        NBX_TRY(sampled_ms(w, nsamp < w.h_st->k ? nsamp : w.h_st->k, &stats->ms_spmv_avg));
+    }
+    return w.h_st->status ? NBX_NOT_CONVERGED : NBX_OK;
+}
+
+int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, const MixedOpts& opt,
+                uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats) {
+    if (!m.blocked || !m.has_val32) {
+        set_last_error("solve_mixed: matrix has no scaled FP32 copy");
+        return NBX_ERR_ARG;
+    }
+    int rc = ws_alloc(w, m.n);
+    if (rc) return rc;
+    if (opt.variant == CHEB_FCG) return solve_fcg_cheb(m, b, x, rtol, opt, max_iter, w, s, stats);
+    return solve_mixed_cg(m, b, x, rtol, opt, max_iter, w, s, stats);
 }
 
 }  // namespace nbx

@@ -37,8 +37,23 @@ namespace {
 
 nbx::MixedOpts mixed_opts(double inner_rtol) {
     nbx::MixedOpts o;
-    o.restart = inner_rtol > 0.0;
-    if (o.restart) o.inner_rtol = inner_rtol;
+    if (inner_rtol > 0.0) {
+        o.variant = nbx::CG_RESTART;
+        o.inner_rtol = inner_rtol;
+    } else if (inner_rtol < 0.0) {
+        o.variant = nbx::CG_RELIABLE;
+    }
+    return o;
+}
+
+nbx::MixedOpts mixed_opts(const nbx_solve_opts_t* p) {
+    nbx::MixedOpts o;
+    if (!p) return o;
+    o.variant = (nbx::MixedVariant)p->mixed_variant;
+    if (p->cheb_degree) o.cheb_degree = p->cheb_degree;
+    if (p->cheb_ratio > 1.0) o.cheb_ratio = p->cheb_ratio;
+    if (p->inner_rtol > 0.0) o.inner_rtol = p->inner_rtol;
+    if (p->reliable_delta > 0.0) o.delta = p->reliable_delta;
     return o;
 }
 
@@ -80,6 +95,9 @@ int sync_device(const nb_sparse_t* A) {
 void stale(nb_sparse_t* A) { A->dev_values = false; }
 
 }  // namespace
+
+static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
+                        uint32_t max_iter, const nbx::MixedOpts& mo, nbx_solve_info_t* info);
 
 extern "C" {
 
@@ -280,7 +298,7 @@ int nb_sparse_solve_CG_precond_Jacobi(const nb_sparse_t* const A, const double* 
                                       uint32_t* niter_performed, double* tolerance_reached,
                                       uint32_t) {
     nbx_solve_info_t info;
-    int rc = nbx_sparse_solve(A, b, _x, NBX_SOLVE_REFERENCE, tolerance, max_iter, 0.0, &info);
+    int rc = sparse_solve(A, b, _x, NBX_SOLVE_REFERENCE, tolerance, max_iter, nbx::MixedOpts(), &info);
     if (rc != NBX_OK && rc != NBX_NOT_CONVERGED) {
         fprintf(stderr, "nb_sparse_solve_CG_precond_Jacobi: GPU failure: %s\n", nbx::last_error());
         return -1;
@@ -330,8 +348,8 @@ int nbx_sparse_spmv_dev(const nb_sparse_t* A, const double* x_dev, double* y_dev
     return nbx::spmv(A->dev, x_dev, y_dev, A->stream);
 }
 
-int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
-                     uint32_t max_iter, double inner_rtol, nbx_solve_info_t* info) {
+static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
+                        uint32_t max_iter, const nbx::MixedOpts& mo, nbx_solve_info_t* info) {
     nbx_solve_info_t tmp;
     if (!info) info = &tmp;
     memset(info, 0, sizeof *info);
@@ -363,7 +381,7 @@ int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode,
             rc = nbx::mat_make_scaled32(A->dev, A->stream);
             if (rc) return rc;
         }
-        status = nbx::solve_mixed(A->dev, db, dx, tol, mixed_opts(inner_rtol), max_iter,
+        status = nbx::solve_mixed(A->dev, db, dx, tol, mo, max_iter,
                                   A->ws, A->stream, &st);
         if (status > 1) return status;
         info->iters = st.iters;
@@ -381,6 +399,16 @@ int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode,
     NBX_HIP(hipMemcpyAsync(x, dx, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream));
     NBX_HIP(hipStreamSynchronize(A->stream));
     return status;
+}
+
+int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
+                     uint32_t max_iter, double inner_rtol, nbx_solve_info_t* info) {
+    return sparse_solve(A, b, x, mode, tol, max_iter, mixed_opts(inner_rtol), info);
+}
+
+int nbx_sparse_solve_ex(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
+                        uint32_t max_iter, const nbx_solve_opts_t* opts, nbx_solve_info_t* info) {
+    return sparse_solve(A, b, x, mode, tol, max_iter, mixed_opts(opts), info);
 }
 
 }  // extern "C"

@@ -6,9 +6,26 @@ import ctypes as C
 
 import numpy as np
 
-from .lib import SolveInfo, check, last_error, load
+from .lib import SolveInfo, SolveOpts, check, last_error, load
 
 SOLVE_REFERENCE, SOLVE_FP64, SOLVE_MIXED = 0, 1, 2
+MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
+
+
+def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,
+               reliable_delta=None):
+    o = SolveOpts()
+    load().nbx_solve_opts_default(C.byref(o))
+    o.mixed_variant = variant
+    if cheb_degree is not None:
+        o.cheb_degree = cheb_degree
+    if cheb_ratio is not None:
+        o.cheb_ratio = cheb_ratio
+    if inner_rtol is not None:
+        o.inner_rtol = inner_rtol
+    if reliable_delta is not None:
+        o.reliable_delta = reliable_delta
+    return o
 
 
 def _c(a, dt):
@@ -78,11 +95,16 @@ class Fem:
             None if ns_fval is None else _c(ns_fval, np.float64), factor)
         check(rc, "nbx_fem_set_bconditions")
 
-    def solve(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=None, inner_rtol=0.0, u0=None):
+    def solve(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=None, inner_rtol=0.0, u0=None,
+              opts=None):
+        """opts (solve_opts(...)) overrides inner_rtol's variant selection."""
         info = SolveInfo()
-        rc = self.L.nbx_fem_solve(self.h, mode, tol, 20 * self.N if max_iter is None else max_iter,
-                                  inner_rtol, None if u0 is None else _c(u0, np.float64),
-                                  C.byref(info))
+        mi = 20 * self.N if max_iter is None else max_iter
+        u0p = None if u0 is None else _c(u0, np.float64)
+        if opts is not None:
+            rc = self.L.nbx_fem_solve_ex(self.h, mode, tol, mi, C.byref(opts), u0p, C.byref(info))
+        else:
+            rc = self.L.nbx_fem_solve(self.h, mode, tol, mi, inner_rtol, u0p, C.byref(info))
         check(rc, "nbx_fem_solve", ok=(0, 1))
         return info.as_dict()
 

@@ -42,6 +42,13 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
+class SolveOpts(C.Structure):
+    """nbx_solve_opts_t"""
+    _fields_ = [("mixed_variant", C.c_int32), ("cheb_degree", C.c_uint32),
+                ("cheb_ratio", C.c_double), ("inner_rtol", C.c_double),
+                ("reliable_delta", C.c_double)]
+
+
 class Graph(C.Structure):
     """nb_graph_t (headers/nb/graph_bot/graph.h:25-31)."""
     _fields_ = [("N", C.c_uint32), ("N_adj", C.POINTER(C.c_uint32)),
@@ -119,6 +126,9 @@ SIGS = {
     "nbx_last_error": (C.c_char_p, []),
     "nbx_device_info": (I32, [C.c_char_p, U32, C.POINTER(U32), C.POINTER(F64)]),
     "nbx_sparse_solve": (I32, [VP, f64p, f64p, I32, F64, U32, F64, C.POINTER(SolveInfo)]),
+    "nbx_sparse_solve_ex": (I32, [VP, f64p, f64p, I32, F64, U32, C.POINTER(SolveOpts),
+                                  C.POINTER(SolveInfo)]),
+    "nbx_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "nbx_sparse_spmv_dev": (I32, [VP, VP, VP]),
     "nbx_sparse_is_blocked": (I32, [VP]),
     "nbx_sparse_export_csr": (I32, [VP, u32o, u32o, f64o]),
@@ -130,6 +140,8 @@ SIGS = {
     "nbx_fem_set_bconditions": (I32, [VP, u32p, u32p, u32p]
                                 + [U32, u32p, u8p, f64p] * 4 + [i32o, f64o, F64]),
     "nbx_fem_solve": (I32, [VP, I32, F64, U32, F64, f64o, C.POINTER(SolveInfo)]),
+    "nbx_fem_solve_ex": (I32, [VP, I32, F64, U32, C.POINTER(SolveOpts), f64o,
+                               C.POINTER(SolveInfo)]),
     "nbx_fem_get_displacement": (I32, [VP, f64p]),
     "nbx_fem_get_system": (I32, [VP, u32p, u32p, f64p, f64p]),
     "nbx_fem_strain_stress": (I32, [VP, f64p, f64p]),
