@@ -333,9 +333,44 @@ int nbx_fem_compliance(nbx_fem_t *sys, const double *rho, double penal,
 int nbx_fem_adjoint(nbx_fem_t *sys, const double *rhs, double *lambda,
 		    int mode, double rtol, uint32_t max_iter,
 		    nbx_solve_info_t *info);
+/* out_e = a_e' K0_e b_e per element, K0_e the element matrix at unit
+ * modulus; a or b NULL means the current displacement.  With a = lambda
+ * of nbx_fem_adjoint for dL/du, dL/drho_e = -E'(rho_e) out_e.             */
+int nbx_fem_elem_cross(nbx_fem_t *sys, const double *a, const double *b,
+		       double *out);
 /* Device time of the last SpMV-dominated kernel sequence (ms). */
 int nbx_fem_spmv_bench(nbx_fem_t *sys, int precision, uint32_t reps,
 		       double *ms_per_spmv);
+
+/* ---- SIMP topology optimisation on a device system ---------------------- */
+/* No counterpart in the reference (it has no topology optimisation).  The
+ * density-filter 88-line algorithm (Andreassen et al., Struct. Multidisc.
+ * Optim. 43, 2011): filter H_ej = max(0, rmin - |c_e - c_j|) over element
+ * centroids, E_e = Emin + xPhys_e^penal (E0 - Emin), OC update with move
+ * limit and bisection on the volume multiplier; volume is area-weighted. */
+typedef struct nbx_simp_s nbx_simp_t;
+
+/* The system must have been assembled (nu, thickness) and given its
+ * boundary conditions; x0 (nullable) is the start design, else volfrac.  */
+int nbx_simp_create(nbx_simp_t **out, nbx_fem_t *sys, double rmin,
+		    double volfrac, double penal, double E0, double Emin,
+		    double move, const double *x0);
+void nbx_simp_destroy(nbx_simp_t *simp);
+/* One iteration: reassemble with E(xPhys), re-apply the boundary
+ * conditions, solve (mode, relative tol, warm start from the previous
+ * displacement), compliance of the current design, sensitivities, filter
+ * and OC update.  change = max |x_new - x|.                             */
+int nbx_simp_iterate(nbx_simp_t *simp, int mode, double tol,
+		     uint32_t max_iter, const nbx_solve_opts_t *opts,
+		     double *compliance, double *change,
+		     nbx_solve_info_t *info);
+/* design x, filtered densities xPhys, filtered sensitivities of the last
+ * iteration, bisection steps of the last OC update (each nullable).     */
+int nbx_simp_get(const nbx_simp_t *simp, double *x, double *xphys,
+		 double *dc, uint32_t *oc_steps);
+/* The filter as CSR (ascending columns per row) and its row sums.        */
+int nbx_simp_filter(const nbx_simp_t *simp, uint64_t *n_entries,
+		    uint32_t *Hptr, uint32_t *Hidx, double *Hval, double *Hs);
 
 #ifdef __cplusplus
 }

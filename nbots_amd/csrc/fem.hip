@@ -476,11 +476,12 @@ __global__ void k_strain_stress(const double2* __restrict__ nod, const uint32_t*
     }
 }
 
-// ce_e = u_e' K0_e u_e with K0_e the unit-modulus element matrix;
-// dc_e = -p rho^(p-1) (E0 - Emin) ce_e; C = sum_e E_e ce_e.
+// ce_e = w_e' K0_e u_e with K0_e the unit-modulus element matrix (w = u for
+// compliance); dc_e = -p rho^(p-1) (E0 - Emin) ce_e; C = sum_e E_e ce_e.
 __global__ __launch_bounds__(BLOCK) void k_compliance(const double2* __restrict__ nod,
                                                       const uint32_t* __restrict__ conn,
-                                                      const double* __restrict__ u, double nu,
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ w, double nu,
                                                       double thickness, const double* __restrict__ rho,
                                                       double penal, double E0, double Emin,
                                                       const double* __restrict__ elemE, uint32_t ne,
@@ -494,10 +495,12 @@ __global__ __launch_bounds__(BLOCK) void k_compliance(const double2* __restrict_
         double detJ, dx[3], dy[3];
         double c = 0.0;
         if (t3_geom(nod, n3, detJ, dx, dy)) {
-            double ue[6];
+            double ue[6], we[6];
             for (int i = 0; i < 3; ++i) {
                 ue[2 * i] = u[2 * n3[i]];
                 ue[2 * i + 1] = u[2 * n3[i] + 1];
+                we[2 * i] = w[2 * n3[i]];
+                we[2 * i + 1] = w[2 * n3[i] + 1];
             }
             for (int r = 0; r < 6; ++r) {
                 double t = 0.0;
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(BLOCK) void k_compliance(const double2* __restrict_
                     t += kb[2 * (r & 1)] * ue[2 * q];
                     t += kb[2 * (r & 1) + 1] * ue[2 * q + 1];
                 }
-                c += ue[r] * t;
+                c += we[r] * t;
             }
         }
         if (ce) ce[e] = c;
@@ -643,8 +646,8 @@ int fem_create(FemSys& F, uint32_t nn, const double* nod, uint32_t ne, const uin
 void fem_free(FemSys& F) {
     mat_free(F.K);
     ws_free(F.ws);
-    void* ps[] = {F.nod, F.conn, F.n2e_ptr, F.n2e, F.F, F.u, F.bc_rank, F.bc_vf, F.bc_vl,
-                  F.elemE, F.enabled, F.err, F.rho};
+    void* ps[] = {F.nod, F.conn, F.n2e_ptr, F.n2e, F.F, F.F_load, F.u, F.aux, F.aux2, F.bc_rank,
+                  F.bc_vf, F.bc_vl, F.elemE, F.enabled, F.err, F.rho, F.dc, F.ce};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (F.s) (void)hipStreamDestroy(F.s);
@@ -727,6 +730,10 @@ int fem_apply_bc(FemSys& F, const BcPlan& plan) {
         (void)hipFree(aptr);
         (void)hipFree(add);
     }
+    // the load before elimination, kept so that a reassembled K (new moduli)
+    // can take the same boundary conditions without the host
+    if (!F.F_load) NBX_HIP(hipMalloc(&F.F_load, 2 * (size_t)F.nn * 8 + 8));
+    NBX_HIP(hipMemcpyAsync(F.F_load, F.F, 2 * (size_t)F.nn * 8, hipMemcpyDeviceToDevice, s));
     // Dirichlet: per-dof rank of first elimination, first and last values
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(2 * (size_t)F.nn)), dim3(BLOCK), 0, s, F.bc_rank,
                        2 * (size_t)F.nn, FREE);
@@ -757,6 +764,25 @@ int fem_apply_bc(FemSys& F, const BcPlan& plan) {
     return mat_extract_diag(F.K, s);
 }
 
+// The boundary conditions of the last fem_apply_bc on a freshly assembled K:
+// the same kernels, from the saved pre-elimination load.
+int fem_reapply_bc(FemSys& F) {
+    const hipStream_t s = F.s;
+    if (!F.F_load) {
+        set_last_error("no boundary conditions to re-apply");
+        return NBX_ERR_ARG;
+    }
+    NBX_HIP(hipMemcpyAsync(F.F, F.F_load, 2 * (size_t)F.nn * 8, hipMemcpyDeviceToDevice, s));
+    if (F.n_constrained && F.nn) {
+        hipLaunchKernelGGL(k_bc_rhs, dim3(grid_for(F.nn)), dim3(BLOCK), 0, s, F.K.s, F.bc_rank,
+                           F.bc_vf, F.bc_vl, F.F);
+        hipLaunchKernelGGL(k_bc_matrix, dim3(grid_for(F.nn)), dim3(BLOCK), 0, s, F.K.s, F.bc_rank);
+        NBX_HIP(hipGetLastError());
+    }
+    F.bc_applied = true;
+    return mat_extract_diag(F.K, s);
+}
+
 int fem_zero_constrained(FemSys& F, double* x) {
     if (F.nn)
         hipLaunchKernelGGL(k_zero_constrained, dim3(grid_for(2 * (size_t)F.nn)), dim3(BLOCK), 0, F.s, x,
@@ -778,12 +804,23 @@ int fem_compliance(FemSys& F, const double* rho, double penal, double E0, double
     double* out = F.ws.v64[5] + F.ws.n;
     unsigned g = grid_for(F.ne);
     if (g > (unsigned)MAX_GRID) g = MAX_GRID;
-    hipLaunchKernelGGL(k_compliance, dim3(g), dim3(BLOCK), 0, F.s, F.nod, F.conn, F.u, F.P.nu,
+    hipLaunchKernelGGL(k_compliance, dim3(g), dim3(BLOCK), 0, F.s, F.nod, F.conn, F.u, F.u, F.P.nu,
                        F.P.thickness, rho, penal, E0, Emin, F.P.elemE, F.ne, ce, dc, F.ws.partials);
     hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(FIN), 0, F.s, F.ws.partials, g, out);
     NBX_HIP(hipGetLastError());
     NBX_HIP(hipMemcpyAsync(C_host, out, sizeof(double), hipMemcpyDeviceToHost, F.s));
     NBX_HIP(hipStreamSynchronize(F.s));
+    return NBX_OK;
+}
+
+// out_e = a_e' K0_e b_e (device vectors of 2 nn values)
+int fem_elem_cross(FemSys& F, const double* a, const double* b, double* out) {
+    unsigned g = grid_for(F.ne);
+    if (g > (unsigned)MAX_GRID) g = MAX_GRID;
+    hipLaunchKernelGGL(k_compliance, dim3(g), dim3(BLOCK), 0, F.s, F.nod, F.conn, b, a, F.P.nu,
+                       F.P.thickness, nullptr, 3.0, 1.0, 0.0, nullptr, F.ne, out, nullptr,
+                       F.ws.partials);
+    NBX_HIP(hipGetLastError());
     return NBX_OK;
 }
 

@@ -36,6 +36,7 @@ struct FemSys {
     uint32_t* n2e = nullptr;           // [3 ne], ascending per node
     DevMatrix K;
     double* F = nullptr;               // [2 nn]
+    double* F_load = nullptr;          // [2 nn] F before Dirichlet elimination
     double* u = nullptr;               // [2 nn]
     double* aux = nullptr;             // [2 nn] scratch for adjoint / bench
     double* aux2 = nullptr;
@@ -60,10 +61,12 @@ int fem_assemble(FemSys& F, double E, double nu, double thickness, const double*
 int fem_assemble_device(FemSys& F);
 int fem_body_force(FemSys& F, int self_weight, double density, const double* gravity);
 int fem_apply_bc(FemSys& F, const BcPlan& plan);
+int fem_reapply_bc(FemSys& F);
 int fem_zero_constrained(FemSys& F, double* x);
 int fem_strain_stress(FemSys& F, double* strain, double* stress);
 int fem_compliance(FemSys& F, const double* rho, double penal, double E0, double Emin, double* ce,
                    double* dc, double* C_host);
+int fem_elem_cross(FemSys& F, const double* a, const double* b, double* out);
 int stress_from_strain(uint32_t ne, double E, double nu, const uint8_t* enabled_host,
                        const double* strain_host, double* stress_host);
 int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* val);

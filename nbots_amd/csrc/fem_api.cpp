@@ -14,6 +14,7 @@
 
 #include "../../include/nbots_amd.h"
 #include "fem.hpp"
+#include "simp.hpp"
 
 struct nbx_fem_s {
     nbx::FemSys sys;
@@ -429,10 +430,109 @@ int nbx_fem_adjoint(nbx_fem_t* f, const double* rhs, double* lambda, int mode, d
     return rc;
 }
 
+int nbx_fem_elem_cross(nbx_fem_t* f, const double* a, const double* b, double* out) {
+    nbx::FemSys& S = f->sys;
+    const size_t bytes = 2 * (size_t)S.nn * 8;
+    if (!S.aux) NBX_HIP(hipMalloc(&S.aux, bytes + 8));
+    if (!S.aux2) NBX_HIP(hipMalloc(&S.aux2, bytes + 8));
+    if (!S.ce) NBX_HIP(hipMalloc(&S.ce, (size_t)S.ne * 8 + 8));
+    double* da = S.u;
+    double* db = S.u;
+    if (a) {
+        NBX_HIP(hipMemcpyAsync(S.aux, a, bytes, hipMemcpyHostToDevice, S.s));
+        da = S.aux;
+    }
+    if (b) {
+        NBX_HIP(hipMemcpyAsync(S.aux2, b, bytes, hipMemcpyHostToDevice, S.s));
+        db = S.aux2;
+    }
+    NBX_TRY(nbx::fem_elem_cross(S, da, db, S.ce));
+    NBX_HIP(hipMemcpyAsync(out, S.ce, (size_t)S.ne * 8, hipMemcpyDeviceToHost, S.s));
+    NBX_HIP(hipStreamSynchronize(S.s));
+    return nbx::NBX_OK;
+}
+
 int nbx_fem_spmv_bench(nbx_fem_t* f, int precision, uint32_t reps, double* ms_per_spmv) {
     nbx::FemSys& S = f->sys;
     if (precision == 32 && !S.K.has_val32) NBX_TRY(nbx::mat_make_scaled32(S.K, S.s));
     return nbx::spmv_bench(S.K, precision, reps, S.ws, S.s, ms_per_spmv);
+}
+
+// ---- SIMP loop ----------------------------------------------------------------
+struct nbx_simp_s {
+    nbx_fem_t* fem;
+    nbx::Simp o;
+};
+
+int nbx_simp_create(nbx_simp_t** out, nbx_fem_t* f, double rmin, double volfrac, double penal,
+                    double E0, double Emin, double move, const double* x0) {
+    if (!out || !f) {
+        nbx::set_last_error("nbx_simp_create: null argument");
+        return nbx::NBX_ERR_ARG;
+    }
+    *out = nullptr;
+    nbx_simp_t* h = new (std::nothrow) nbx_simp_s;
+    if (!h) {
+        nbx::set_last_error("out of host memory");
+        return nbx::NBX_ERR_ARG;
+    }
+    h->fem = f;
+    const int rc = nbx::simp_create(h->o, f->sys, rmin, volfrac, penal, E0, Emin, move, x0);
+    if (rc) {
+        nbx::simp_free(h->o);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return nbx::NBX_OK;
+}
+
+void nbx_simp_destroy(nbx_simp_t* h) {
+    if (!h) return;
+    nbx::simp_free(h->o);
+    delete h;
+}
+
+int nbx_simp_iterate(nbx_simp_t* h, int mode, double tol, uint32_t max_iter,
+                     const nbx_solve_opts_t* opts, double* compliance, double* change,
+                     nbx_solve_info_t* info) {
+    nbx::FemSys& S = h->fem->sys;
+    nbx_solve_info_t tmp;
+    if (!info) info = &tmp;
+    // K(E(xPhys)) with the system's boundary conditions; the solve starts
+    // from the previous displacement
+    NBX_TRY(nbx::simp_moduli(h->o, S));
+    NBX_TRY(nbx::fem_assemble_device(S));
+    NBX_TRY(nbx::fem_reapply_bc(S));
+    const int rc = fem_solve(S, mode, tol, max_iter ? max_iter : 20 * 2 * S.nn, mixed_opts(opts),
+                             info);
+    if (rc > 1) return rc;
+    double C = 0.0;
+    NBX_TRY(nbx::fem_compliance(S, h->o.xphys, h->o.penal, h->o.E0, h->o.Emin, S.ce, S.dc, &C));
+    if (compliance) *compliance = C;
+    NBX_TRY(nbx::simp_update(h->o, S, change));
+    return rc;
+}
+
+int nbx_simp_get(const nbx_simp_t* h, double* x, double* xphys, double* dc, uint32_t* oc_steps) {
+    const nbx::FemSys& S = h->fem->sys;
+    const size_t bytes = (size_t)S.ne * 8;
+    if (x) NBX_HIP(hipMemcpy(x, h->o.x, bytes, hipMemcpyDeviceToHost));
+    if (xphys) NBX_HIP(hipMemcpy(xphys, h->o.xphys, bytes, hipMemcpyDeviceToHost));
+    if (dc) NBX_HIP(hipMemcpy(dc, h->o.dcf, bytes, hipMemcpyDeviceToHost));
+    if (oc_steps) *oc_steps = h->o.last_oc_steps;
+    return nbx::NBX_OK;
+}
+
+int nbx_simp_filter(const nbx_simp_t* h, uint64_t* n_entries, uint32_t* Hptr, uint32_t* Hidx,
+                    double* Hval, double* Hs) {
+    const nbx::Simp& o = h->o;
+    if (n_entries) *n_entries = o.h_idx.size();
+    if (Hptr) std::copy(o.h_ptr.begin(), o.h_ptr.end(), Hptr);
+    if (Hidx) std::copy(o.h_idx.begin(), o.h_idx.end(), Hidx);
+    if (Hval) std::copy(o.h_val.begin(), o.h_val.end(), Hval);
+    if (Hs) std::copy(o.h_hs.begin(), o.h_hs.end(), Hs);
+    return nbx::NBX_OK;
 }
 
 // ---- drop-in objects ----------------------------------------------------------

@@ -146,10 +146,72 @@ class Fem:
         check(rc, "nbx_fem_adjoint", ok=(0, 1))
         return lam, info.as_dict()
 
+    def elem_cross(self, a=None, b=None):
+        """a_e' K0_e b_e per element (None = the current displacement)."""
+        out = np.zeros(self.N_elem)
+        check(self.L.nbx_fem_elem_cross(self.h, None if a is None else _c(a, np.float64),
+                                        None if b is None else _c(b, np.float64), out),
+              "nbx_fem_elem_cross")
+        return out
+
     def spmv_ms(self, precision=64, reps=20):
         ms = C.c_double()
         check(self.L.nbx_fem_spmv_bench(self.h, precision, reps, C.byref(ms)), "nbx_fem_spmv_bench")
         return ms.value
+
+
+class Simp:
+    """SIMP topology optimisation on a device system (nbx_simp_*): density
+    filter of radius rmin, E = Emin + xPhys^penal (E0 - Emin), OC update."""
+
+    def __init__(self, fem, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2,
+                 x0=None):
+        self.L = fem.L
+        self.fem = fem
+        self.N_elem = fem.N_elem
+        h = C.c_void_p()
+        check(self.L.nbx_simp_create(C.byref(h), fem.h, rmin, volfrac, penal, E0, Emin, move,
+                                     None if x0 is None else _c(x0, np.float64)),
+              "nbx_simp_create")
+        self.h = h.value
+
+    def destroy(self):
+        if self.h:
+            self.L.nbx_simp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def iterate(self, mode=SOLVE_MIXED, tol=1e-8, max_iter=0, opts=None):
+        """One SIMP iteration; returns (compliance, change, solve info)."""
+        Cv, ch, info = C.c_double(), C.c_double(), SolveInfo()
+        rc = self.L.nbx_simp_iterate(self.h, mode, tol, max_iter,
+                                     None if opts is None else C.addressof(opts),
+                                     C.byref(Cv), C.byref(ch), C.byref(info))
+        check(rc, "nbx_simp_iterate", ok=(0, 1))
+        return Cv.value, ch.value, info.as_dict()
+
+    def state(self):
+        """(x, xPhys, filtered dc of the last iteration, OC bisection steps)"""
+        x, xp, dc = (np.zeros(self.N_elem) for _ in range(3))
+        steps = C.c_uint32()
+        check(self.L.nbx_simp_get(self.h, x, xp, dc, C.byref(steps)), "nbx_simp_get")
+        return x, xp, dc, steps.value
+
+    def filter(self):
+        """H as CSR (Hptr, Hidx, Hval) and Hs."""
+        n = C.c_uint64()
+        check(self.L.nbx_simp_filter(self.h, C.byref(n), None, None, None, None), "nbx_simp_filter")
+        ptr = np.zeros(self.N_elem + 1, np.uint32)
+        idx = np.zeros(n.value, np.uint32)
+        val = np.zeros(n.value)
+        hs = np.zeros(self.N_elem)
+        check(self.L.nbx_simp_filter(self.h, None, ptr, idx, val, hs), "nbx_simp_filter")
+        return ptr, idx, val, hs
 
 
 def compute_2D_solid_mechanics(p, enabled=None, self_weight=False, gravity=(0.0, 0.0),

@@ -147,7 +147,14 @@ SIGS = {
     "nbx_fem_strain_stress": (I32, [VP, f64p, f64p]),
     "nbx_fem_compliance": (I32, [VP, f64o, F64, F64, F64, C.POINTER(F64), f64o, f64o]),
     "nbx_fem_adjoint": (I32, [VP, f64p, f64p, I32, F64, U32, C.POINTER(SolveInfo)]),
+    "nbx_fem_elem_cross": (I32, [VP, f64o, f64o, f64p]),
     "nbx_fem_spmv_bench": (I32, [VP, I32, U32, C.POINTER(F64)]),
+    "nbx_simp_create": (I32, [C.POINTER(VP), VP, F64, F64, F64, F64, F64, F64, f64o]),
+    "nbx_simp_destroy": (None, [VP]),
+    "nbx_simp_iterate": (I32, [VP, I32, F64, U32, C.c_void_p, C.POINTER(F64), C.POINTER(F64),
+                               C.POINTER(SolveInfo)]),
+    "nbx_simp_get": (I32, [VP, f64o, f64o, f64o, C.POINTER(U32)]),
+    "nbx_simp_filter": (I32, [VP, C.POINTER(U64), u32o, u32o, f64o, f64o]),
 }
 
 
