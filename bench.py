@@ -34,9 +34,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from nbots_amd.probe import spmv_bytes  # noqa: E402  (algorithmic bytes of one SpMV)
+
 METRIC = ("DOF·CG-iter/s + time-to-1e-8 residual (FP32+IR vs FP64); "
           "ms per SIMP iteration")
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_pcg_A measured by two rocprofv3 --pmc passes
+# (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
+# 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md
+PMC_TRAFFIC = {"c3": 599187989}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
                                       "right-edge shear traction"),
@@ -47,15 +53,6 @@ CONFIGS = {
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def spmv_bytes(n_nodes, n_blocks, prec):
-    """Algorithmic bytes of one 2x2-block sliced-ELL SpMV fused with its dot
-    product: each stored block's values (4 x sv) and node column (4 B) once,
-    row lengths (1 B/node) and slice offsets (4 B/64 nodes), the input
-    vector read once (2 x sv per node) and the output written once."""
-    sv = 8 if prec == 64 else 4
-    return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sv
 
 
 def pcg_iter_bytes(n_nodes, n_blocks, prec):
@@ -173,6 +170,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mixed", default="cheb", choices=["cheb", "restart", "reliable"])
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-max-iter", type=int, default=200000)
     ap.add_argument("--cheb-degree", type=int, default=8)
     ap.add_argument("--cheb-ratio", type=float, default=100.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -234,7 +232,7 @@ def main():
     b64 = spmv_bytes(n_nodes, n_valid, 64)
     achieved = b64 / (spmv_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=PMC_TRAFFIC.get(args.config),
                     kernel="k_pcg_A<OpS2> (FP64 2x2-block sliced-ELL SpMV + p.w partials)",
                     bytes_per_launch=b64, us_per_launch=round(spmv_ms * 1e3, 2))
 
@@ -244,11 +242,11 @@ def main():
         variant = dict(cheb=MIXED_CHEB_FCG, restart=MIXED_CG_RESTART,
                        reliable=MIXED_CG_RELIABLE)[args.mixed]
         opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio)
-        f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, opts=opts)
+        f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
         mtimes, minfos = [], []
         for s in range(2):
             t0 = time.perf_counter()
-            info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, opts=opts)
+            info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
             mtimes.append(time.perf_counter() - t0)
             minfos.append(info)
             log(f"[bench] mixed ({args.mixed}) {s}: {info['iters']} it, {info['outer']} inner, "

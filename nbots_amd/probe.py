@@ -1,0 +1,68 @@
+"""Counter probe: a short FP64 Jacobi-PCG run on the cantilever system and a
+plain device copy of known size, for hardware-counter runs of rocprofv3.
+
+    rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'k_pcg_A|elementwise' \\
+        -d <dir> -- python -m nbots_amd.probe --nx 2048 --ny 1024 --iters 50
+
+The SpMV kernel of every CG iteration (k_pcg_A) streams the 2x2-block
+matrix once and gathers x; the copy reads and writes exactly --copy-mib MiB,
+so the ratio of the copy's counter reading to its size calibrates the
+counter for the SpMV.  The probe prints the algorithmic bytes of one SpMV
+launch for comparison.  No timing is taken here.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def spmv_bytes(n_nodes: int, n_blocks: int, prec: int) -> int:
+    """Algorithmic HBM bytes of one 2x2-block sliced-ELL SpMV: per valid
+    block its column (4 B) and 4 values; per node its row length (1 B) and
+    the x read and y write of 2 values; per 64-node slice its offset."""
+    sv = 8 if prec == 64 else 4
+    return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sv
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=20261202)
+    args = ap.parse_args()
+
+    import torch
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_FP64
+
+    p = synth.cantilever_traction(args.nx, args.ny, seed=args.seed)
+    f = Fem.from_problem(p)
+    N, nnz, nblk = f.sizes()
+    n_nodes, n_valid = N // 2, nnz // 4
+    print(f"[probe] {args.nx}x{args.ny} cantilever: {N} DOF, {n_valid} blocks, "
+          f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64)", flush=True)
+    info = f.solve(SOLVE_FP64, tol=0.0, max_iter=args.iters)
+    print(f"[probe] FP64 PCG: {info['iters']} iterations", flush=True)
+    f.destroy()
+
+    # b = 2 a: an elementwise kernel that reads and writes every byte once
+    # (a same-type copy_ would go to the runtime's blit path instead)
+    n = args.copy_mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(f"[probe] elementwise b = 2a: 3 launches of {n * 4} bytes read + {n * 4} bytes written",
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()
