@@ -171,8 +171,9 @@ def main():
     ap.add_argument("--mixed", default="cheb", choices=["cheb", "restart", "reliable"])
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
-    ap.add_argument("--cheb-degree", type=int, default=8)
-    ap.add_argument("--cheb-ratio", type=float, default=100.0)
+    ap.add_argument("--cheb-degree", type=int, default=None, help="library default 16")
+    ap.add_argument("--cheb-ratio", type=float, default=None, help="library default 3000")
+    ap.add_argument("--precond-bits", type=int, default=16, choices=[16, 32])
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--simp-iters", type=int, default=50)
@@ -241,7 +242,8 @@ def main():
     if not args.no_mixed:
         variant = dict(cheb=MIXED_CHEB_FCG, restart=MIXED_CG_RESTART,
                        reliable=MIXED_CG_RELIABLE)[args.mixed]
-        opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio)
+        opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+                          precond_bits=args.precond_bits)
         f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
         mtimes, minfos = [], []
         for s in range(2):
@@ -255,13 +257,15 @@ def main():
         t_mixed = float(np.median(mtimes))
         t_fp64 = float(np.median(times))
         ms32 = f.spmv_ms(32, 20)
+        ms16 = f.spmv_ms(16, 20)
         ms64 = f.spmv_ms(64, 20)
         extra = dict(
             time_to_1e8_fp64_s=round(t_fp64, 4),
             time_to_1e8_mixed_s=round(t_mixed, 4),
             mixed_speedup_vs_fp64=round(t_fp64 / t_mixed, 3),
             mixed_variant=args.mixed,
-            mixed_solver=("FP64 flexible CG, FP32 Chebyshev-Jacobi preconditioner "
+            mixed_solver=("FP64 flexible CG, Chebyshev-Jacobi preconditioner in FP32 arithmetic on "
+                          f"{'fp16' if opts.precond_bits == 16 else 'FP32'} matrix values "
                           f"(degree {opts.cheb_degree}, [lmax/{opts.cheb_ratio:g}, lmax])"
                           if variant == MIXED_CHEB_FCG else
                           "FP32 CG sweeps (inner rtol 1e-4) in FP64 iterative refinement"
@@ -274,6 +278,8 @@ def main():
             spmv64_gbs=round(b64 / (ms64 * 1e-3) / 1e9, 1),
             spmv32_us=round(ms32 * 1e3, 2),
             spmv32_gbs=round(spmv_bytes(n_nodes, n_valid, 32) / (ms32 * 1e-3) / 1e9, 1),
+            spmv16_us=round(ms16 * 1e3, 2),
+            spmv16_gbs=round(spmv_bytes(n_nodes, n_valid, 16) / (ms16 * 1e-3) / 1e9, 1),
             fp64_iter_us=round(1e6 * t_fp64 / iters[-1], 1),
             fp64_iter_gbs=round(pcg_iter_bytes(n_nodes, n_valid, 64) * iters[-1] / t_fp64 / 1e9, 1))
     cpu = None
@@ -286,7 +292,8 @@ def main():
     simp = {}
     if not args.no_simp:
         simp = simp_bench(args, solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree,
-                                           cheb_ratio=args.cheb_ratio))
+                                           cheb_ratio=args.cheb_ratio,
+                                           precond_bits=args.precond_bits))
     line = dict(
         metric=METRIC, value=value, unit="DOF*CG-iter/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True,

@@ -221,15 +221,18 @@ enum {
 	NBX_MIXED_CG_RESTART = 0,   /* FP32 CG sweeps to inner_rtol inside FP64
 				       iterative refinement                   */
 	NBX_MIXED_CG_RELIABLE = 1,  /* FP32 CG, FP64 reliable updates          */
-	NBX_MIXED_CHEB_FCG = 2      /* FP64 flexible CG, FP32 Chebyshev-Jacobi
-				       preconditioner (default)               */
+	NBX_MIXED_CHEB_FCG = 2      /* FP64 flexible CG, Chebyshev-Jacobi
+				       preconditioner in FP32 arithmetic on
+				       fp16 or FP32 matrix values (default)  */
 };
 typedef struct {
 	int32_t mixed_variant;   /* NBX_MIXED_*                               */
-	uint32_t cheb_degree;    /* SpMVs per preconditioner application (8)  */
-	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (100) */
+	uint32_t cheb_degree;    /* SpMVs per preconditioner application (16) */
+	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (3000) */
 	double inner_rtol;       /* CG_RESTART inner tolerance (1e-4)         */
 	double reliable_delta;   /* CG_RELIABLE update threshold (0.1)        */
+	int32_t precond_bits;    /* CHEB_FCG matrix values: 16 (default) or 32 */
+	int32_t reserved;        /* 0                                          */
 } nbx_solve_opts_t;
 /* Fills *opts with the defaults above. */
 void nbx_solve_opts_default(nbx_solve_opts_t *opts);

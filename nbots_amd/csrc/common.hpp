@@ -90,6 +90,7 @@ struct Sell2 {
     uint32_t* bcol = nullptr;      // [n_blocks]
     double* val = nullptr;         // [4 * n_blocks]
     float* val32 = nullptr;        // [4 * n_blocks], Jacobi-scaled FP32 copy
+    _Float16* val16 = nullptr;     // [4 * n_blocks], Jacobi-scaled fp16 copy
 };
 
 // index of entry (i, j) of stored block idx in Sell2::val

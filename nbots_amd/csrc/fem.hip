@@ -683,7 +683,7 @@ int fem_assemble_device(FemSys& F) {
     NBX_HIP(hipGetLastError());
     unsigned herr = 0;
     NBX_TRY(read_err(F.err, s, &herr));
-    F.K.has_val32 = false;
+    mat_scaled_stale(F.K);
     F.bc_applied = false;
     if (herr & 4u) {
         set_last_error("distorted element (detJ < 0)");

@@ -14,6 +14,7 @@
 
 #include "../../include/nbots_amd.h"
 #include "fem.hpp"
+#include "opts.hpp"
 #include "simp.hpp"
 
 struct nbx_fem_s {
@@ -36,27 +37,6 @@ struct nb_material_s {
 
 namespace {
 
-nbx::MixedOpts mixed_opts(double inner_rtol) {
-    nbx::MixedOpts o;
-    if (inner_rtol > 0.0) {
-        o.variant = nbx::CG_RESTART;
-        o.inner_rtol = inner_rtol;
-    } else if (inner_rtol < 0.0) {
-        o.variant = nbx::CG_RELIABLE;
-    }
-    return o;
-}
-
-nbx::MixedOpts mixed_opts(const nbx_solve_opts_t* p) {
-    nbx::MixedOpts o;
-    if (!p) return o;
-    o.variant = (nbx::MixedVariant)p->mixed_variant;
-    if (p->cheb_degree) o.cheb_degree = p->cheb_degree;
-    if (p->cheb_ratio > 1.0) o.cheb_ratio = p->cheb_ratio;
-    if (p->inner_rtol > 0.0) o.inner_rtol = p->inner_rtol;
-    if (p->reliable_delta > 0.0) o.delta = p->reliable_delta;
-    return o;
-}
 
 struct BcAtom {
     uint32_t id;
@@ -235,7 +215,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
         info->fp64_iters = st.fp64_iters;
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
-        if (!S.K.has_val32) NBX_TRY(nbx::mat_make_scaled32(S.K, S.s));
+        NBX_TRY(nbx::mat_prepare_mixed(S.K, S.s, mo));
         status = nbx::solve_mixed(S.K, S.F, S.u, tol, mo, max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
@@ -351,16 +331,20 @@ void nbx_solve_opts_default(nbx_solve_opts_t* o) {
     o->cheb_ratio = d.cheb_ratio;
     o->inner_rtol = d.inner_rtol;
     o->reliable_delta = d.delta;
+    o->precond_bits = d.precond_bits;
+    o->reserved = 0;
 }
 
 int nbx_fem_solve(nbx_fem_t* f, int mode, double tol, uint32_t max_iter, double inner_rtol,
                   const double* u_init, nbx_solve_info_t* info) {
-    return fem_solve_entry(f, mode, tol, max_iter, mixed_opts(inner_rtol), u_init, info);
+    return fem_solve_entry(f, mode, tol, max_iter, nbx::mixed_opts(inner_rtol), u_init, info);
 }
 
 int nbx_fem_solve_ex(nbx_fem_t* f, int mode, double tol, uint32_t max_iter,
                      const nbx_solve_opts_t* opts, const double* u_init, nbx_solve_info_t* info) {
-    return fem_solve_entry(f, mode, tol, max_iter, mixed_opts(opts), u_init, info);
+    nbx::MixedOpts mo;
+    NBX_TRY(nbx::mixed_opts(opts, &mo));
+    return fem_solve_entry(f, mode, tol, max_iter, mo, u_init, info);
 }
 
 int nbx_fem_get_displacement(const nbx_fem_t* f, double* u) {
@@ -454,7 +438,7 @@ int nbx_fem_elem_cross(nbx_fem_t* f, const double* a, const double* b, double* o
 
 int nbx_fem_spmv_bench(nbx_fem_t* f, int precision, uint32_t reps, double* ms_per_spmv) {
     nbx::FemSys& S = f->sys;
-    if (precision == 32 && !S.K.has_val32) NBX_TRY(nbx::mat_make_scaled32(S.K, S.s));
+    if (precision == 32 || precision == 16) NBX_TRY(nbx::mat_make_scaled(S.K, S.s, precision));
     return nbx::spmv_bench(S.K, precision, reps, S.ws, S.s, ms_per_spmv);
 }
 
@@ -499,13 +483,14 @@ int nbx_simp_iterate(nbx_simp_t* h, int mode, double tol, uint32_t max_iter,
     nbx::FemSys& S = h->fem->sys;
     nbx_solve_info_t tmp;
     if (!info) info = &tmp;
+    nbx::MixedOpts mo;
+    NBX_TRY(nbx::mixed_opts(opts, &mo));
     // K(E(xPhys)) with the system's boundary conditions; the solve starts
     // from the previous displacement
     NBX_TRY(nbx::simp_moduli(h->o, S));
     NBX_TRY(nbx::fem_assemble_device(S));
     NBX_TRY(nbx::fem_reapply_bc(S));
-    const int rc = fem_solve(S, mode, tol, max_iter ? max_iter : 20 * 2 * S.nn, mixed_opts(opts),
-                             info);
+    const int rc = fem_solve(S, mode, tol, max_iter ? max_iter : 20 * 2 * S.nn, mo, info);
     if (rc > 1) return rc;
     double C = 0.0;
     NBX_TRY(nbx::fem_compliance(S, h->o.xphys, h->o.penal, h->o.E0, h->o.Emin, S.ce, S.dc, &C));

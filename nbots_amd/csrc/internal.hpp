@@ -32,8 +32,10 @@ struct DevMatrix {
     Sell2 s;
     DevCsr c;
     double* diag = nullptr;    // [n] diagonal A(i,i) (0.0 if not stored)
-    double* dis = nullptr;     // [n] 1/sqrt(diag) for the scaled FP32 copy
+    double* dis = nullptr;     // [n] 1/sqrt(diag) for the scaled copies
+    bool has_scale = false;    // dis and lambda_max are current
     bool has_val32 = false;
+    bool has_val16 = false;
     double lambda_max = 0.0;   // Gershgorin bound of the scaled matrix
 };
 
@@ -72,8 +74,9 @@ struct MixedOpts {
     MixedVariant variant = CHEB_FCG;
     double inner_rtol = 1e-4;
     double delta = 0.1;        // reliable update when r.r < delta^2 max r.r
-    uint32_t cheb_degree = 8;  // SpMVs per preconditioner application
-    double cheb_ratio = 100.0; // Chebyshev interval [lambda_max/ratio, lambda_max]
+    uint32_t cheb_degree = 16;  // SpMVs per preconditioner application
+    double cheb_ratio = 3000.0; // Chebyshev interval [lambda_max/ratio, lambda_max]
+    int precond_bits = 16;     // Chebyshev preconditioner matrix: fp16 or FP32 values
 };
 
 int ws_alloc(Workspace& w, uint32_t n);
@@ -85,7 +88,13 @@ void mat_free(DevMatrix& m);
 int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rowptr, const uint32_t* col,
                    const double* val, bool values_only, hipStream_t s);
 int mat_extract_diag(DevMatrix& m, hipStream_t s);
-int mat_make_scaled32(DevMatrix& m, hipStream_t s);
+// The values changed: the scaled copies and their bound are stale.
+void mat_scaled_stale(DevMatrix& m);
+// Jacobi-scaled copy D^-1/2 A D^-1/2 with 16- or 32-bit values (and the
+// Gershgorin bound lambda_max); a no-op when current.
+int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits);
+// the scaled copy a mixed-precision solve with these options reads
+int mat_prepare_mixed(DevMatrix& m, hipStream_t s, const MixedOpts& o);
 
 int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s);
 

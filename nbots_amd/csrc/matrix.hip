@@ -31,10 +31,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(Op op, const typename Op::V* __r
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) y[u] = op.row((uint32_t)u, x);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_spmv32(Sell2 A, const float2* __restrict__ x,
-                                                  float2* __restrict__ y, uint32_t n) {
+template <int BITS>
+__global__ __launch_bounds__(BLOCK) void k_spmv_scaled(Sell2 A, const float2* __restrict__ x,
+                                                       float2* __restrict__ y, uint32_t n) {
     const Chunk c = my_chunk(n);
-    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) y[i] = row32(A, (uint32_t)i, x);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK)
+        y[i] = row_scaled<BITS>(A, (uint32_t)i, x);
 }
 
 __global__ void k_norm2(const double* __restrict__ v, size_t n, double* partials) {
@@ -86,7 +88,8 @@ __global__ void k_diag_csr(DevCsr A, double* __restrict__ diag) {
     diag[r] = d;
 }
 
-// dis = 1/sqrt(diag); val32 = fl32(a_ij * dis_i * dis_j), computed in FP64.
+// dis = 1/sqrt(diag); val32 = fl32(a_ij * dis_i * dis_j) and
+// val16 = fl16(a_ij * dis_i * dis_j), the products in FP64.
 __global__ void k_dis(const double* __restrict__ diag, double* __restrict__ dis, uint32_t n,
                       unsigned* bad) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -120,17 +123,41 @@ __global__ void k_scale32(Sell2 A, const double* __restrict__ dis) {
     }
 }
 
-// Gershgorin bound of the scaled FP32 matrix: max over rows of sum |a_ij|.
-__global__ void k_gersh(Sell2 A, double* partials) {
+__global__ void k_scale16(Sell2 A, const double* __restrict__ dis) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    const double r0 = dis[2 * v], r1 = dis[2 * v + 1];
+    for (int k = 0; k < len; ++k, idx += SLICE) {
+        const uint32_t w = A.bcol[idx];
+        const double c0 = dis[2 * w], c1 = dis[2 * w + 1];
+        const double* a = A.val + ent(A.n_blocks, idx, 0, 0);
+        const double* b = A.val + ent(A.n_blocks, idx, 1, 0);
+        hv4 o;
+        o.x = (_Float16)(float)(a[0] * r0 * c0);
+        o.y = (_Float16)(float)(a[1] * r0 * c1);
+        o.z = (_Float16)(float)(b[0] * r1 * c0);
+        o.w = (_Float16)(float)(b[1] * r1 * c1);
+        reinterpret_cast<hv4*>(A.val16)[idx] = o;
+    }
+}
+
+// Gershgorin bound of the scaled matrix: max over rows of sum_j |a_ij dis_i dis_j|.
+__global__ void k_gersh(Sell2 A, const double* __restrict__ dis, double* partials) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     double m = 0.0;
     if (v < A.n_nodes) {
         size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+        const double r0 = dis[2 * v], r1 = dis[2 * v + 1];
         double s0 = 0.0, s1 = 0.0;
         for (int k = 0; k < A.rowlen[v]; ++k, idx += SLICE) {
-            const float4 a = reinterpret_cast<const float4*>(A.val32)[idx];
-            s0 += fabs((double)a.x) + fabs((double)a.y);
-            s1 += fabs((double)a.z) + fabs((double)a.w);
+            const uint32_t w = A.bcol[idx];
+            const double c0 = dis[2 * w], c1 = dis[2 * w + 1];
+            const double* a = A.val + ent(A.n_blocks, idx, 0, 0);
+            const double* b = A.val + ent(A.n_blocks, idx, 1, 0);
+            s0 += fabs(a[0] * r0 * c0) + fabs(a[1] * r0 * c1);
+            s1 += fabs(b[0] * r1 * c0) + fabs(b[1] * r1 * c1);
         }
         m = fmax(s0, s1);
     }
@@ -214,7 +241,7 @@ void ws_free(Workspace& w) {
 }
 
 void mat_free(DevMatrix& m) {
-    void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32,
+    void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32, m.s.val16,
                   m.c.rowptr,    m.c.col,    m.c.val,  m.diag,  m.dis};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -311,7 +338,7 @@ int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rp, const uint32_t*
                                    hipMemcpyHostToDevice, s));
             NBX_HIP(hipStreamSynchronize(s));
         }
-        m.has_val32 = false;
+        mat_scaled_stale(m);
     } else {
         DevCsr& Cm = m.c;
         const size_t nnz = n ? rp[n] : 0;
@@ -339,43 +366,69 @@ int mat_extract_diag(DevMatrix& m, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_diag_csr, dim3(grid_for(m.c.n)), dim3(BLOCK), 0, s, m.c, m.diag);
     NBX_HIP(hipGetLastError());
-    m.has_val32 = false;
+    mat_scaled_stale(m);
     return NBX_OK;
 }
 
-int mat_make_scaled32(DevMatrix& m, hipStream_t s) {
+void mat_scaled_stale(DevMatrix& m) {
+    m.has_scale = false;
+    m.has_val32 = false;
+    m.has_val16 = false;
+}
+
+int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
     if (!m.blocked) {
         set_last_error("mixed precision needs a 2-dof node matrix");
         return NBX_ERR_ARG;
     }
-    if (!m.dis) NBX_HIP(hipMalloc(&m.dis, (size_t)m.n * sizeof(double)));
-    if (!m.s.val32) NBX_HIP(hipMalloc(&m.s.val32, (4 * m.s.n_blocks + 4) * sizeof(float)));
-    unsigned* bad = nullptr;
-    NBX_HIP(hipMalloc(&bad, sizeof(unsigned)));
-    NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
-    hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
-    hipLaunchKernelGGL(k_scale32, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
-    // Gershgorin bound of the scaled matrix (Chebyshev preconditioner)
-    const unsigned gg = grid_for(m.s.n_nodes);
-    double *part = nullptr, *dmax = nullptr;
-    NBX_HIP(hipMalloc(&part, (size_t)gg * sizeof(double)));
-    NBX_HIP(hipMalloc(&dmax, sizeof(double)));
-    hipLaunchKernelGGL(k_gersh, dim3(gg), dim3(BLOCK), 0, s, m.s, part);
-    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(FIN), 0, s, part, gg, dmax);
-    NBX_HIP(hipGetLastError());
-    unsigned hbad = 0;
-    NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipMemcpyAsync(&m.lambda_max, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
-    (void)hipFree(bad);
-    (void)hipFree(part);
-    (void)hipFree(dmax);
-    if (hbad) {
-        set_last_error("matrix diagonal has %u non-positive entries", hbad);
+    if (bits != 16 && bits != 32) {
+        set_last_error("scaled copy of %d-bit values (16 or 32)", bits);
         return NBX_ERR_ARG;
     }
-    m.has_val32 = true;
+    if (!m.has_scale) {
+        if (!m.dis) NBX_HIP(hipMalloc(&m.dis, (size_t)m.n * sizeof(double)));
+        const unsigned gg = grid_for(m.s.n_nodes);
+        unsigned* bad = nullptr;
+        double *part = nullptr, *dmax = nullptr;
+        NBX_HIP(hipMalloc(&bad, sizeof(unsigned)));
+        NBX_HIP(hipMalloc(&part, (size_t)gg * sizeof(double)));
+        NBX_HIP(hipMalloc(&dmax, sizeof(double)));
+        NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
+        hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
+        // Gershgorin bound of the scaled matrix (Chebyshev preconditioner)
+        hipLaunchKernelGGL(k_gersh, dim3(gg), dim3(BLOCK), 0, s, m.s, m.dis, part);
+        hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(FIN), 0, s, part, gg, dmax);
+        NBX_HIP(hipGetLastError());
+        unsigned hbad = 0;
+        NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        NBX_HIP(hipMemcpyAsync(&m.lambda_max, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
+        NBX_HIP(hipStreamSynchronize(s));
+        (void)hipFree(bad);
+        (void)hipFree(part);
+        (void)hipFree(dmax);
+        if (hbad) {
+            set_last_error("matrix diagonal has %u non-positive entries", hbad);
+            return NBX_ERR_ARG;
+        }
+        m.has_scale = true;
+    }
+    if (bits == 32 && !m.has_val32) {
+        if (!m.s.val32) NBX_HIP(hipMalloc(&m.s.val32, (4 * m.s.n_blocks + 4) * sizeof(float)));
+        hipLaunchKernelGGL(k_scale32, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
+        NBX_HIP(hipGetLastError());
+        m.has_val32 = true;
+    }
+    if (bits == 16 && !m.has_val16) {
+        if (!m.s.val16) NBX_HIP(hipMalloc(&m.s.val16, (4 * m.s.n_blocks + 4) * sizeof(_Float16)));
+        hipLaunchKernelGGL(k_scale16, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
+        NBX_HIP(hipGetLastError());
+        m.has_val16 = true;
+    }
     return NBX_OK;
+}
+
+int mat_prepare_mixed(DevMatrix& m, hipStream_t s, const MixedOpts& o) {
+    return mat_make_scaled(m, s, o.variant == CHEB_FCG ? o.precond_bits : 32);
 }
 
 int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s) {
@@ -407,10 +460,17 @@ int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStr
     NBX_HIP(hipMemcpyAsync(w.v64[0], ones.data(), m.n * 8, hipMemcpyHostToDevice, s));
     NBX_HIP(hipMemcpyAsync(w.v32[0], onesf.data(), m.n * 4, hipMemcpyHostToDevice, s));
     const unsigned G = map_grid(nu);
+    if ((prec == 32 && !m.has_val32) || (prec == 16 && !m.has_val16)) {
+        set_last_error("spmv_bench: no scaled %d-bit copy", prec);
+        return NBX_ERR_ARG;
+    }
     auto launch = [&]() {
         if (prec == 32)
-            hipLaunchKernelGGL(k_spmv32, dim3(G), dim3(BLOCK), 0, s, m.s, (const float2*)w.v32[0],
-                               (float2*)w.v32[1], nu);
+            hipLaunchKernelGGL(k_spmv_scaled<32>, dim3(G), dim3(BLOCK), 0, s, m.s,
+                               (const float2*)w.v32[0], (float2*)w.v32[1], nu);
+        else if (prec == 16)
+            hipLaunchKernelGGL(k_spmv_scaled<16>, dim3(G), dim3(BLOCK), 0, s, m.s,
+                               (const float2*)w.v32[0], (float2*)w.v32[1], nu);
         else
             hipLaunchKernelGGL(k_spmv<OpS2>, dim3(G), dim3(BLOCK), 0, s, OpS2{m.s},
                                (const double2*)w.v64[0], (double2*)w.v64[1], nu);

@@ -407,6 +407,7 @@ __global__ __launch_bounds__(FIN) void k_fin_fcg_update(PcgState* st, const doub
 }
 
 // one Chebyshev step: res -= Kt d; d' = c1 d + c2 res; z += d'
+template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __restrict__ din,
                                                      float2* __restrict__ dout,
                                                      float2* __restrict__ res,
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
     if (st->done) return;
     const Chunk c = my_chunk(n);
     for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
-        const float2 t = row32(K, (uint32_t)i, din);
+        const float2 t = row_scaled<BITS>(K, (uint32_t)i, din);
         float2 rs = res[i];
         rs.x = rs.x - t.x;
         rs.y = rs.y - t.y;
@@ -755,8 +756,12 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
         float2* din = da;
         float2* dout = db;
         for (uint32_t k = 1; k < deg; ++k) {
-            hipLaunchKernelGGL(k_cheb_step, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z, w.st, nu,
-                               c1[k], c2[k]);
+            if (opt.precond_bits == 16)
+                hipLaunchKernelGGL(k_cheb_step<16>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
+                                   w.st, nu, c1[k], c2[k]);
+            else
+                hipLaunchKernelGGL(k_cheb_step<32>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
+                                   w.st, nu, c1[k], c2[k]);
             float2* t = din;
             din = dout;
             dout = t;
@@ -810,8 +815,9 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
 
 int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, const MixedOpts& opt,
                 uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats) {
-    if (!m.blocked || !m.has_val32) {
-        set_last_error("solve_mixed: matrix has no scaled FP32 copy");
+    const bool fp16 = opt.variant == CHEB_FCG && opt.precond_bits == 16;
+    if (!m.blocked || !m.has_scale || !(fp16 ? m.has_val16 : m.has_val32)) {
+        set_last_error("solve_mixed: matrix has no scaled %d-bit copy", fp16 ? 16 : 32);
         return NBX_ERR_ARG;
     }
     int rc = ws_alloc(w, m.n);

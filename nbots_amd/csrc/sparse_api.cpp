@@ -19,6 +19,7 @@
 
 #include "../../include/nbots_amd.h"
 #include "internal.hpp"
+#include "opts.hpp"
 
 struct nb_sparse_s {
     uint32_t N = 0;
@@ -35,27 +36,6 @@ struct nb_sparse_s {
 
 namespace {
 
-nbx::MixedOpts mixed_opts(double inner_rtol) {
-    nbx::MixedOpts o;
-    if (inner_rtol > 0.0) {
-        o.variant = nbx::CG_RESTART;
-        o.inner_rtol = inner_rtol;
-    } else if (inner_rtol < 0.0) {
-        o.variant = nbx::CG_RELIABLE;
-    }
-    return o;
-}
-
-nbx::MixedOpts mixed_opts(const nbx_solve_opts_t* p) {
-    nbx::MixedOpts o;
-    if (!p) return o;
-    o.variant = (nbx::MixedVariant)p->mixed_variant;
-    if (p->cheb_degree) o.cheb_degree = p->cheb_degree;
-    if (p->cheb_ratio > 1.0) o.cheb_ratio = p->cheb_ratio;
-    if (p->inner_rtol > 0.0) o.inner_rtol = p->inner_rtol;
-    if (p->reliable_delta > 0.0) o.delta = p->reliable_delta;
-    return o;
-}
 
 int64_t find_slot(const nb_sparse_t* A, uint32_t i, uint32_t j) {
     if (i >= A->N) return -1;
@@ -377,10 +357,8 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
         info->fp64_iters = st.fp64_iters;
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
-        if (!A->dev.has_val32) {
-            rc = nbx::mat_make_scaled32(A->dev, A->stream);
-            if (rc) return rc;
-        }
+        rc = nbx::mat_prepare_mixed(A->dev, A->stream, mo);
+        if (rc) return rc;
         status = nbx::solve_mixed(A->dev, db, dx, tol, mo, max_iter,
                                   A->ws, A->stream, &st);
         if (status > 1) return status;
@@ -403,12 +381,14 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
 
 int nbx_sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
                      uint32_t max_iter, double inner_rtol, nbx_solve_info_t* info) {
-    return sparse_solve(A, b, x, mode, tol, max_iter, mixed_opts(inner_rtol), info);
+    return sparse_solve(A, b, x, mode, tol, max_iter, nbx::mixed_opts(inner_rtol), info);
 }
 
 int nbx_sparse_solve_ex(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
                         uint32_t max_iter, const nbx_solve_opts_t* opts, nbx_solve_info_t* info) {
-    return sparse_solve(A, b, x, mode, tol, max_iter, mixed_opts(opts), info);
+    nbx::MixedOpts mo;
+    NBX_TRY(nbx::mixed_opts(opts, &mo));
+    return sparse_solve(A, b, x, mode, tol, max_iter, mo, info);
 }
 
 }  // extern "C"

@@ -13,7 +13,8 @@ MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
 
 
 def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,
-               reliable_delta=None):
+               reliable_delta=None, precond_bits=None):
+    """nbx_solve_opts_t with the library's defaults, then the given fields."""
     o = SolveOpts()
     load().nbx_solve_opts_default(C.byref(o))
     o.mixed_variant = variant
@@ -25,6 +26,8 @@ def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_
         o.inner_rtol = inner_rtol
     if reliable_delta is not None:
         o.reliable_delta = reliable_delta
+    if precond_bits is not None:
+        o.precond_bits = precond_bits
     return o
 
 

@@ -46,7 +46,8 @@ class SolveOpts(C.Structure):
     """nbx_solve_opts_t"""
     _fields_ = [("mixed_variant", C.c_int32), ("cheb_degree", C.c_uint32),
                 ("cheb_ratio", C.c_double), ("inner_rtol", C.c_double),
-                ("reliable_delta", C.c_double)]
+                ("reliable_delta", C.c_double), ("precond_bits", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Graph(C.Structure):

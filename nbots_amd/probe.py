@@ -24,9 +24,11 @@ if ROOT not in sys.path:
 def spmv_bytes(n_nodes: int, n_blocks: int, prec: int) -> int:
     """Algorithmic HBM bytes of one 2x2-block sliced-ELL SpMV: per valid
     block its column (4 B) and 4 values; per node its row length (1 B) and
-    the x read and y write of 2 values; per 64-node slice its offset."""
-    sv = 8 if prec == 64 else 4
-    return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sv
+    the x read and y write of 2 values; per 64-node slice its offset.
+    prec 64, 32 or 16 (fp16 values with FP32 vectors)."""
+    sv = {64: 8, 32: 4, 16: 2}[prec]          # matrix value bytes
+    sx = 8 if prec == 64 else 4               # vector value bytes (fp16 values: FP32 vectors)
+    return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sx
 
 
 def main():

@@ -112,21 +112,23 @@ def test_strain_stress_bit_exact():
     f.destroy()
 
 
-@pytest.mark.parametrize("mode,variant", [(1, None), (2, 0), (2, 1), (2, 2)])
-def test_solve_modes_traction(mode, variant):
-    """FP64 PCG and the three mixed-precision variants (FP32 CG sweeps in
-    FP64 refinement, FP32 CG with FP64 reliable updates, FP64 flexible CG
-    with an FP32 Chebyshev-Jacobi preconditioner) reach relative residual
-    1e-8 (checked with the bit-exact SpMV of the oracle on the device's K)
-    and agree with a 1e-13 oracle solve within rel-L2 1e-6 (the error bound
-    of a 1e-8 residual times the condition number is far larger; 1e-6 is
-    what all modes achieve here)."""
+@pytest.mark.parametrize("mode,variant,bits", [(1, None, None), (2, 0, None), (2, 1, None),
+                                               (2, 2, 16), (2, 2, 32)])
+def test_solve_modes_traction(mode, variant, bits):
+    """FP64 PCG and the mixed-precision variants (FP32 CG sweeps in FP64
+    refinement, FP32 CG with FP64 reliable updates, FP64 flexible CG with a
+    Chebyshev-Jacobi preconditioner on fp16 or FP32 matrix values) reach
+    relative residual 1e-8 (checked with the bit-exact SpMV of the oracle
+    on the device's K) and agree with a 1e-13 oracle solve within rel-L2
+    1e-6 (the error bound of a 1e-8 residual times the condition number is
+    far larger; 1e-6 is what all modes achieve here)."""
     import oracle
     g = load_golden("traction_128x64")
     p = problem_from_golden(g)
     from nbots_amd.fem import Fem, solve_opts
     f = Fem.from_problem(p)
-    info = f.solve(mode, tol=1e-8, opts=None if variant is None else solve_opts(variant))
+    info = f.solve(mode, tol=1e-8,
+                   opts=None if variant is None else solve_opts(variant, precond_bits=bits))
     assert info["status"] == 0, info
     u = f.displacement()
     rp, col, val, F = f.system()
