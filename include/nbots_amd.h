@@ -57,6 +57,16 @@ nb_sparse_t *nb_sparse_create(const nb_graph_t *const graph,
 			      uint32_t vars_per_node);
 nb_sparse_t *nb_sparse_clone(nb_sparse_t *A);                  /* sparse.h:32 */
 void nb_sparse_save(const nb_sparse_t *const A, const char *filename); /* :33 */
+/* headers/nb/solver_bot/matlab_v4.h:8-18 -- MATLAB v4 records; writers
+ * append byte-identical records, readers match the label exactly.        */
+void nb_sparse_read_mat4(nb_sparse_t *A, const char *url, const char *label);
+void nb_sparse_save_mat4(const nb_sparse_t *const A, const char *url,
+			 char *label);
+short nb_mat4_exist(const char *url, char *label);
+void nb_mat4_clear(const char *url);
+void nb_mat4_read_vec(const char *url, char *label, double *_x);
+void nb_mat4_save_vec(const char *url, char *label, const double *const x,
+		      uint32_t N);
 void nb_sparse_destroy(nb_sparse_t *A);                        /* sparse.h:34 */
 void nb_sparse_reset(nb_sparse_t *A);                          /* sparse.h:35 */
 void nb_sparse_set(nb_sparse_t *A, uint32_t i, uint32_t j, double value); /* :36 */
@@ -93,6 +103,15 @@ int nb_sparse_solve_CG_precond_Jacobi(const nb_sparse_t *const A,
 				      uint32_t *niter_performed,
 				      double *tolerance_reached,
 				      uint32_t omp_parallel_threads);
+/* headers/nb/solver_bot/sparse/solvers/conjugate_gradient.h:8-15
+ * Unpreconditioned CG on the GPU, same conventions as above.            */
+int nb_sparse_solve_conjugate_gradient(const nb_sparse_t *const A,
+				       const double *const b,
+				       double *_x,
+				       uint32_t max_iter, double tolerance,
+				       uint32_t *niter_performed,
+				       double *tolerance_reached,
+				       uint32_t omp_parallel_threads);
 
 /* ======================================================================== */
 /* FEM objects used by the driver                                          */

@@ -100,9 +100,10 @@ int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s);
 
 // Reference-semantics Jacobi PCG (absolute tolerance on ||Ax-b||, lagged stop
 // test); x holds the initial guess.  Returns 0 converged / 1 not, <0 error.
+// unpreconditioned: the reference's plain CG (conjugate_gradient.c:13-77).
 int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter,
                double tol, uint32_t* niter, double* tol_reached, Workspace& w,
-               hipStream_t s, SolveStats* stats);
+               hipStream_t s, SolveStats* stats, bool unpreconditioned = false);
 
 // FP32 inner CG on the Jacobi-scaled FP32 copy inside FP64 iterative
 // refinement; stops when ||b - A x||_2 <= rtol * ||b||_2.

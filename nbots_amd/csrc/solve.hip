@@ -507,15 +507,17 @@ static int sampled_ms(Workspace& w, uint32_t n, double* avg) {
     return NBX_OK;
 }
 
+// diag: the Jacobi diagonal, or all ones for unpreconditioned CG (q = g/1
+// is g exactly, so the same kernels give conjugate_gradient.c:13-77).
 template <class Op>
-static int pcg_run(const Op& op, uint32_t nu, const DevMatrix& m, const typename Op::V* b,
+static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typename Op::V* b,
                    typename Op::V* x, uint32_t max_iter, double tol, Workspace& w,
                    hipStream_t s, SolveStats* stats) {
     using V = typename Op::V;
     V* g = (V*)w.v64[0];
     V* p = (V*)w.v64[1];
     V* wv = (V*)w.v64[2];
-    const V* diag = (const V*)m.diag;
+    const V* diag = (const V*)diag_in;
     PcgState h{};
     h.tol2 = tol * tol;
     h.max_iter = max_iter;
@@ -564,22 +566,35 @@ static int pcg_run(const Op& op, uint32_t nu, const DevMatrix& m, const typename
     return NBX_OK;
 }
 
+__global__ void k_fill_one(double* __restrict__ v, uint32_t n) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) v[i] = 1.0;
+}
+
 int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter, double tol,
                uint32_t* niter, double* tol_reached, Workspace& w, hipStream_t s,
-               SolveStats* stats) {
+               SolveStats* stats, bool unpreconditioned) {
     if (m.n == 0) {
-        // empty system: no iteration, ||g|| = 0 (cg_precond_jacobi.c:44-89)
+        // empty system: no iteration, ||g|| = 0 (cg_precond_jacobi.c:44-89,
+        // conjugate_gradient.c:38-76)
         if (niter) *niter = 0;
         if (tol_reached) *tol_reached = 0.0;
         return 0;
     }
     int rc = ws_alloc(w, m.n);
     if (rc) return -rc;
+    const double* diag = m.diag;
+    if (unpreconditioned) {
+        // v64[5] is free during a PCG run (pcg_run uses v64[0..2])
+        hipLaunchKernelGGL(k_fill_one, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, w.v64[5], m.n);
+        if (hipGetLastError() != hipSuccess) return -NBX_ERR_HIP;
+        diag = w.v64[5];
+    }
     if (m.blocked)
-        rc = pcg_run(OpS2{m.s}, m.s.n_nodes, m, (const double2*)b, (double2*)x, max_iter, tol, w,
+        rc = pcg_run(OpS2{m.s}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol, w,
                      s, stats);
     else
-        rc = pcg_run(OpCsr{m.c}, m.c.n, m, b, x, max_iter, tol, w, s, stats);
+        rc = pcg_run(OpCsr{m.c}, m.c.n, diag, b, x, max_iter, tol, w, s, stats);
     if (rc) return -rc;
     if (niter) *niter = w.h_st->k;
     if (tol_reached) *tol_reached = w.h_st->tol_reached;

@@ -78,6 +78,8 @@ void stale(nb_sparse_t* A) { A->dev_values = false; }
 
 static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
                         uint32_t max_iter, const nbx::MixedOpts& mo, nbx_solve_info_t* info);
+// solve mode of the drop-in nb_sparse_solve_conjugate_gradient (not public)
+constexpr int MODE_PLAIN_CG = 0x100;
 
 extern "C" {
 
@@ -149,6 +151,165 @@ void nb_sparse_save(const nb_sparse_t* const A, const char* filename) {
     for (uint32_t i = 0; i < A->N; ++i)
         for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1]; ++s)
             fprintf(fp, "%i %i %e \n", i, A->col[s], A->val[s]);
+    fclose(fp);
+}
+
+// ---- MATLAB v4 files (solver_bot/matlab_v4.c) --------------------------------
+// Record header: five int32 {type, rows, cols, imag, len(label)+1}, the label
+// with its '\0', then the doubles column by column.  A sparse matrix is type 2
+// with nnz+1 rows and 3 columns: 1-based row indices and N, 1-based column
+// indices and N, values and a closing 0, entries ordered by column then row
+// (matlab_v4.c:186-245).  A vector is a full N x 1 matrix, type 0
+// (matlab_v4.c:493-513).  Writers append and are byte-identical with the
+// reference's.  Readers take the record whose label is equal and fill the
+// caller's object or buffer; the reference skips only records for which
+// strcmp() returns 1 (matlab_v4.c:84) and its vector reader fills a buffer of
+// its own (matlab_v4.c:478), so it agrees with these readers on the files its
+// writers produce with distinct labels.
+
+static void mat4_fail(const char* msg) {
+    printf("ERROR: %s\n", msg);
+    exit(1);
+}
+
+static void mat4_header(FILE* fp, int32_t type, int32_t rows, int32_t cols, const char* label) {
+    const int32_t len = (int32_t)strlen(label);
+    const int32_t info[5] = {type, rows, cols, 0, len + 1};
+    const char end = '\0';
+    fwrite(info, 4, 5, fp);
+    fwrite(label, 1, (size_t)len, fp);
+    fwrite(&end, 1, 1, fp);
+}
+
+// Positions fp after the header and label of the record named label; returns
+// its info, or false at the end of the file.
+static bool mat4_find(FILE* fp, const char* label, int32_t info[5]) {
+    while (fread(info, 4, 5, fp) == 5) {
+        if (info[4] <= 0) mat4_fail("The matlab v4 file is corrupted.");
+        std::vector<char> name((size_t)info[4] + 1, '\0');
+        if (fread(name.data(), 1, (size_t)info[4], fp) != (size_t)info[4])
+            mat4_fail("The matlab v4 file is corrupted.");
+        if (info[0] != 0 && info[0] != 2)
+            mat4_fail("Support only for complete and sparse matrix with double precision.");
+        if (strcmp(name.data(), label) == 0) return true;
+        const long skip = (long)info[1] * (long)info[2] * (long)sizeof(double);
+        if (fseek(fp, skip, SEEK_CUR) != 0) mat4_fail("The matlab v4 file is corrupted.");
+    }
+    return false;
+}
+
+void nb_sparse_save_mat4(const nb_sparse_t* const A, const char* url, char* label) {
+    FILE* fp = fopen(url, "ab");
+    if (!fp) mat4_fail("Impossible to open matlab v4 file.");
+    const uint32_t nnz = (uint32_t)A->col.size();
+    mat4_header(fp, 2, (int32_t)nnz + 1, 3, label);
+    std::vector<uint32_t> order(nnz), row(nnz);
+    for (uint32_t i = 0; i < A->N; ++i)
+        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1]; ++s) row[s] = i;
+    for (uint32_t s = 0; s < nnz; ++s) order[s] = s;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return A->col[a] != A->col[b] ? A->col[a] < A->col[b] : row[a] < row[b];
+    });
+    std::vector<double> out(3 * (size_t)nnz + 3);
+    const double N = (double)A->N;
+    for (uint32_t k = 0; k < nnz; ++k) {
+        out[k] = (double)row[order[k]] + 1.0;
+        out[nnz + 1 + k] = (double)A->col[order[k]] + 1.0;
+        out[2 * (size_t)nnz + 2 + k] = A->val[order[k]];
+    }
+    out[nnz] = N;
+    out[2 * (size_t)nnz + 1] = N;
+    out[3 * (size_t)nnz + 2] = 0.0;
+    fwrite(out.data(), sizeof(double), out.size(), fp);
+    fclose(fp);
+}
+
+void nb_sparse_read_mat4(nb_sparse_t* A, const char* url, const char* label) {
+    FILE* fp = fopen(url, "rb");
+    if (!fp) mat4_fail("Impossible to open matlab v4 file.");
+    int32_t info[5];
+    if (!mat4_find(fp, label, info) || info[0] != 2) {
+        fclose(fp);
+        printf("ERROR: Sparse matrix \"%s\" not found in \"%s\".\n", label, url);
+        exit(1);
+    }
+    if (info[2] != 3) mat4_fail("Complex numbers unsupported.");
+    const uint32_t nnz = (uint32_t)(info[1] - 1);
+    std::vector<double> d(3 * (size_t)nnz + 3);
+    if (fread(d.data(), sizeof(double), d.size(), fp) != d.size())
+        mat4_fail("The matlab v4 file is corrupted.");
+    fclose(fp);
+    const uint32_t N = (uint32_t)d[nnz];
+    std::vector<uint32_t> rowptr(N + 1, 0), col(nnz);
+    std::vector<double> val(nnz);
+    for (uint32_t k = 0; k < nnz; ++k) {
+        const double r = d[k];
+        if (!(r >= 1.0 && r <= (double)N)) mat4_fail("The matlab v4 file is corrupted.");
+        rowptr[(uint32_t)r]++;
+    }
+    for (uint32_t i = 0; i < N; ++i) rowptr[i + 1] += rowptr[i];
+    std::vector<uint32_t> fill(rowptr.begin(), rowptr.end() - 1);
+    for (uint32_t k = 0; k < nnz; ++k) {
+        const uint32_t i = (uint32_t)d[k] - 1;
+        const double c = d[nnz + 1 + k];
+        if (!(c >= 1.0 && c <= (double)N)) mat4_fail("The matlab v4 file is corrupted.");
+        col[fill[i]] = (uint32_t)c - 1;
+        val[fill[i]++] = d[2 * (size_t)nnz + 2 + k];
+    }
+    // columns ascending within a row, values moved with them
+    for (uint32_t i = 0; i < N; ++i) {
+        std::vector<std::pair<uint32_t, double>> e;
+        for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; ++s) e.emplace_back(col[s], val[s]);
+        std::stable_sort(e.begin(), e.end(),
+                         [](const std::pair<uint32_t, double>& a,
+                            const std::pair<uint32_t, double>& b) { return a.first < b.first; });
+        for (uint32_t s = rowptr[i], t = 0; s < rowptr[i + 1]; ++s, ++t) {
+            col[s] = e[t].first;
+            val[s] = e[t].second;
+        }
+    }
+    A->N = N;
+    A->rowptr.swap(rowptr);
+    A->col.swap(col);
+    A->val.swap(val);
+    A->dev_pattern = false;
+    A->dev_values = false;
+}
+
+void nb_mat4_clear(const char* url) {
+    FILE* fp = fopen(url, "wb");
+    if (fp) fclose(fp);
+}
+
+short nb_mat4_exist(const char* url, char* label) {
+    FILE* fp = fopen(url, "rb");
+    if (!fp) mat4_fail("Impossible to open matlab v4 file.");
+    int32_t info[5];
+    const bool found = mat4_find(fp, label, info);
+    fclose(fp);
+    return found ? 1 : 0;
+}
+
+void nb_mat4_save_vec(const char* url, char* label, const double* const x, uint32_t N) {
+    FILE* fp = fopen(url, "ab");
+    if (!fp) mat4_fail("Impossible to open matlab v4 file.");
+    mat4_header(fp, 0, (int32_t)N, 1, label);
+    fwrite(x, sizeof(double), N, fp);
+    fclose(fp);
+}
+
+void nb_mat4_read_vec(const char* url, char* label, double* _x) {
+    FILE* fp = fopen(url, "rb");
+    if (!fp) mat4_fail("Impossible to open matlab v4 file.");
+    int32_t info[5];
+    if (!mat4_find(fp, label, info) || info[0] != 0 || (info[1] != 1 && info[2] != 1)) {
+        fclose(fp);
+        printf("ERROR: Vector \"%s\" not found in \"%s\".\n", label, url);
+        exit(1);
+    }
+    if (info[3] == 1) mat4_fail("Complex numbers unsupported.");
+    const size_t n = (size_t)(info[1] == 1 ? info[2] : info[1]);
+    if (fread(_x, sizeof(double), n, fp) != n) mat4_fail("The matlab v4 file is corrupted.");
     fclose(fp);
 }
 
@@ -288,6 +449,23 @@ int nb_sparse_solve_CG_precond_Jacobi(const nb_sparse_t* const A, const double* 
     return rc;
 }
 
+// conjugate_gradient.c:13-77 (declared in solvers/conjugate_gradient.h:8-15):
+// the Jacobi-PCG kernels with a unit diagonal, same stop test and outputs.
+int nb_sparse_solve_conjugate_gradient(const nb_sparse_t* const A, const double* const b,
+                                       double* _x, uint32_t max_iter, double tolerance,
+                                       uint32_t* niter_performed, double* tolerance_reached,
+                                       uint32_t) {
+    nbx_solve_info_t info;
+    int rc = sparse_solve(A, b, _x, MODE_PLAIN_CG, tolerance, max_iter, nbx::MixedOpts(), &info);
+    if (rc != NBX_OK && rc != NBX_NOT_CONVERGED) {
+        fprintf(stderr, "nb_sparse_solve_conjugate_gradient: GPU failure: %s\n", nbx::last_error());
+        return -1;
+    }
+    if (niter_performed) *niter_performed = info.iters;
+    if (tolerance_reached) *tolerance_reached = info.tol_reached;
+    return rc;
+}
+
 const char* nbx_last_error(void) { return nbx::last_error(); }
 
 int nbx_device_info(char* name, uint32_t len, uint32_t* n_cu, double* hbm_gb) {
@@ -342,10 +520,11 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
     NBX_HIP(hipMemcpyAsync(dx, x, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));
     nbx::SolveStats st;
     int status = 0;
-    if (mode == NBX_SOLVE_REFERENCE) {
+    if (mode == NBX_SOLVE_REFERENCE || mode == MODE_PLAIN_CG) {
         uint32_t it = 0;
         double tr = 0;
-        status = nbx::pcg_jacobi(A->dev, db, dx, max_iter, tol, &it, &tr, A->ws, A->stream, &st);
+        status = nbx::pcg_jacobi(A->dev, db, dx, max_iter, tol, &it, &tr, A->ws, A->stream, &st,
+                                 mode == MODE_PLAIN_CG);
         if (status < 0) return -status;
         info->iters = it;
         info->fp64_iters = it;

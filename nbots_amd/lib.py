@@ -68,6 +68,12 @@ SIGS = {
     "nb_sparse_create": (VP, [VP, u32o, U32]),
     "nb_sparse_clone": (VP, [VP]),
     "nb_sparse_save": (None, [VP, C.c_char_p]),
+    "nb_sparse_read_mat4": (None, [VP, C.c_char_p, C.c_char_p]),
+    "nb_sparse_save_mat4": (None, [VP, C.c_char_p, C.c_char_p]),
+    "nb_mat4_exist": (C.c_short, [C.c_char_p, C.c_char_p]),
+    "nb_mat4_clear": (None, [C.c_char_p]),
+    "nb_mat4_read_vec": (None, [C.c_char_p, C.c_char_p, f64p]),
+    "nb_mat4_save_vec": (None, [C.c_char_p, C.c_char_p, f64p, U32]),
     "nb_sparse_destroy": (None, [VP]),
     "nb_sparse_reset": (None, [VP]),
     "nb_sparse_set": (None, [VP, U32, U32, F64]),
@@ -87,6 +93,8 @@ SIGS = {
     "nb_sparse_set_Dirichlet_condition": (None, [VP, f64p, U32, F64]),
     "nb_sparse_solve_CG_precond_Jacobi": (I32, [VP, f64p, f64p, U32, F64, C.POINTER(U32),
                                                 C.POINTER(F64), U32]),
+    "nb_sparse_solve_conjugate_gradient": (I32, [VP, f64p, f64p, U32, F64, C.POINTER(U32),
+                                                 C.POINTER(F64), U32]),
     "nb_mesh2D_create": (VP, [I32]),
     "nb_mesh2D_destroy": (None, [VP]),
     "nb_mesh2D_get_N_invtx": (U32, [VP]),

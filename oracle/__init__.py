@@ -79,6 +79,9 @@ def lib():
         L.orc_pcg_jacobi.restype = I32
         L.orc_pcg_jacobi.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
                                      C.POINTER(U32), C.POINTER(F64)]
+        L.orc_cg.restype = I32
+        L.orc_cg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
+                             C.POINTER(U32), C.POINTER(F64)]
         L.orc_set_bconditions.argtypes = (
             [f64p, u32p, u32p, u32p]
             + [U32, u32p, u8p, f64p] * 3
@@ -122,6 +125,12 @@ def ref():
         R.ref_pcg.restype = I32
         R.ref_pcg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64, U32,
                               C.POINTER(U32), C.POINTER(F64)]
+        R.ref_sparse_save.argtypes = [U32, u32p, u32p, f64p, C.c_char_p]
+        R.ref_sparse_save_mat4.argtypes = [U32, u32p, u32p, f64p, C.c_char_p, C.c_char_p]
+        R.ref_mat4_save_vec.argtypes = [C.c_char_p, C.c_char_p, f64p, U32]
+        R.ref_cg.restype = I32
+        R.ref_cg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64, U32,
+                             C.POINTER(U32), C.POINTER(F64)]
         R.ref_sparse_import.restype = C.c_void_p
         R.ref_sparse_import.argtypes = [U32, u32p, u32p, f64p]
         R.ref_sparse_free.argtypes = [C.c_void_p]
@@ -210,6 +219,16 @@ def pcg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8):
     st = lib().orc_pcg_jacobi(N, rowptr, col, val, _c(b, np.float64), x,
                               N if max_iter is None else max_iter, tol,
                               C.byref(it), C.byref(tr))
+    return x, it.value, tr.value, st
+
+
+def cg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8):
+    """Unpreconditioned CG restated from conjugate_gradient.c:13-77."""
+    N = rowptr.size - 1
+    x = np.zeros(N) if x0 is None else np.array(x0, np.float64)
+    it, tr = U32(0), F64(0)
+    st = lib().orc_cg(N, rowptr, col, val, _c(b, np.float64), x,
+                      N if max_iter is None else max_iter, tol, C.byref(it), C.byref(tr))
     return x, it.value, tr.value, st
 
 
@@ -338,6 +357,30 @@ def ref_pcg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8, threads=1):
                        N if max_iter is None else max_iter, tol, threads,
                        C.byref(it), C.byref(tr))
     return x, it.value, tr.value, st
+
+
+def ref_cg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8, threads=1):
+    """The compiled reference's nb_sparse_solve_conjugate_gradient."""
+    N = rowptr.size - 1
+    x = np.zeros(N) if x0 is None else np.array(x0, np.float64)
+    it, tr = U32(0), F64(0)
+    st = ref().ref_cg(N, rowptr, col, val, _c(b, np.float64), x,
+                      N if max_iter is None else max_iter, tol, threads,
+                      C.byref(it), C.byref(tr))
+    return x, it.value, tr.value, st
+
+
+def ref_sparse_save(rowptr, col, val, path):
+    ref().ref_sparse_save(rowptr.size - 1, rowptr, col, val, path.encode())
+
+
+def ref_sparse_save_mat4(rowptr, col, val, path, label):
+    ref().ref_sparse_save_mat4(rowptr.size - 1, rowptr, col, val, path.encode(), label.encode())
+
+
+def ref_mat4_save_vec(path, label, x):
+    x = _c(x, np.float64)
+    ref().ref_mat4_save_vec(path.encode(), label.encode(), x, x.size)
 
 
 def ref_spmv(rowptr, col, val, x, threads=1):

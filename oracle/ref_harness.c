@@ -355,6 +355,43 @@ int ref_pcg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 	return st;
 }
 
+int ref_cg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+	   const double *val, const double *b, double *x,
+	   uint32_t max_iter, double tol, uint32_t threads,
+	   uint32_t *niter, double *tol_reached)
+{
+	nb_sparse_t *A = import_csr(N, rowptr, col, val);
+	int st = nb_sparse_solve_conjugate_gradient(A, b, x, max_iter, tol,
+						    niter, tol_reached, threads);
+	nb_sparse_destroy(A);
+	return st;
+}
+
+/* The reference's file writers on an imported matrix (sparse.c:97-112,
+ * matlab_v4.c:186-245, :493-513). */
+void ref_sparse_save(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		     const double *val, const char *path)
+{
+	nb_sparse_t *A = import_csr(N, rowptr, col, val);
+	nb_sparse_save(A, path);
+	nb_sparse_destroy(A);
+}
+
+void ref_sparse_save_mat4(uint32_t N, const uint32_t *rowptr,
+			  const uint32_t *col, const double *val,
+			  const char *path, char *label)
+{
+	nb_sparse_t *A = import_csr(N, rowptr, col, val);
+	nb_sparse_save_mat4(A, path, label);
+	nb_sparse_destroy(A);
+}
+
+void ref_mat4_save_vec(const char *path, char *label, const double *x,
+		       uint32_t N)
+{
+	nb_mat4_save_vec(path, label, x, N);
+}
+
 /* Handle-based variant so that the matrix conversion is not timed. */
 void *ref_sparse_import(uint32_t N, const uint32_t *rowptr,
 			const uint32_t *col, const double *val)

@@ -425,6 +425,57 @@ int orc_pcg_jacobi(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 	return (dot_gg > tol * tol) ? 1 : 0;
 }
 
+/* conjugate_gradient.c:13-77: g = Ax - b, p = -g; while ||g_lagged|| > tol:
+ * w = Ap with p.w and g.g summed in the same pass (g.g of the g before the
+ * update, hence the lag), alpha = gg/pw, x += alpha p, g += alpha w,
+ * beta = g_new.g_new / gg, p = -g + beta p. */
+int orc_cg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+	   const double *val, const double *b, double *x,
+	   uint32_t max_iter, double tol,
+	   uint32_t *niter, double *tol_reached)
+{
+	double *g = calloc((size_t)N * 3 + 1, sizeof(double));
+	double *p = g + N, *w = g + 2 * (size_t)N;
+	double dot_gg = 0;
+	for (uint32_t i = 0; i < N; i++) {                  /* :29-37 */
+		double sum = 0;
+		for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; s++)
+			sum += val[s] * x[col[s]];
+		g[i] = sum - b[i];
+		p[i] = -g[i];
+		dot_gg += g[i] * g[i];
+	}
+	uint32_t k = 0;
+	while (dot_gg > tol * tol && k < max_iter) {        /* :39 */
+		double dot_pw = 0;
+		dot_gg = 0;
+		for (uint32_t i = 0; i < N; i++) {          /* :42-49 */
+			w[i] = 0;
+			for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; s++)
+				w[i] += val[s] * p[col[s]];
+			dot_pw += p[i] * w[i];
+			dot_gg += g[i] * g[i];
+		}
+		double alphak = dot_gg / dot_pw;
+		double dot_gkgk = 0;
+		for (uint32_t i = 0; i < N; i++) {          /* :52-57 */
+			x[i] += alphak * p[i];
+			g[i] += alphak * w[i];
+			dot_gkgk += g[i] * g[i];
+		}
+		double betak = dot_gkgk / dot_gg;
+		for (uint32_t i = 0; i < N; i++)            /* :58-62 */
+			p[i] = -g[i] + betak * p[i];
+		k++;
+	}
+	free(g);
+	if (NULL != niter)
+		*niter = k;
+	if (NULL != tol_reached)
+		*tol_reached = sqrt(dot_gg);
+	return (dot_gg > tol * tol) ? 1 : 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Boundary conditions                                                      */
 /* ------------------------------------------------------------------------ */

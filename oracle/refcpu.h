@@ -21,6 +21,8 @@
  *   orc_set_bconditions    golden K/F after BCs, dense elimination identity
  *   orc_pcg_jacobi         golden x/niter/tol_reached, hand-traced 3x3 case,
  *                          dense numpy solve
+ *   orc_cg                 golden x/niter/tol_reached of the compiled
+ *                          reference's plain CG, dense numpy solve
  *   orc_spmv               golden y, dense product
  *   orc_strain/orc_stress  golden strain/stress, patch test (linear field)
  *   orc_elem_energy, orc_simp_*, orc_filter_*, orc_oc_update
@@ -73,6 +75,12 @@ int orc_pcg_jacobi(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 		   const double *val, const double *b, double *x,
 		   uint32_t max_iter, double tol,
 		   uint32_t *niter, double *tol_reached);
+/* Unpreconditioned CG (solver_bot/sparse/solvers/conjugate_gradient.c:13-77):
+ * the same absolute stop test on ||Ax-b|| lagged by one iteration.       */
+int orc_cg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+	   const double *val, const double *b, double *x,
+	   uint32_t max_iter, double tol,
+	   uint32_t *niter, double *tol_reached);
 
 /* ---- boundary conditions ----------------------------------------------- */
 void orc_set_bconditions(const double *nod,

@@ -169,8 +169,57 @@ def synth_pin():
         json.dump(pin, f, indent=1)
 
 
-def main():
+def cg_cases():
+    """The reference's unpreconditioned CG (conjugate_gradient.c:13-77) on the
+    K and F (after BCs) of two systems, to convergence and cut at 7
+    iterations, and on the 1x1 system that converges in one step.  The
+    systems themselves are not stored: the oracle's assembly reproduces them
+    bit for bit (pinned by the other fixtures)."""
+    out = {}
+    for key, p in (("traction_16x8", synth.cantilever_traction(16, 8)),
+                   ("c1_64x32", synth.cantilever_point(64, 32, jitter=0.1))):
+        rp, col, val, F, st = oracle.ref_assemble(p, apply_bc=True)
+        assert st == 0
+        tol = 1e-8 * float(np.linalg.norm(F))
+        x, it, tr, cst = oracle.ref_cg(rp, col, val, F, tol=tol, max_iter=10 * p.N)
+        x7, it7, tr7, st7 = oracle.ref_cg(rp, col, val, F, tol=tol, max_iter=7)
+        out.update({key + "_tol": tol, key + "_x": x, key + "_niter": it, key + "_tol_reached": tr,
+                    key + "_status": cst, key + "_x7": x7, key + "_niter7": it7,
+                    key + "_tol_reached7": tr7, key + "_status7": st7})
+        print(f"cg {key}: niter {it} status {cst} tol_reached {tr:.3e}")
+    x, it, tr, st = oracle.ref_cg(np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                                  np.array([4.0]), np.array([2.0]), max_iter=10, tol=1e-8)
+    out.update(one_x=x, one_niter=it, one_tol_reached=tr, one_status=st)
+    print("cg one_by_one:", x, it, tr, st)
+    np.savez_compressed(os.path.join(OUT, "cg_cases.npz"), **out)
+
+
+def file_cases():
+    """Files written by the reference's own writers for the 4x2 jittered
+    cantilever: nb_sparse_save text of K, and a MATLAB v4 file holding K
+    (after BCs) as "K", F as "F" and K before BCs as "K0"."""
+    p = synth.cantilever_point(4, 2, jitter=0.1)
+    rp, col, val0, F0, st = oracle.ref_assemble(p, apply_bc=False)
+    _, _, val, F, _ = oracle.ref_assemble(p, apply_bc=True)
+    oracle.ref_sparse_save(rp, col, val, os.path.join(OUT, "cant_4x2_K.txt"))
+    mat = os.path.join(OUT, "cant_4x2.mat")
+    if os.path.exists(mat):
+        os.remove(mat)
+    oracle.ref_sparse_save_mat4(rp, col, val, mat, "K")
+    oracle.ref_mat4_save_vec(mat, "F", F)
+    oracle.ref_sparse_save_mat4(rp, col, val0, mat, "K0")
+    print("files:", os.path.getsize(mat), "B mat4,",
+          os.path.getsize(os.path.join(OUT, "cant_4x2_K.txt")), "B text")
+
+
+def main(argv):
     oracle.build(ref=True)
+    if argv[1:] == ["cg"]:
+        cg_cases()
+        return
+    if argv[1:] == ["files"]:
+        file_cases()
+        return
     synth_pin()
     full_case(synth.cantilever_point(2, 1, jitter=0.0), "cant_2x1")
     full_case(synth.cantilever_point(4, 2, jitter=0.1), "cant_4x2_jit")
@@ -182,7 +231,9 @@ def main():
               keep_x=False)
     beam_case("beam_cantilever.txt", "beam_cantilever")
     degenerate_cases()
+    cg_cases()
+    file_cases()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv)

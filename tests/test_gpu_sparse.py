@@ -123,3 +123,50 @@ def test_solve_modes_small():
         assert np.linalg.norm(r) <= 1.0000001e-8 * np.linalg.norm(F), info
         assert np.linalg.norm(x - g["x"]) <= 1e-6 * np.linalg.norm(g["x"])
     A.destroy()
+
+
+CG_CASES = {"traction_16x8": lambda s: s.cantilever_traction(16, 8),
+            "c1_64x32": lambda s: s.cantilever_point(64, 32, jitter=0.1)}
+
+
+@pytest.mark.parametrize("key", sorted(CG_CASES))
+def test_plain_cg_reference_semantics(key):
+    """nb_sparse_solve_conjugate_gradient (conjugate_gradient.c:13-77) against
+    the compiled reference's run on the same K and F: status equal, niter
+    within 2 and x within 1e-10 at convergence; cut at 7 iterations the
+    status and niter are equal and tol_reached within 1e-9."""
+    import nbots_amd as nb
+    import oracle
+    from nbots_amd import synth
+    g = load_golden("cg_cases")
+    p = CG_CASES[key](synth)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    tol = float(g[key + "_tol"])
+    x, st, it, tr = A.solve_cg(F, tol=tol, max_iter=10 * p.N)
+    assert st == int(g[key + "_status"])
+    assert abs(it - int(g[key + "_niter"])) <= 2, (it, int(g[key + "_niter"]))
+    ref = g[key + "_x"]
+    assert np.linalg.norm(x - ref) <= 1e-10 * np.linalg.norm(ref)
+    x7, st7, it7, tr7 = A.solve_cg(F, tol=tol, max_iter=7)
+    assert (st7, it7) == (int(g[key + "_status7"]), int(g[key + "_niter7"]))
+    assert math.isclose(tr7, float(g[key + "_tol_reached7"]), rel_tol=1e-9)
+    assert np.linalg.norm(x7 - g[key + "_x7"]) <= 1e-11 * np.linalg.norm(g[key + "_x7"])
+    A.destroy()
+
+
+def test_plain_cg_edge_cases():
+    """1x1 exact convergence then the lagged 0/0 iteration (x = NaN, niter
+    2, as the compiled reference); empty right-hand side."""
+    import nbots_amd as nb
+    g = load_golden("cg_cases")
+    A = nb.Sparse.from_csr(np.array([0, 1], np.uint32), np.array([0], np.uint32),
+                           np.array([4.0]), vars_per_node=1)
+    x, st, it, tr = A.solve_cg(np.array([2.0]), max_iter=10, tol=1e-8)
+    assert math.isnan(x[0])
+    assert (it, st, tr) == (int(g["one_niter"]), int(g["one_status"]), float(g["one_tol_reached"]))
+    x, st, it, tr = A.solve_cg(np.zeros(1), max_iter=10, tol=1e-8)
+    assert (st, it, tr) == (0, 0, 0.0) and x[0] == 0.0
+    A.destroy()
