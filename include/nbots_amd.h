@@ -177,6 +177,19 @@ void nb_material_set_density(nb_material_t *mat, double density);
 double nb_material_get_poisson_module(const nb_material_t *const mat);
 double nb_material_get_elasticity_module(const nb_material_t *const mat);
 double nb_material_get_density(const nb_material_t *const mat);
+nb_material_t *nb_material_clone(nb_material_t *mat);             /* :7 */
+/* material.h:13-29 -- the damage model's parameters                    */
+void nb_material_set_fracture_energy(nb_material_t *mat, double frac_energy);
+void nb_material_set_traction_limit_stress(nb_material_t *mat, double stress);
+void nb_material_set_compression_limit_stress(nb_material_t *mat,
+					      double stress);
+void nb_material_set_energy_release_rate(nb_material_t *mat, double energy_rr);
+void nb_material_set_damage_length_scale(nb_material_t *mat, double length);
+double nb_material_get_fracture_energy(const nb_material_t *const mat);
+double nb_material_get_traction_limit_stress(const nb_material_t *const mat);
+double nb_material_get_compression_limit_stress(const nb_material_t *const mat);
+double nb_material_get_energy_release_rate(const nb_material_t *const mat);
+double nb_material_get_damage_length_scale(const nb_material_t *const mat);
 
 /* headers/nb/pde_bot/boundary_conditions/bcond.h:19-45 (FIFO lists) */
 uint16_t nb_bcond_get_memsize(uint8_t N_dof);
@@ -217,6 +230,47 @@ void nb_fem_compute_stress_from_strain(uint32_t N_elements,
 				       double *strain,
 				       const bool *elements_enabled,
 				       double *stress);
+
+/* headers/nb/pde_bot/finite_element/solid_mechanics/static_damage2D.h:13-47
+ * -- quasi-static damage model, load steps of an implicit scheme.         */
+typedef struct nb_fem_implicit_s nb_fem_implicit_t;
+nb_fem_implicit_t *nb_fem_implicit_create(void);
+void nb_fem_implicit_destroy(nb_fem_implicit_t *isparams);
+void nb_fem_implicit_set_N_steps(nb_fem_implicit_t *isparams,
+				 uint32_t N_time_steps);
+void nb_fem_implicit_set_N_max_iter(nb_fem_implicit_t *isparams,
+				    uint32_t N_max_iter);
+void nb_fem_implicit_set_N_max_iter_without_enhance(nb_fem_implicit_t *isparams,
+						    uint32_t N_max_iter);
+void nb_fem_implicit_set_residual_tolerance(nb_fem_implicit_t *isparams,
+					    double penergy_tol);
+uint32_t nb_fem_implicit_get_N_steps(nb_fem_implicit_t *isparams);
+uint32_t nb_fem_implicit_get_N_max_iter(nb_fem_implicit_t *isparams);
+uint32_t nb_fem_implicit_get_N_max_iter_without_enhance(nb_fem_implicit_t *isparams);
+double nb_fem_implicit_get_residual_tolerance(nb_fem_implicit_t *isparams);
+/* Linear triangles; the log file receives the reference's lines (residual
+ * of every iteration of every load step).  Conjugate-gradient path: the
+ * direct-solver switch is accepted and runs the same Jacobi-PCG.        */
+void nb_fem_compute_2D_Non_Linear_Solid_Mechanics
+		(const nb_mesh2D_t *const part,
+		 const nb_fem_elem_t *const elemtype,
+		 const nb_material_t *const material,
+		 const nb_bcond_t *const bcond,
+		 bool enable_self_weight,
+		 double gravity[2],
+		 bool enable_Cholesky_solver,
+		 nb_analysis2D_t analysis2D,
+		 nb_analysis2D_params *params2D,
+		 nb_fem_implicit_t *params,
+		 const char *logfile);
+/* The same solve returning the final displacement [2 N_nod] and damage
+ * [N_elem] (each nullable) and an NBX_* status (logfile nullable).      */
+int nbx_fem_damage_solve(const nb_mesh2D_t *part, const nb_fem_elem_t *elemtype,
+			 const nb_material_t *material, const nb_bcond_t *bcond,
+			 int enable_self_weight, const double *gravity,
+			 const nb_analysis2D_params *params2D,
+			 nb_fem_implicit_t *params, const char *logfile,
+			 double *displacement, double *damage);
 
 /* ======================================================================== */
 /* nbx extensions                                                           */

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fem.hpp"
+#include "rows.hpp"
 
 namespace nbx {
 
@@ -79,6 +80,13 @@ __device__ __forceinline__ void elem_D(uint32_t e, const ElemParams& P, double D
         D[0] = D[1] = D[2] = D[3] = 1e-6;   // pipeline.c:93
     } else {
         plane_stress(P.elemE ? P.elemE[e] : P.E, P.nu, D);
+    }
+    if (P.dmg) {   // static_damage2D.c:534-540
+        const double d = P.dmg[e];
+        D[0] *= (1.0 - d);
+        D[1] *= (1.0 - d);
+        D[2] *= (1.0 - d);
+        D[3] *= (1.0 - d);
     }
 }
 
@@ -824,6 +832,84 @@ int fem_elem_cross(FemSys& F, const double* a, const double* b, double* out) {
     return NBX_OK;
 }
 
+// ---- quasi-static damage ------------------------------------------------------
+// Strain of each T3 from u (DMG_pipeline_compute_strain,
+// static_damage2D.c:571-631) and tension_damage (:134-185): effective stress
+// D eps, positive principal stresses by Mohr's circle, tau = sqrt(s+' D^-1
+// s+), r = max(r_prev, tau), d = 1 - (r0/r) exp(A (1 - r/r0)) with r0 =
+// ft/sqrt(E), A = 1/((Gf/l)(E/ft^2) - 1/2), l = sqrt(2 area) (get_clfd, :633-647).
+__global__ void k_damage_update(const double2* __restrict__ nod, const uint32_t* __restrict__ conn,
+                                const double* __restrict__ u, uint32_t ne, double E, double nu,
+                                double Gf, double ft, const double* __restrict__ r_prev,
+                                double* __restrict__ r, double* __restrict__ dmg,
+                                double* __restrict__ strain, unsigned* err) {
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= ne) return;
+    const uint32_t n3[3] = {conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]};
+    double detJ, dx[3], dy[3];
+    if (!t3_geom(nod, n3, detJ, dx, dy)) {
+        atomicOr(err, 4u);
+        return;
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const double ux = u[2 * n3[i]], uy = u[2 * n3[i] + 1];
+        s0 += dx[i] * ux;
+        s1 += dy[i] * uy;
+        s2 += (dy[i] * ux + dx[i] * uy);
+    }
+    strain[3 * e] = s0;
+    strain[3 * e + 1] = s1;
+    strain[3 * e + 2] = s2;
+    const double2 a = nod[n3[0]], b = nod[n3[1]], c = nod[n3[2]];
+    const double clfd = sqrt((b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x));
+    double D[4];
+    plane_stress(E, nu, D);
+    const double r0 = ft / sqrt(E);
+    const double e0 = D[0] * s0 + D[1] * s1;
+    const double e1 = D[1] * s0 + D[2] * s1;
+    const double e2 = D[3] * s2;
+    const double avg = 0.5 * (e0 + e1);
+    const double R = sqrt(0.25 * ((e0 - e1) * (e0 - e1)) + e2 * e2);
+    const double p1 = 0.0 > avg + R ? 0.0 : avg + R;
+    const double p2 = 0.0 > avg - R ? 0.0 : avg - R;
+    const double det = D[0] * D[2] - D[1] * D[1];
+    const double i11 = D[2] / det, i12 = -D[1] / det, i22 = D[0] / det;
+    const double sTs = i11 * (p1 * p1) + 2 * i12 * p1 * p2 + i22 * (p2 * p2);
+    const double tau = sqrt(sTs);
+    const double rp = r_prev[e];
+    const double re = rp > tau ? rp : tau;
+    r[e] = re;
+    const double div = (Gf / clfd) * (E / (ft * ft));
+    const double A = 1.0 / (div - 0.5);
+    dmg[e] = 1.0 - (r0 / re) * exp(A * (1.0 - (re / r0)));
+}
+
+// res = F - K u per row (the row product as nb_sparse_multiply_vector forms
+// it), partial sums of res^2
+__global__ __launch_bounds__(BLOCK) void k_residual64(Sell2 K, const double2* __restrict__ F,
+                                                      const double2* __restrict__ u,
+                                                      double2* __restrict__ res, double* partials,
+                                                      uint32_t n) {
+    double acc[1] = {0.0};
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 ku = row64(K, (uint32_t)i, u);
+        const double2 f = F[i];
+        const double2 rv = make_double2(f.x - ku.x, f.y - ku.y);
+        res[i] = rv;
+        acc[0] += rv.x * rv.x + rv.y * rv.y;
+    }
+    __shared__ double red[1][BLOCK / WAVE];
+    block_sum<1>(acc, red);
+    put_partials<1>(acc, partials);
+}
+
+__global__ void k_add_f64(double* __restrict__ x, const double* __restrict__ dx, size_t n) {
+    const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) x[i] += dx[i];
+}
+
 // stress from given strain (static_elasticity2D.c:99-127)
 __global__ void k_stress_only(const double* __restrict__ strain, ElemParams P, uint32_t ne,
                               double* __restrict__ stress) {
@@ -891,6 +977,43 @@ int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* v
                 val[o + 1] = hv[ent(S.n_blocks, idx, i, 1)];
             }
         }
+    }
+    return NBX_OK;
+}
+
+int fem_residual(FemSys& F, double* res, double* norm) {
+    const uint32_t nu = F.K.s.n_nodes;
+    const unsigned G = red_grid(nu);
+    double* out = F.ws.v64[5] + F.ws.n;
+    hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, F.s, F.K.s, (const double2*)F.F,
+                       (const double2*)F.u, (double2*)res, F.ws.partials, nu);
+    hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(FIN), 0, F.s, F.ws.partials, G, out);
+    NBX_HIP(hipGetLastError());
+    double h = 0.0;
+    NBX_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, F.s));
+    NBX_HIP(hipStreamSynchronize(F.s));
+    *norm = sqrt(h);
+    return NBX_OK;
+}
+
+int fem_add(FemSys& F, double* x, const double* dx) {
+    const size_t n = 2 * (size_t)F.nn;
+    hipLaunchKernelGGL(k_add_f64, dim3(grid_for(n)), dim3(BLOCK), 0, F.s, x, dx, n);
+    NBX_HIP(hipGetLastError());
+    return NBX_OK;
+}
+
+int fem_damage_update(FemSys& F, double E, double nu, double Gf, double ft, const double* r_prev,
+                      double* r, double* dmg, double* strain) {
+    NBX_HIP(hipMemsetAsync(F.err, 0, sizeof(unsigned), F.s));
+    hipLaunchKernelGGL(k_damage_update, dim3(grid_for(F.ne)), dim3(BLOCK), 0, F.s, F.nod, F.conn, F.u,
+                       F.ne, E, nu, Gf, ft, r_prev, r, dmg, strain, F.err);
+    NBX_HIP(hipGetLastError());
+    unsigned herr = 0;
+    NBX_TRY(read_err(F.err, F.s, &herr));
+    if (herr & 4u) {
+        set_last_error("distorted element (detJ < 0)");
+        return NBX_ERR_DISTORTED;
     }
     return NBX_OK;
 }

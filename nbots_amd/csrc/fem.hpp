@@ -11,6 +11,7 @@ struct ElemParams {
     double E = 1.0, nu = 0.3, thickness = 1.0;
     const double* elemE = nullptr;     // device, per-element modulus (nullable)
     const uint8_t* enabled = nullptr;  // device, element switch (nullable)
+    const double* dmg = nullptr;       // device, damage scaling D by (1 - d) (nullable)
 };
 
 // Boundary conditions reduced to what the device applies, in the
@@ -67,6 +68,12 @@ int fem_strain_stress(FemSys& F, double* strain, double* stress);
 int fem_compliance(FemSys& F, const double* rho, double penal, double E0, double Emin, double* ce,
                    double* dc, double* C_host);
 int fem_elem_cross(FemSys& F, const double* a, const double* b, double* out);
+// quasi-static damage (static_damage2D.c): res = F - K u and ||res||_2;
+// x += dx; strain, damage state r and damage of every element from u
+int fem_residual(FemSys& F, double* res, double* norm);
+int fem_add(FemSys& F, double* x, const double* dx);
+int fem_damage_update(FemSys& F, double E, double nu, double Gf, double ft, const double* r_prev,
+                      double* r, double* dmg, double* strain);
 int stress_from_strain(uint32_t ne, double E, double nu, const uint8_t* enabled_host,
                        const double* strain_host, double* stress_host);
 int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* val);

@@ -33,6 +33,16 @@ struct nb_fem_elem_s {
 
 struct nb_material_s {
     double E = 0.0, nu = 0.0, density = 0.0;
+    double fracture_energy = 0.0, traction_limit = 0.0, compression_limit = 0.0;
+    double energy_release_rate = 0.0, damage_length_scale = 0.0;
+};
+
+// static_damage2D.c:29-34
+struct nb_fem_implicit_s {
+    uint32_t N_steps = 0;
+    uint32_t N_max_iter = 0;
+    uint32_t N_max_iter_without_enhance = 0;
+    double residual_tolerance = 0.0;
 };
 
 namespace {
@@ -579,6 +589,47 @@ void nb_material_set_density(nb_material_t* m, double v) { m->density = v; }
 double nb_material_get_poisson_module(const nb_material_t* const m) { return m->nu; }
 double nb_material_get_elasticity_module(const nb_material_t* const m) { return m->E; }
 double nb_material_get_density(const nb_material_t* const m) { return m->density; }
+nb_material_t* nb_material_clone(nb_material_t* m) { return new nb_material_s(*m); }
+void nb_material_set_fracture_energy(nb_material_t* m, double v) { m->fracture_energy = v; }
+void nb_material_set_traction_limit_stress(nb_material_t* m, double v) { m->traction_limit = v; }
+void nb_material_set_compression_limit_stress(nb_material_t* m, double v) {
+    m->compression_limit = v;
+}
+void nb_material_set_energy_release_rate(nb_material_t* m, double v) { m->energy_release_rate = v; }
+void nb_material_set_damage_length_scale(nb_material_t* m, double v) { m->damage_length_scale = v; }
+double nb_material_get_fracture_energy(const nb_material_t* const m) { return m->fracture_energy; }
+double nb_material_get_traction_limit_stress(const nb_material_t* const m) {
+    return m->traction_limit;
+}
+double nb_material_get_compression_limit_stress(const nb_material_t* const m) {
+    return m->compression_limit;
+}
+double nb_material_get_energy_release_rate(const nb_material_t* const m) {
+    return m->energy_release_rate;
+}
+double nb_material_get_damage_length_scale(const nb_material_t* const m) {
+    return m->damage_length_scale;
+}
+
+// ---- implicit-integration parameters (static_damage2D.c:79-126) -------------
+nb_fem_implicit_t* nb_fem_implicit_create(void) { return new nb_fem_implicit_s; }
+void nb_fem_implicit_destroy(nb_fem_implicit_t* p) { delete p; }
+void nb_fem_implicit_set_N_steps(nb_fem_implicit_t* p, uint32_t n) { p->N_steps = n; }
+void nb_fem_implicit_set_N_max_iter(nb_fem_implicit_t* p, uint32_t n) { p->N_max_iter = n; }
+void nb_fem_implicit_set_N_max_iter_without_enhance(nb_fem_implicit_t* p, uint32_t n) {
+    p->N_max_iter_without_enhance = n;
+}
+void nb_fem_implicit_set_residual_tolerance(nb_fem_implicit_t* p, double t) {
+    p->residual_tolerance = t;
+}
+uint32_t nb_fem_implicit_get_N_steps(nb_fem_implicit_t* p) { return p->N_steps; }
+uint32_t nb_fem_implicit_get_N_max_iter(nb_fem_implicit_t* p) { return p->N_max_iter; }
+uint32_t nb_fem_implicit_get_N_max_iter_without_enhance(nb_fem_implicit_t* p) {
+    return p->N_max_iter_without_enhance;
+}
+double nb_fem_implicit_get_residual_tolerance(nb_fem_implicit_t* p) {
+    return p->residual_tolerance;
+}
 
 uint16_t nb_bcond_get_memsize(uint8_t) { return (uint16_t)sizeof(nb_bcond_s); }
 void nb_bcond_init(void* p, uint8_t N_dof) {
@@ -621,6 +672,35 @@ void nb_bcond_push_function(nb_bcond_t* b, nb_bcond_id t, nb_bcond_where w, uint
 
 // static_elasticity2D.c:31-97 on the GPU: pattern, assembly, BCs, Jacobi-PCG
 // with x = 0, max_iter = N, tol = 1e-8 (absolute), strain.
+}  // extern "C"
+
+// The boundary conditions of a bcond object on a mesh, values scaled by
+// factor (nb_fem_set_bconditions, set_bconditions.c:52-61); function-valued
+// Neumann segments are sampled at their nodes at time 0.
+static int plan_for(const nb_mesh2D_t* part, const nb_bcond_t* bcond, double factor,
+                    nbx::BcPlan& plan) {
+    std::vector<std::vector<double>> buf(bcond->ns.size());
+    std::vector<const double*> samples(bcond->ns.size(), nullptr);
+    for (size_t c = 0; c < bcond->ns.size(); ++c) {
+        const BcAtom& a = bcond->ns[c];
+        if (!a.fval || a.id >= nb_mesh2D_get_N_insgm(part)) continue;
+        const uint32_t N = nb_mesh2D_insgm_get_N_nodes(part, a.id);
+        buf[c].resize(2 * (size_t)N);
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint32_t v = nb_mesh2D_insgm_get_node(part, a.id, i);
+            const double x[2] = {part->nod[2 * v], part->nod[2 * v + 1]};
+            a.fval(x, 0.0, &buf[c][2 * i]);
+        }
+        samples[c] = buf[c].data();
+    }
+    return plan_from_lists(part->nod.data(), nb_mesh2D_get_N_nodes(part), part->vtx.data(),
+                           (uint32_t)part->vtx.size(), part->sgm_ptr.data(),
+                           part->sgm_nodes.data(), nb_mesh2D_get_N_insgm(part), bcond->dv,
+                           bcond->nv, bcond->ds, bcond->ns, samples, factor, plan);
+}
+
+extern "C" {
+
 int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t* const part,
                                       const nb_fem_elem_t* const elemtype,
                                       const nb_material_t* const material,
@@ -659,26 +739,8 @@ int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t* const part,
     }
     if (rc) goto FAIL;
     {
-        // function-valued Neumann segments are sampled at their nodes
-        std::vector<std::vector<double>> buf(bcond->ns.size());
-        std::vector<const double*> samples(bcond->ns.size(), nullptr);
-        for (size_t c = 0; c < bcond->ns.size(); ++c) {
-            const BcAtom& a = bcond->ns[c];
-            if (!a.fval || a.id >= nb_mesh2D_get_N_insgm(part)) continue;
-            const uint32_t N = nb_mesh2D_insgm_get_N_nodes(part, a.id);
-            buf[c].resize(2 * (size_t)N);
-            for (uint32_t i = 0; i < N; ++i) {
-                const uint32_t v = nb_mesh2D_insgm_get_node(part, a.id, i);
-                const double x[2] = {part->nod[2 * v], part->nod[2 * v + 1]};
-                a.fval(x, 0.0, &buf[c][2 * i]);
-            }
-            samples[c] = buf[c].data();
-        }
         nbx::BcPlan plan;
-        rc = plan_from_lists(part->nod.data(), nn, part->vtx.data(), (uint32_t)part->vtx.size(),
-                             part->sgm_ptr.data(), part->sgm_nodes.data(),
-                             nb_mesh2D_get_N_insgm(part), bcond->dv, bcond->nv, bcond->ds,
-                             bcond->ns, samples, 1.0, plan);
+        rc = plan_for(part, bcond, 1.0, plan);
         if (rc) goto FAIL;
         rc = nbx::fem_apply_bc(f->sys, plan);
         if (rc) goto FAIL;
@@ -699,6 +761,147 @@ FAIL:
     fprintf(stderr, "nb_fem_compute_2D_Solid_Mechanics: GPU failure: %s\n", nbx::last_error());
     nbx_fem_destroy(f);
     return -1;
+}
+
+// Quasi-static damage (static_damage2D.c:239-472) on the device, with the
+// conjugate-gradient path: for each load step n the boundary values are
+// scaled by (n+1)/N_steps, and the iteration assembles K with D (1 - d),
+// computes F - K u, stops on (||F - K u|| < tol after the first iteration)
+// or N_max_iter or N_max_iter_without_enhance iterations without a new best
+// residual, else solves K du = F - K u with Jacobi-PCG (absolute tol 1e-8,
+// N iterations, from the previous du; du = 0 at the start of a step),
+// u += du, and updates strain and damage.  The log has the reference's lines.
+// The direct-solver option runs the same PCG (its factorisations are not on
+// this path).
+int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
+                         const nb_material_t* material, const nb_bcond_t* bcond,
+                         int enable_self_weight, const double* gravity,
+                         const nb_analysis2D_params* params2D, nb_fem_implicit_t* params,
+                         const char* logfile, double* displacement, double* damage) {
+    if (elemtype->type != NB_TRG_LINEAR || bcond->N_dof != 2) {
+        nbx::set_last_error("the damage solver needs linear triangles with 2 dofs per node");
+        return nbx::NBX_ERR_ARG;
+    }
+    const uint32_t nn = nb_mesh2D_get_N_nodes(part), ne = nb_mesh2D_get_N_elems(part);
+    const uint32_t N = 2 * nn;
+    FILE* log = logfile ? fopen(logfile, "a") : nullptr;
+    auto logf = [&](const char* fmt, auto... a) {
+        if (log) {
+            fprintf(log, fmt, a...);
+            fflush(log);
+        }
+    };
+    logf("FEM: Damage Model\n");
+    nbx_fem_t* f = nullptr;
+    int rc = nbx_fem_create(&f, nn, part->nod.data(), ne, part->adj.data());
+    double *dmg = nullptr, *r = nullptr, *r_prev = nullptr, *strain = nullptr, *res = nullptr,
+           *du = nullptr;
+    const double E = material->E, nu = material->nu;
+    const double Gf = material->fracture_energy, ft = material->traction_limit;
+    const double g0[2] = {0.0, 0.0};
+    const double* grav = gravity ? gravity : g0;
+    if (rc) goto END;
+    {
+        nbx::FemSys& S = f->sys;
+        const size_t eb = (size_t)(ne ? ne : 1) * 8, vb = (size_t)N * 8 + 8;
+        if (hipMalloc(&dmg, eb) != hipSuccess || hipMalloc(&r, eb) != hipSuccess ||
+            hipMalloc(&r_prev, eb) != hipSuccess || hipMalloc(&strain, 3 * eb) != hipSuccess ||
+            hipMalloc(&res, vb) != hipSuccess || hipMalloc(&du, vb) != hipSuccess) {
+            nbx::set_last_error("out of device memory");
+            rc = nbx::NBX_ERR_HIP;
+            goto END;
+        }
+        {
+            std::vector<double> r0(ne, ft / sqrt(E));   // :275-276
+            if (hipMemcpy(r, r0.data(), (size_t)ne * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(dmg, 0, eb) != hipSuccess || hipMemset(S.u, 0, vb - 8) != hipSuccess) {
+                rc = nbx::NBX_ERR_HIP;
+                goto END;
+            }
+        }
+        S.P = nbx::ElemParams();
+        S.P.E = E;
+        S.P.nu = nu;
+        S.P.thickness = params2D->thickness;
+        S.P.dmg = dmg;
+        for (uint32_t n = 0; n < params->N_steps; ++n) {
+            logf("  [ Load step %i]\n", n + 1);
+            if (hipMemcpyAsync(r_prev, r, (size_t)ne * 8, hipMemcpyDeviceToDevice, S.s) !=
+                    hipSuccess ||
+                hipMemsetAsync(du, 0, vb - 8, S.s) != hipSuccess) {
+                rc = nbx::NBX_ERR_HIP;
+                goto END;
+            }
+            uint32_t it = 0, wo = 0;
+            double best = 0.0;
+            const double factor = (n + 1.0) / (double)params->N_steps;
+            for (;;) {
+                if ((rc = nbx::fem_assemble_device(S))) goto END;
+                if ((rc = nbx::fem_body_force(S, enable_self_weight, material->density, grav)))
+                    goto END;
+                nbx::BcPlan plan;
+                if ((rc = plan_for(part, bcond, factor, plan))) goto END;
+                if ((rc = nbx::fem_apply_bc(S, plan))) goto END;
+                double nrm = 0.0;
+                if ((rc = nbx::fem_residual(S, res, &nrm))) goto END;
+                if (it == 0) {
+                    best = nrm;
+                } else if (nrm < best) {
+                    best = nrm;
+                    wo = 0;
+                } else {
+                    wo += 1;
+                }
+                if ((nrm < params->residual_tolerance && it > 0) || it >= params->N_max_iter ||
+                    wo >= params->N_max_iter_without_enhance) {
+                    logf("    [%i: Residual: %e ]\n", it, nrm);
+                    break;
+                }
+                uint32_t iters = 0;
+                double err = 0.0;
+                const int st = nbx::pcg_jacobi(S.K, res, du, N, 1e-8, &iters, &err, S.ws, S.s,
+                                               nullptr);
+                if (st < 0) {
+                    rc = -st;
+                    goto END;
+                }
+                if (st != 0)   // the reference then tries Gauss-Seidel (:417-427)
+                    logf("    [Conjugate Gradient (p. Jacobi) did not converge: %u iterations, "
+                         "error %e ]\n", iters, err);
+                logf("    [%i: Residual: %e]\n", it, nrm);
+                if ((rc = nbx::fem_add(S, S.u, du))) goto END;
+                if ((rc = nbx::fem_damage_update(S, E, nu, Gf, ft, r_prev, r, dmg, strain)))
+                    goto END;
+                ++it;
+            }
+        }
+        if (displacement &&
+            hipMemcpy(displacement, S.u, (size_t)N * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = nbx::NBX_ERR_HIP;
+        if (!rc && damage && hipMemcpy(damage, dmg, (size_t)ne * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = nbx::NBX_ERR_HIP;
+    }
+END:
+    if (log) fclose(log);
+    void* ps[] = {dmg, r, r_prev, strain, res, du};
+    for (void* q : ps)
+        if (q) (void)hipFree(q);
+    if (f) {
+        f->sys.P.dmg = nullptr;
+        nbx_fem_destroy(f);
+    }
+    return rc;
+}
+
+void nb_fem_compute_2D_Non_Linear_Solid_Mechanics(
+    const nb_mesh2D_t* const part, const nb_fem_elem_t* const elemtype,
+    const nb_material_t* const material, const nb_bcond_t* const bcond, bool enable_self_weight,
+    double gravity[2], bool, nb_analysis2D_t, nb_analysis2D_params* params2D,
+    nb_fem_implicit_t* params, const char* logfile) {
+    const int rc = nbx_fem_damage_solve(part, elemtype, material, bcond, enable_self_weight ? 1 : 0,
+                                        gravity, params2D, params, logfile, nullptr, nullptr);
+    if (rc)
+        fprintf(stderr, "nb_fem_compute_2D_Non_Linear_Solid_Mechanics: %s\n", nbx::last_error());
 }
 
 void nb_fem_compute_stress_from_strain(uint32_t N_elements, const nb_fem_elem_t* const,

@@ -217,50 +217,99 @@ class Simp:
         return ptr, idx, val, hs
 
 
+class _Params2D(C.Structure):
+    """nb_analysis2D_params (analysis2D.h:10-14)."""
+    _fields_ = [("thickness", C.c_double), ("revolution_axe", C.c_double),
+                ("revolution_coordinate", C.c_char)]
+
+
+class _Objects:
+    """The reference's object API for a Problem: mesh, element type,
+    material and boundary conditions (freed on exit)."""
+
+    def __init__(self, p, density=0.0, Gf=0.0, ft=0.0):
+        L = self.L = load()
+        self.mesh = L.nb_mesh2D_create(0)
+        check(L.nbx_mesh2D_set_trg(self.mesh, p.N_nod, _c(p.nod, np.float64), p.N_elem,
+                                   _c(p.conn, np.uint32), p.vtx.size, _c(p.vtx, np.uint32),
+                                   p.sgm_ptr.size - 1, _c(p.sgm_ptr, np.uint32),
+                                   _c(p.sgm_nodes, np.uint32)), "nbx_mesh2D_set_trg")
+        self.elem = L.nb_fem_elem_create(0)
+        self.mat = L.nb_material_create()
+        L.nb_material_set_elasticity_module(self.mat, p.E)
+        L.nb_material_set_poisson_module(self.mat, p.nu)
+        L.nb_material_set_density(self.mat, density)
+        L.nb_material_set_fracture_energy(self.mat, Gf)
+        L.nb_material_set_traction_limit_stress(self.mat, ft)
+        self.bc = L.nb_bcond_create(2)
+        for k, (t, w) in (("dv", (0, 0)), ("nv", (1, 0)), ("ds", (0, 1)), ("ns", (1, 1))):
+            ids, m, v = getattr(p, k)
+            for c in range(ids.size):
+                mask = (C.c_bool * 2)(bool(m[2 * c]), bool(m[2 * c + 1]))
+                val = (C.c_double * 2)(v[2 * c], v[2 * c + 1])
+                L.nb_bcond_push(self.bc, t, w, int(ids[c]), mask, val)
+        self.params = _Params2D(p.thickness, 0.0, b"\0")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        L = self.L
+        L.nb_bcond_destroy(self.bc)
+        L.nb_material_destroy(self.mat)
+        L.nb_fem_elem_destroy(self.elem)
+        L.nb_mesh2D_destroy(self.mesh)
+
+
 def compute_2D_solid_mechanics(p, enabled=None, self_weight=False, gravity=(0.0, 0.0),
                                density=0.0):
     """Run the drop-in nb_fem_compute_2D_Solid_Mechanics on a Problem through
     the reference's object API (mesh, element, material, bcond)."""
-    L = load()
-    mesh = L.nb_mesh2D_create(0)
-    check(L.nbx_mesh2D_set_trg(mesh, p.N_nod, _c(p.nod, np.float64), p.N_elem,
-                               _c(p.conn, np.uint32), p.vtx.size, _c(p.vtx, np.uint32),
-                               p.sgm_ptr.size - 1, _c(p.sgm_ptr, np.uint32),
-                               _c(p.sgm_nodes, np.uint32)), "nbx_mesh2D_set_trg")
-    elem = L.nb_fem_elem_create(0)
-    mat = L.nb_material_create()
-    L.nb_material_set_elasticity_module(mat, p.E)
-    L.nb_material_set_poisson_module(mat, p.nu)
-    L.nb_material_set_density(mat, density)
-    bc = L.nb_bcond_create(2)
-    for k, (t, w) in (("dv", (0, 0)), ("nv", (1, 0)), ("ds", (0, 1)), ("ns", (1, 1))):
-        ids, m, v = getattr(p, k)
-        for c in range(ids.size):
-            mask = (C.c_bool * 2)(bool(m[2 * c]), bool(m[2 * c + 1]))
-            val = (C.c_double * 2)(v[2 * c], v[2 * c + 1])
-            L.nb_bcond_push(bc, t, w, int(ids[c]), mask, val)
-
-    class Params(C.Structure):
-        _fields_ = [("thickness", C.c_double), ("revolution_axe", C.c_double),
-                    ("revolution_coordinate", C.c_char)]
-    params = Params(p.thickness, 0.0, b"\0")
-    en = None
-    if enabled is not None:
-        en = (C.c_bool * p.N_elem)(*[bool(x) for x in enabled])
-    disp = np.zeros(p.N)
-    strain = np.zeros(3 * p.N_elem)
-    en_p = None if en is None else C.cast(en, C.c_void_p)
-    st = L.nb_fem_compute_2D_Solid_Mechanics(mesh, elem, mat, bc, bool(self_weight),
-                                             _c(gravity, np.float64), 0,
-                                             C.cast(C.pointer(params), C.c_void_p),
-                                             en_p, disp, strain)
-    stress = np.zeros(3 * p.N_elem)
-    if st == 0:
-        L.nb_fem_compute_stress_from_strain(p.N_elem, elem, mat, 0, strain, en_p, stress)
-    L.nb_bcond_destroy(bc)
-    L.nb_material_destroy(mat)
-    L.nb_fem_elem_destroy(elem)
-    L.nb_mesh2D_destroy(mesh)
+    with _Objects(p, density) as o:
+        L = o.L
+        en = None
+        if enabled is not None:
+            en = (C.c_bool * p.N_elem)(*[bool(x) for x in enabled])
+        disp = np.zeros(p.N)
+        strain = np.zeros(3 * p.N_elem)
+        en_p = None if en is None else C.cast(en, C.c_void_p)
+        st = L.nb_fem_compute_2D_Solid_Mechanics(o.mesh, o.elem, o.mat, o.bc, bool(self_weight),
+                                                 _c(gravity, np.float64), 0,
+                                                 C.cast(C.pointer(o.params), C.c_void_p),
+                                                 en_p, disp, strain)
+        stress = np.zeros(3 * p.N_elem)
+        if st == 0:
+            L.nb_fem_compute_stress_from_strain(p.N_elem, o.elem, o.mat, 0, strain, en_p, stress)
     if st < 0:
         raise RuntimeError(f"nb_fem_compute_2D_Solid_Mechanics: {last_error()}")
     return st, disp, strain, stress
+
+
+def compute_2D_damage(p, Gf, ft, n_steps, max_iter, max_iter_wo_enhance, res_tol, logfile=None,
+                      drop_in=False):
+    """The quasi-static damage solver on a Problem through the object API:
+    drop_in=True calls nb_fem_compute_2D_Non_Linear_Solid_Mechanics (its
+    only output is the log); otherwise nbx_fem_damage_solve, which also
+    returns the final displacement and damage."""
+    with _Objects(p, 0.0, Gf, ft) as o:
+        L = o.L
+        imp = L.nb_fem_implicit_create()
+        L.nb_fem_implicit_set_N_steps(imp, n_steps)
+        L.nb_fem_implicit_set_N_max_iter(imp, max_iter)
+        L.nb_fem_implicit_set_N_max_iter_without_enhance(imp, max_iter_wo_enhance)
+        L.nb_fem_implicit_set_residual_tolerance(imp, res_tol)
+        g = np.zeros(2)
+        lf = None if logfile is None else logfile.encode()
+        u = np.zeros(p.N)
+        d = np.zeros(p.N_elem)
+        if drop_in:
+            L.nb_fem_compute_2D_Non_Linear_Solid_Mechanics(
+                o.mesh, o.elem, o.mat, o.bc, False, g, False, 0,
+                C.cast(C.pointer(o.params), C.c_void_p), imp, lf)
+            rc = 0
+        else:
+            rc = L.nbx_fem_damage_solve(o.mesh, o.elem, o.mat, o.bc, 0, g,
+                                        C.cast(C.pointer(o.params), C.c_void_p), imp, lf, u, d)
+        L.nb_fem_implicit_destroy(imp)
+    check(rc, "nbx_fem_damage_solve")
+    return u, d

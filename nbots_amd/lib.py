@@ -120,6 +120,30 @@ SIGS = {
     "nb_material_get_poisson_module": (F64, [VP]),
     "nb_material_get_elasticity_module": (F64, [VP]),
     "nb_material_get_density": (F64, [VP]),
+    "nb_material_clone": (VP, [VP]),
+    "nb_material_set_fracture_energy": (None, [VP, F64]),
+    "nb_material_set_traction_limit_stress": (None, [VP, F64]),
+    "nb_material_set_compression_limit_stress": (None, [VP, F64]),
+    "nb_material_set_energy_release_rate": (None, [VP, F64]),
+    "nb_material_set_damage_length_scale": (None, [VP, F64]),
+    "nb_material_get_fracture_energy": (F64, [VP]),
+    "nb_material_get_traction_limit_stress": (F64, [VP]),
+    "nb_material_get_compression_limit_stress": (F64, [VP]),
+    "nb_material_get_energy_release_rate": (F64, [VP]),
+    "nb_material_get_damage_length_scale": (F64, [VP]),
+    "nb_fem_implicit_create": (VP, []),
+    "nb_fem_implicit_destroy": (None, [VP]),
+    "nb_fem_implicit_set_N_steps": (None, [VP, U32]),
+    "nb_fem_implicit_set_N_max_iter": (None, [VP, U32]),
+    "nb_fem_implicit_set_N_max_iter_without_enhance": (None, [VP, U32]),
+    "nb_fem_implicit_set_residual_tolerance": (None, [VP, F64]),
+    "nb_fem_implicit_get_N_steps": (U32, [VP]),
+    "nb_fem_implicit_get_N_max_iter": (U32, [VP]),
+    "nb_fem_implicit_get_N_max_iter_without_enhance": (U32, [VP]),
+    "nb_fem_implicit_get_residual_tolerance": (F64, [VP]),
+    "nb_fem_compute_2D_Non_Linear_Solid_Mechanics": (None, [VP, VP, VP, VP, C.c_bool, f64p,
+                                                            C.c_bool, I32, VP, VP, C.c_char_p]),
+    "nbx_fem_damage_solve": (I32, [VP, VP, VP, VP, I32, f64o, VP, VP, C.c_char_p, f64o, f64o]),
     "nb_bcond_get_memsize": (C.c_uint16, [C.c_uint8]),
     "nb_bcond_init": (None, [VP, C.c_uint8]),
     "nb_bcond_finish": (None, [VP]),

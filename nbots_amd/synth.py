@@ -164,6 +164,18 @@ def cantilever_traction(nx: int, ny: int, seed: int = DEFAULT_SEED,
                    name=f"cantilever_traction_{nx}x{ny}")
 
 
+def damage_beam(nx: int = 24, ny: int = 12, jitter: float = 0.1, deflection: float = -0.6,
+                seed: int = DEFAULT_SEED) -> Problem:
+    """Jittered beam for the quasi-static damage solver: left edge clamped,
+    right edge given the vertical displacement `deflection` (a Dirichlet
+    segment the solver scales by the load factor of each step), E=1,
+    nu=0.3, thickness 1."""
+    nod, conn, vtx, sp, sn = grid(nx, ny, jitter=jitter, seed=seed)
+    return Problem(nx, ny, nod, conn, vtx, sp, sn,
+                   ds=_bc([3, 1], [1, 1, 0, 1], [0.0, 0.0, 0.0, deflection]),
+                   name=f"damage_{nx}x{ny}")
+
+
 def simp_modulus(rho: np.ndarray, penal: float = 3.0, E0: float = 1.0,
                  Emin: float = 1e-9) -> np.ndarray:
     return Emin + rho ** penal * (E0 - Emin)

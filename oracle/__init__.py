@@ -79,6 +79,14 @@ def lib():
         L.orc_pcg_jacobi.restype = I32
         L.orc_pcg_jacobi.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
                                      C.POINTER(U32), C.POINTER(F64)]
+        L.orc_assemble_t3_damage.restype = I32
+        L.orc_assemble_t3_damage.argtypes = [U32, f64p, U32, u32p, F64, F64, f64p, F64, I32, F64,
+                                             f64o, u32p, u32p, f64o, f64o]
+        L.orc_residual.restype = F64
+        L.orc_residual.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, f64o]
+        L.orc_damage_update.restype = I32
+        L.orc_damage_update.argtypes = [U32, f64p, u32p, f64p, F64, F64, F64, F64, f64p, f64o,
+                                        f64o, f64o]
         L.orc_cg.restype = I32
         L.orc_cg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
                              C.POINTER(U32), C.POINTER(F64)]
@@ -122,6 +130,9 @@ def ref():
                                                 I32, u32p, u32o, f64o, f64o]
         R.ref_fem_compute.restype = I32
         R.ref_fem_compute.argtypes = mesh + bcs + [F64, F64, F64, u8o, f64p, f64p, f64p]
+        R.ref_damage_compute.restype = None
+        R.ref_damage_compute.argtypes = mesh + bcs + [F64, F64, F64, F64, F64, U32, U32, U32, F64,
+                                                      C.c_char_p]
         R.ref_pcg.restype = I32
         R.ref_pcg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64, U32,
                               C.POINTER(U32), C.POINTER(F64)]
@@ -230,6 +241,72 @@ def cg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8):
     st = lib().orc_cg(N, rowptr, col, val, _c(b, np.float64), x,
                       N if max_iter is None else max_iter, tol, C.byref(it), C.byref(tr))
     return x, it.value, tr.value, st
+
+
+def damage_solve(p, Gf, ft, n_steps, max_iter, max_iter_wo_enhance, res_tol, logfile=None):
+    """nb_fem_compute_2D_Non_Linear_Solid_Mechanics (static_damage2D.c:239-472)
+    on the restatement, with the conjugate-gradient path
+    (enable_Cholesky_solver false), plane stress, no self weight: load
+    factor (n+1)/n_steps on the boundary conditions, residual F - K u of the
+    damaged K, Jacobi-PCG (tol 1e-8, max_iter N) for du from the previous du,
+    u += du, strain and damage update.  Writes the reference's log lines
+    (to logfile if given) and returns u, damage, r and the log text."""
+    L = lib()
+    N = p.N
+    rp, col = pattern(p.N_nod, p.conn)
+    nod, conn = _c(p.nod, np.float64), _c(p.conn, np.uint32)
+    u = np.zeros(N)
+    du = np.zeros(N)
+    dmg = np.zeros(p.N_elem)
+    r = np.full(p.N_elem, ft / np.sqrt(p.E))
+    strain_ = np.zeros(3 * p.N_elem)
+    res = np.zeros(N)
+    lines = ["FEM: Damage Model"]
+    for n in range(n_steps):
+        lines.append(f"  [ Load step {n + 1}]")
+        r_prev = r.copy()
+        du[:] = 0.0
+        it = 0
+        wo = 0
+        best = 0.0
+        while True:
+            val = np.zeros(rp[-1])
+            F = np.zeros(N)
+            st = L.orc_assemble_t3_damage(p.N_nod, nod, p.N_elem, conn, p.E, p.nu, dmg,
+                                          p.thickness, 0, 0.0, None, rp, col, val, F)
+            if st:
+                raise RuntimeError("distorted element")
+            set_bconditions(p, rp, col, val, F, factor=(n + 1.0) / float(n_steps))
+            nrm = L.orc_residual(N, rp, col, val, F, u, res)
+            if it == 0:
+                best = nrm
+            elif nrm < best:
+                best = nrm
+                wo = 0
+            else:
+                wo += 1
+            if (nrm < res_tol and it > 0) or it >= max_iter or wo >= max_iter_wo_enhance:
+                lines.append("    [%i: Residual: %s ]" % (it, _fmt_e(nrm)))
+                break
+            du, _, _, st = pcg(rp, col, val, res, x0=du, max_iter=N, tol=1e-8)
+            if st != 0:
+                # the reference then runs Gauss-Seidel (static_damage2D.c:417-427)
+                raise NotImplementedError("Jacobi-PCG did not converge in a damage iteration")
+            lines.append("    [%i: Residual: %s]" % (it, _fmt_e(nrm)))
+            u += du
+            L.orc_damage_update(p.N_elem, nod, conn, u, p.E, p.nu, Gf, ft, r_prev, r, dmg,
+                                strain_)
+            it += 1
+    text = "\n".join(lines) + "\n"
+    if logfile:
+        with open(logfile, "a") as f:
+            f.write(text)
+    return dict(u=u, damage=dmg, r=r, strain=strain_, log=text)
+
+
+def _fmt_e(v):
+    """C's %e."""
+    return "%e" % v
 
 
 def strain(p, u):
@@ -347,6 +424,13 @@ def ref_fem_compute(p, enabled=None):
                                None if enabled is None else _c(enabled, np.uint8),
                                disp, eps, sig)
     return st, disp, eps, sig
+
+
+def ref_damage_compute(p, Gf, ft, n_steps, max_iter, max_iter_wo_enhance, res_tol, logfile):
+    """The compiled reference's nb_fem_compute_2D_Non_Linear_Solid_Mechanics;
+    appends its log to logfile."""
+    ref().ref_damage_compute(*_mesh_args(p), *_bc_args(p), p.E, p.nu, p.thickness, Gf, ft,
+                             n_steps, max_iter, max_iter_wo_enhance, res_tol, logfile.encode())
 
 
 def ref_pcg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8, threads=1):

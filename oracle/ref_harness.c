@@ -24,6 +24,7 @@
 #include "nb/solver_bot.h"
 #include "nb/geometric_bot.h"
 #include "nb/pde_bot.h"
+#include "nb/pde_bot/finite_element/solid_mechanics/static_damage2D.h"
 #include "nb/io_bot.h"
 
 /* private headers of the reference, visible at compile time here */
@@ -341,6 +342,57 @@ int ref_fem_compute(uint32_t N_nod, const double *nod,
 	nb_bcond_destroy(bc);
 	mesh_free(&h);
 	return status;
+}
+
+/* The reference's quasi-static damage solver on the same inputs, with the
+ * conjugate-gradient path; its only output is the log file. */
+void ref_damage_compute(uint32_t N_nod, const double *nod,
+			uint32_t N_elem, const uint32_t *conn,
+			uint32_t N_vtx, const uint32_t *vtx,
+			uint32_t N_sgm, const uint32_t *sgm_ptr,
+			const uint32_t *sgm_nodes,
+			uint32_t n_dv, const uint32_t *dv_id,
+			const uint8_t *dv_mask, const double *dv_val,
+			uint32_t n_nv, const uint32_t *nv_id,
+			const uint8_t *nv_mask, const double *nv_val,
+			uint32_t n_ds, const uint32_t *ds_id,
+			const uint8_t *ds_mask, const double *ds_val,
+			uint32_t n_ns, const uint32_t *ns_id,
+			const uint8_t *ns_mask, const double *ns_val,
+			double E, double nu, double thickness,
+			double Gf, double ft, uint32_t N_steps,
+			uint32_t N_max_iter, uint32_t N_max_iter_wo,
+			double residual_tol, const char *logfile)
+{
+	hmesh_t h;
+	mesh_build(&h, N_nod, nod, N_elem, conn, N_vtx, vtx, N_sgm, sgm_ptr,
+		   sgm_nodes);
+	nb_bcond_t *bc = bcond_build(n_dv, dv_id, dv_mask, dv_val,
+				     n_nv, nv_id, nv_mask, nv_val,
+				     n_ds, ds_id, ds_mask, ds_val,
+				     n_ns, ns_id, ns_mask, ns_val);
+	nb_material_t *mat = nb_material_create();
+	nb_material_set_elasticity_module(mat, E);
+	nb_material_set_poisson_module(mat, nu);
+	nb_material_set_fracture_energy(mat, Gf);
+	nb_material_set_traction_limit_stress(mat, ft);
+	nb_fem_elem_t *elem = nb_fem_elem_create(NB_TRG_LINEAR);
+	nb_analysis2D_params params = {thickness, 0, 0};
+	nb_fem_implicit_t *imp = nb_fem_implicit_create();
+	nb_fem_implicit_set_N_steps(imp, N_steps);
+	nb_fem_implicit_set_N_max_iter(imp, N_max_iter);
+	nb_fem_implicit_set_N_max_iter_without_enhance(imp, N_max_iter_wo);
+	nb_fem_implicit_set_residual_tolerance(imp, residual_tol);
+	double gravity[2] = {0.0, 0.0};
+	nb_fem_compute_2D_Non_Linear_Solid_Mechanics(h.part, elem, mat, bc,
+						     false, gravity, false,
+						     NB_PLANE_STRESS, &params,
+						     imp, logfile);
+	nb_fem_implicit_destroy(imp);
+	nb_fem_elem_destroy(elem);
+	nb_material_destroy(mat);
+	nb_bcond_destroy(bc);
+	mesh_free(&h);
 }
 
 int ref_pcg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,

@@ -256,13 +256,14 @@ static void sparse_add(const uint32_t *rowptr, const uint32_t *col,
  * (scatter).  Extension: elem_E (nullable) replaces the material modulus per
  * element, the per-element stiffness scaling that static_damage2D.c:531-540
  * applies per Gauss point.  Returns 0 or 1 like pipeline_assemble_system. */
-int orc_assemble_t3(uint32_t N_nod, const double *nod,
-		    uint32_t N_elem, const uint32_t *conn,
-		    double E, double nu, const double *elem_E,
-		    double thickness, const uint8_t *enabled,
-		    int self_weight, double density, const double *gravity,
-		    const uint32_t *rowptr, const uint32_t *col, double *val,
-		    double *F)
+static int assemble_core(uint32_t N_nod, const double *nod,
+			 uint32_t N_elem, const uint32_t *conn,
+			 double E, double nu, const double *elem_E,
+			 const double *damage,
+			 double thickness, const uint8_t *enabled,
+			 int self_weight, double density, const double *gravity,
+			 const uint32_t *rowptr, const uint32_t *col, double *val,
+			 double *F)
 {
 	uint32_t N = 2 * N_nod;
 	memset(val, 0, (size_t)rowptr[N] * sizeof(*val));
@@ -275,6 +276,12 @@ int orc_assemble_t3(uint32_t N_nod, const double *nod,
 			double Ee = (NULL == elem_E) ? E : elem_E[e];
 			orc_plane_stress_D(Ee, nu, D);
 			rho = density;
+		}
+		if (NULL != damage) {   /* static_damage2D.c:534-540 */
+			D[0] *= (1.0 - damage[e]);
+			D[1] *= (1.0 - damage[e]);
+			D[2] *= (1.0 - damage[e]);
+			D[3] *= (1.0 - damage[e]);
 		}
 		double fx = 0.0, fy = 0.0;
 		if (self_weight) {
@@ -313,6 +320,101 @@ int orc_assemble_t3(uint32_t N_nod, const double *nod,
 			F[v1 * 2] += Fe[i * 2];
 			F[v1 * 2 + 1] += Fe[i * 2 + 1];
 		}
+	}
+	return 0;
+}
+
+int orc_assemble_t3(uint32_t N_nod, const double *nod,
+		    uint32_t N_elem, const uint32_t *conn,
+		    double E, double nu, const double *elem_E,
+		    double thickness, const uint8_t *enabled,
+		    int self_weight, double density, const double *gravity,
+		    const uint32_t *rowptr, const uint32_t *col, double *val,
+		    double *F)
+{
+	return assemble_core(N_nod, nod, N_elem, conn, E, nu, elem_E, NULL,
+			     thickness, enabled, self_weight, density, gravity,
+			     rowptr, col, val, F);
+}
+
+/* DMG_pipeline_assemble_system (static_damage2D.c:474-569): the same
+ * element loop with the constitutive matrix of each Gauss point (one per
+ * T3) scaled by (1 - damage). */
+int orc_assemble_t3_damage(uint32_t N_nod, const double *nod,
+			   uint32_t N_elem, const uint32_t *conn,
+			   double E, double nu, const double *damage,
+			   double thickness, int self_weight, double density,
+			   const double *gravity, const uint32_t *rowptr,
+			   const uint32_t *col, double *val, double *F)
+{
+	return assemble_core(N_nod, nod, N_elem, conn, E, nu, NULL, damage,
+			     thickness, NULL, self_weight, density, gravity,
+			     rowptr, col, val, F);
+}
+
+/* F - K u and its 2-norm summed in row order (static_damage2D.c:351-360). */
+double orc_residual(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		    const double *val, const double *F, const double *u,
+		    double *res)
+{
+	double nrm = 0;
+	for (uint32_t i = 0; i < N; i++) {
+		double sum = 0;
+		for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; s++)
+			sum += val[s] * u[col[s]];
+		res[i] = F[i] - sum;
+		nrm += POW2(res[i]);
+	}
+	return sqrt(nrm);
+}
+
+/* DMG_pipeline_compute_strain + tension_damage (static_damage2D.c:134-185,
+ * 571-631, get_clfd :633-647): strain of each T3, effective stress D eps,
+ * positive principal stresses by Mohr's circle, tau = sqrt(s+' D^-1 s+),
+ * r = max(r_prev, tau), damage = 1 - (r0/r) exp(A (1 - r/r0)) with
+ * r0 = ft/sqrt(E), A = 1/((Gf/l) (E/ft^2) - 1/2), l = sqrt(2 area). */
+int orc_damage_update(uint32_t N_elem, const double *nod, const uint32_t *conn,
+		      const double *u, double E, double nu, double Gf, double ft,
+		      const double *r_prev, double *r, double *damage,
+		      double *strain)
+{
+	double D[4];
+	orc_plane_stress_D(E, nu, D);
+	double r0 = ft / sqrt(E);
+	memset(strain, 0, 3 * (size_t)N_elem * sizeof(*strain));
+	for (uint32_t e = 0; e < N_elem; e++) {
+		double x[3], y[3];
+		for (int i = 0; i < 3; i++) {
+			x[i] = nod[conn[e * 3 + i] * 2];
+			y[i] = nod[conn[e * 3 + i] * 2 + 1];
+		}
+		double detJ, dNdx[3], dNdy[3];
+		if (orc_t3_derivatives(x, y, &detJ, dNdx, dNdy))
+			return 1;
+		double *eps = strain + 3 * (size_t)e;
+		for (int i = 0; i < 3; i++) {
+			uint32_t n = conn[e * 3 + i];
+			eps[0] += dNdx[i] * u[n * 2];
+			eps[1] += dNdy[i] * u[n * 2 + 1];
+			eps[2] += (dNdy[i] * u[n * 2] + dNdx[i] * u[n * 2 + 1]);
+		}
+		double clfd = sqrt((x[1] - x[0]) * (y[2] - y[0]) -
+				   (y[1] - y[0]) * (x[2] - x[0]));
+		double s0 = D[0] * eps[0] + D[1] * eps[1];
+		double s1 = D[1] * eps[0] + D[2] * eps[1];
+		double s2 = D[3] * eps[2];
+		double avg = 0.5 * (s0 + s1);
+		double R = sqrt(0.25 * POW2(s0 - s1) + POW2(s2));
+		double p1 = (0 > avg + R) ? 0 : avg + R;
+		double p2 = (0 > avg - R) ? 0 : avg - R;
+		double det = D[0] * D[2] - D[1] * D[1];
+		double i11 = D[2] / det, i12 = -D[1] / det, i22 = D[0] / det;
+		double sTs = i11 * POW2(p1) + 2 * i12 * p1 * p2 + i22 * POW2(p2);
+		double tau = sqrt(sTs);
+		r[e] = (r_prev[e] > tau) ? r_prev[e] : tau;
+		double div = (Gf / clfd) * (E / POW2(ft));
+		double A = 1.0 / (div - 0.5);
+		damage[e] = 1.0 - (r0 / r[e]) * exp(A * (1.0 - (r[e] / r0)));
 	}
 	return 0;
 }

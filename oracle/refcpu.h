@@ -25,6 +25,10 @@
  *                          reference's plain CG, dense numpy solve
  *   orc_spmv               golden y, dense product
  *   orc_strain/orc_stress  golden strain/stress, patch test (linear field)
+ *   orc_assemble_t3_damage, orc_residual, orc_damage_update
+ *                          the compiled reference's damage-solver log
+ *                          (residual of every iteration of every load step)
+ *                          reproduced character for character
  *   orc_elem_energy, orc_simp_*, orc_filter_*, orc_oc_update
  *       no reference code exists (nbots has no topology optimisation);
  *       pinned by energy identity sum_e E_e u_e'K0u_e = F'u, central finite
@@ -62,6 +66,20 @@ int orc_assemble_t3(uint32_t N_nod, const double *nod,
 		    int self_weight, double density, const double *gravity,
 		    const uint32_t *rowptr, const uint32_t *col, double *val,
 		    double *F);
+
+int orc_assemble_t3_damage(uint32_t N_nod, const double *nod,
+			   uint32_t N_elem, const uint32_t *conn,
+			   double E, double nu, const double *damage,
+			   double thickness, int self_weight, double density,
+			   const double *gravity, const uint32_t *rowptr,
+			   const uint32_t *col, double *val, double *F);
+double orc_residual(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		    const double *val, const double *F, const double *u,
+		    double *res);
+int orc_damage_update(uint32_t N_elem, const double *nod, const uint32_t *conn,
+		      const double *u, double E, double nu, double Gf, double ft,
+		      const double *r_prev, double *r, double *damage,
+		      double *strain);
 
 /* ---- sparse operations ------------------------------------------------- */
 double orc_sparse_get(const uint32_t *rowptr, const uint32_t *col,

@@ -212,10 +212,31 @@ def file_cases():
           os.path.getsize(os.path.join(OUT, "cant_4x2_K.txt")), "B text")
 
 
+DAMAGE_CASE = dict(Gf=1e-3, ft=0.01, n_steps=5, max_iter=25, max_iter_wo_enhance=5,
+                   res_tol=1e-6)
+
+
+def damage_cases():
+    """The log of the reference's quasi-static damage solver
+    (nb_fem_compute_2D_Non_Linear_Solid_Mechanics, conjugate-gradient path)
+    on the 24x12 jittered beam with a prescribed right-edge deflection."""
+    p = synth.damage_beam(24, 12)
+    log = os.path.join(OUT, "damage_24x12_log.txt")
+    if os.path.exists(log):
+        os.remove(log)
+    a = DAMAGE_CASE
+    oracle.ref_damage_compute(p, a["Gf"], a["ft"], a["n_steps"], a["max_iter"],
+                              a["max_iter_wo_enhance"], a["res_tol"], log)
+    print("damage log:", sum(1 for _ in open(log)), "lines")
+
+
 def main(argv):
     oracle.build(ref=True)
     if argv[1:] == ["cg"]:
         cg_cases()
+        return
+    if argv[1:] == ["damage"]:
+        damage_cases()
         return
     if argv[1:] == ["files"]:
         file_cases()
@@ -233,6 +254,7 @@ def main(argv):
     degenerate_cases()
     cg_cases()
     file_cases()
+    damage_cases()
 
 
 if __name__ == "__main__":

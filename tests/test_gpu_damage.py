@@ -1,0 +1,61 @@
+"""GPU parity of the quasi-static damage solver (static_damage2D.c:239-472)
+on the 24x12 jittered beam with a prescribed right-edge deflection.
+
+Bars:
+  * the drop-in nb_fem_compute_2D_Non_Linear_Solid_Mechanics writes the
+    compiled reference's log line for line: the same load steps, the same
+    iteration count in each, the same stopping line; every residual within
+    1e-5 relative (1e-8 absolute below 1e-7, where the values are the
+    inner CG's 1e-8 tolerance);
+  * the final displacement and damage of nbx_fem_damage_solve against the
+    oracle's run of the same loop: displacement rel-L2 <= 1e-8, damage
+    within 1e-7.  (The device's Jacobi-PCG sums its dot products in a
+    different order, so its iterates are not bit-identical.)
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+CASE = dict(Gf=1e-3, ft=0.01, n_steps=5, max_iter=25, max_iter_wo_enhance=5, res_tol=1e-6)
+NUM = re.compile(r"-?\d+\.\d+e[+-]\d+")
+
+
+def _split(text):
+    lines = text.splitlines()
+    return [NUM.sub("#", ln) for ln in lines], [float(m) for ln in lines for m in NUM.findall(ln)]
+
+
+def test_damage_drop_in_log(tmp_path):
+    from nbots_amd import synth
+    from nbots_amd.fem import compute_2D_damage
+    log = str(tmp_path / "damage.log")
+    compute_2D_damage(synth.damage_beam(24, 12), logfile=log, drop_in=True, **CASE)
+    with open(log) as f:
+        got = f.read()
+    with open(os.path.join(GOLDEN, "damage_24x12_log.txt")) as f:
+        want = f.read()
+    gs, gv = _split(got)
+    ws, wv = _split(want)
+    assert gs == ws, "\n".join(got.splitlines()[:40])
+    gv, wv = np.array(gv), np.array(wv)
+    big = np.abs(wv) > 1e-7
+    assert np.all(np.abs(gv[big] - wv[big]) <= 1e-5 * np.abs(wv[big])), np.c_[gv, wv][big]
+    assert np.all(np.abs(gv[~big] - wv[~big]) <= 1e-8), np.c_[gv, wv][~big]
+
+
+def test_damage_fields_match_oracle():
+    import oracle
+    from nbots_amd import synth
+    from nbots_amd.fem import compute_2D_damage
+    p = synth.damage_beam(24, 12)
+    u, d = compute_2D_damage(p, **CASE)
+    ref = oracle.damage_solve(p, **CASE)
+    assert np.linalg.norm(u - ref["u"]) <= 1e-8 * np.linalg.norm(ref["u"])
+    assert np.abs(d - ref["damage"]).max() <= 1e-7
+    assert d.max() > 0.5

@@ -150,3 +150,19 @@ def test_plain_cg_dense():
     col = np.tile(np.arange(30, dtype=np.uint32), 30)
     x, it, tr, st = oracle.cg(rp, col, A.ravel().copy(), b, tol=1e-13, max_iter=500)
     assert st == 0 and np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+
+
+DAMAGE_CASE = dict(Gf=1e-3, ft=0.01, n_steps=5, max_iter=25, max_iter_wo_enhance=5, res_tol=1e-6)
+
+
+def test_damage_solver_log_matches_reference():
+    """The restated quasi-static damage loop (static_damage2D.c:239-472)
+    writes the compiled reference's log character for character: every
+    residual of every iteration of the 5 load steps (%e, 7 digits)."""
+    import os
+    from conftest import GOLDEN
+    from nbots_amd import synth
+    r = oracle.damage_solve(synth.damage_beam(24, 12), **DAMAGE_CASE)
+    with open(os.path.join(GOLDEN, "damage_24x12_log.txt")) as f:
+        assert r["log"] == f.read()
+    assert 0.5 < r["damage"].max() < 1.0 and (r["damage"] > 0).sum() >= 5
