@@ -600,8 +600,28 @@ int fem_create(FemSys& F, uint32_t nn, const double* nod, uint32_t ne, const uin
                            nn, S.rowlen, F.err);
     NBX_TRY(read_err(F.err, s, &herr));
     if (herr) {
-        set_last_error("a node has more than 254 neighbours");
+        set_last_error("a node has more than %d neighbours (the device pattern's capacity)",
+                       MAX_ADJ - 1);
         return NBX_ERR_PATTERN;
+    }
+    // row lengths on the host: 64-bit total of the stored blocks, which the
+    // 32-bit slice offsets must hold
+    std::vector<uint8_t> hl(nn);
+    if (nn) NBX_HIP(hipMemcpyAsync(hl.data(), S.rowlen, nn, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    {
+        uint64_t stored = 0;
+        for (uint32_t sl = 0; sl < S.n_slices; ++sl) {
+            uint32_t wdt = 0;
+            for (uint32_t v = sl * SLICE; v < nn && v < (sl + 1) * SLICE; ++v)
+                wdt = hl[v] > wdt ? hl[v] : wdt;
+            stored += (uint64_t)wdt * SLICE;
+        }
+        if (stored > 0xffffffffull) {
+            set_last_error("pattern of %llu stored 2x2 blocks exceeds 32-bit slice offsets",
+                           (unsigned long long)stored);
+            return NBX_ERR_PATTERN;
+        }
     }
     uint32_t* sblk = nullptr;
     NBX_HIP(hipMalloc(&sblk, (size_t)(S.n_slices + 1) * sizeof(uint32_t)));
@@ -621,10 +641,7 @@ int fem_create(FemSys& F, uint32_t nn, const double* nod, uint32_t ne, const uin
         hipLaunchKernelGGL(k_fill_cols, dim3(grid_for(nn)), dim3(BLOCK), 0, s, F.n2e_ptr, F.n2e, F.conn,
                            nn, S.slice_ptr, S.bcol);
     NBX_HIP(hipGetLastError());
-    // pattern statistics (host copy of the row lengths is small)
-    std::vector<uint8_t> hl(nn);
-    if (nn) NBX_HIP(hipMemcpyAsync(hl.data(), S.rowlen, nn, hipMemcpyDeviceToHost, s));
-    NBX_HIP(hipStreamSynchronize(s));
+    // pattern statistics
     size_t valid = 0;
     uint32_t mx = 0;
     for (uint8_t l : hl) {

@@ -256,7 +256,7 @@ int nbx_fem_create(nbx_fem_t** out, uint32_t N_nod, const double* nod, uint32_t 
         nbx::set_last_error("empty mesh");
         return nbx::NBX_ERR_ARG;
     }
-    if ((uint64_t)N_nod * 2 >= 0xffffffffull) {
+    if ((uint64_t)N_nod * 2 >= 0xffffffffull || (uint64_t)N_elem * 3 >= 0xffffffffull) {
         nbx::set_last_error("mesh too large for 32-bit indices");
         return nbx::NBX_ERR_ARG;
     }

@@ -301,6 +301,10 @@ int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rp, const uint32_t*
             uint32_t wdt = 0;
             for (uint32_t v = sl * SLICE; v < nn && v < (sl + 1) * SLICE; ++v)
                 if (len[v] > wdt) wdt = len[v];
+            if ((uint64_t)sptr[sl] + (uint64_t)wdt * SLICE > 0xffffffffull) {
+                set_last_error("pattern exceeds 32-bit slice offsets");
+                return NBX_ERR_PATTERN;
+            }
             sptr[sl + 1] = sptr[sl] + wdt * SLICE;
         }
         const size_t nb = sptr[ns];
