@@ -254,6 +254,8 @@ def main():
             log(f"[bench] mixed ({args.mixed}) {s}: {info['iters']} it, {info['outer']} inner, "
                 f"{mtimes[-1]:.3f}s, rel_res {info['rel_res']:.2e}, fallback {info['fallback']}")
         u32 = f.displacement()
+        Fl = f.load_vector()
+        c64, c32 = float(Fl @ u64), float(Fl @ u32)
         t_mixed = float(np.median(mtimes))
         t_fp64 = float(np.median(times))
         ms32 = f.spmv_ms(32, 20)
@@ -274,6 +276,7 @@ def main():
             mixed_iters=minfos[-1]["iters"], mixed_inner=minfos[-1]["outer"],
             mixed_rel_res=minfos[-1]["rel_res"], mixed_fallback=minfos[-1]["fallback"],
             mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+            mixed_vs_fp64_compliance_rel=abs(c32 - c64) / abs(c64),
             spmv64_us=round(ms64 * 1e3, 2),
             spmv64_gbs=round(b64 / (ms64 * 1e-3) / 1e9, 1),
             spmv32_us=round(ms32 * 1e3, 2),

@@ -126,6 +126,12 @@ class Fem:
         check(self.L.nbx_fem_get_system(self.h, rp, col, val, F), "nbx_fem_get_system")
         return rp, col, val, F
 
+    def load_vector(self):
+        """F (after the boundary conditions) without downloading K."""
+        F = np.zeros(self.N)
+        check(self.L.nbx_fem_get_system(self.h, None, None, None, F), "nbx_fem_get_system")
+        return F
+
     def strain_stress(self):
         e = np.zeros(3 * self.N_elem)
         s = np.zeros(3 * self.N_elem)

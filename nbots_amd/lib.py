@@ -176,7 +176,7 @@ SIGS = {
     "nbx_fem_solve_ex": (I32, [VP, I32, F64, U32, C.POINTER(SolveOpts), f64o,
                                C.POINTER(SolveInfo)]),
     "nbx_fem_get_displacement": (I32, [VP, f64p]),
-    "nbx_fem_get_system": (I32, [VP, u32p, u32p, f64p, f64p]),
+    "nbx_fem_get_system": (I32, [VP, u32o, u32o, f64o, f64o]),
     "nbx_fem_strain_stress": (I32, [VP, f64p, f64p]),
     "nbx_fem_compliance": (I32, [VP, f64o, F64, F64, F64, C.POINTER(F64), f64o, f64o]),
     "nbx_fem_adjoint": (I32, [VP, f64p, f64p, I32, F64, U32, C.POINTER(SolveInfo)]),

@@ -207,3 +207,29 @@ def test_spmv_bench_entry():
     f = Fem.from_problem(p)
     assert f.spmv_ms(64, 5) > 0 and f.spmv_ms(32, 5) > 0
     f.destroy()
+
+
+def test_c2_fp64_against_reference_solve():
+    """1024x512 lognormal cantilever, right-edge shear traction (1,051,650
+    DOF): the device's K and F equal the compiled reference's (SHA-256), and
+    its FP64 Jacobi-PCG with the reference's stop test (tol 1e-8 ||F||)
+    takes the reference's iteration count within 0.5 % (11,733; the dot
+    products are summed in another order) and reaches the reference's
+    displacement within 1e-7 (every 101st dof and ||u||)."""
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_REFERENCE
+    g = load_golden("c2_reference_solve")
+    p = synth.cantilever_traction(1024, 512)
+    f = Fem.from_problem(p)
+    rp, col, val, F = f.system()
+    assert sha(val) == str(g["K_sha"]) and sha(F) == str(g["F_sha"])
+    del rp, col, val
+    info = f.solve(SOLVE_REFERENCE, tol=float(g["tol"]), max_iter=p.N)
+    assert info["status"] == int(g["status"]) == 0
+    n_ref = int(g["niter"])
+    assert abs(info["iters"] - n_ref) <= 0.005 * n_ref, (info["iters"], n_ref)
+    u = f.displacement()
+    us = u[g["idx"]]
+    assert np.linalg.norm(us - g["u"]) <= 1e-7 * np.linalg.norm(g["u"])
+    assert abs(np.linalg.norm(u) - float(g["u_norm"])) <= 1e-7 * float(g["u_norm"])
+    f.destroy()
