@@ -50,6 +50,13 @@ constexpr int SLICE = 64;         // sliced-ELL slice height = one wavefront
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, 32 CUs each
 constexpr int MAX_GRID = 65535;   // workgroups of a reduction kernel (partials)
 
+// A copy ordered on a handle's stream, complete on return (a plain hipMemcpy
+// would also wait for every other handle's work).
+inline hipError_t copy_on(void* dst, const void* src, size_t n, hipMemcpyKind kind, hipStream_t s) {
+    const hipError_t e = hipMemcpyAsync(dst, src, n, kind, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 inline unsigned grid_for(size_t n, int per_block = BLOCK) {
     size_t g = (n + per_block - 1) / per_block;
     return (unsigned)(g == 0 ? 1 : g);

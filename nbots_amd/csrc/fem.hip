@@ -974,10 +974,10 @@ int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* v
     std::vector<uint32_t> sp(S.n_slices + 1), bc(S.n_blocks + 1);
     std::vector<uint8_t> len(nn + 1);
     std::vector<double> hv(4 * S.n_blocks + 4);
-    NBX_HIP(hipMemcpy(sp.data(), S.slice_ptr, (S.n_slices + 1) * 4, hipMemcpyDeviceToHost));
-    if (nn) NBX_HIP(hipMemcpy(len.data(), S.rowlen, nn, hipMemcpyDeviceToHost));
-    NBX_HIP(hipMemcpy(bc.data(), S.bcol, (S.n_blocks + 1) * 4, hipMemcpyDeviceToHost));
-    NBX_HIP(hipMemcpy(hv.data(), S.val, (4 * S.n_blocks + 4) * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(copy_on(sp.data(), S.slice_ptr, (S.n_slices + 1) * 4, hipMemcpyDeviceToHost, F.s));
+    if (nn) NBX_HIP(copy_on(len.data(), S.rowlen, nn, hipMemcpyDeviceToHost, F.s));
+    NBX_HIP(copy_on(bc.data(), S.bcol, (S.n_blocks + 1) * 4, hipMemcpyDeviceToHost, F.s));
+    NBX_HIP(copy_on(hv.data(), S.val, (4 * S.n_blocks + 4) * 8, hipMemcpyDeviceToHost, F.s));
     rowptr[0] = 0;
     for (uint32_t v = 0; v < nn; ++v) {
         rowptr[2 * v + 1] = rowptr[2 * v] + 2 * len[v];

@@ -303,7 +303,7 @@ int nbx_fem_set_bconditions(nbx_fem_t* f, const uint32_t* vtx, const uint32_t* s
                             const double* ns_fval, double factor) {
     nbx::FemSys& S = f->sys;
     std::vector<double> nod(2 * (size_t)S.nn);
-    NBX_HIP(hipMemcpy(nod.data(), S.nod, nod.size() * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(nbx::copy_on(nod.data(), S.nod, nod.size() * 8, hipMemcpyDeviceToHost, S.s));
     std::vector<const double*> samples;
     if (ns_fptr) {
         samples.resize(n_ns, nullptr);
@@ -358,13 +358,13 @@ int nbx_fem_solve_ex(nbx_fem_t* f, int mode, double tol, uint32_t max_iter,
 }
 
 int nbx_fem_get_displacement(const nbx_fem_t* f, double* u) {
-    NBX_HIP(hipMemcpy(u, f->sys.u, 2 * (size_t)f->sys.nn * 8, hipMemcpyDeviceToHost));
+    NBX_HIP(nbx::copy_on(u, f->sys.u, 2 * (size_t)f->sys.nn * 8, hipMemcpyDeviceToHost, f->sys.s));
     return nbx::NBX_OK;
 }
 
 int nbx_fem_get_system(const nbx_fem_t* f, uint32_t* rowptr, uint32_t* col, double* val, double* F) {
     if (rowptr && col && val) NBX_TRY(nbx::fem_download_csr(f->sys, rowptr, col, val));
-    if (F) NBX_HIP(hipMemcpy(F, f->sys.F, 2 * (size_t)f->sys.nn * 8, hipMemcpyDeviceToHost));
+    if (F) NBX_HIP(nbx::copy_on(F, f->sys.F, 2 * (size_t)f->sys.nn * 8, hipMemcpyDeviceToHost, f->sys.s));
     return nbx::NBX_OK;
 }
 
@@ -375,9 +375,9 @@ int nbx_fem_strain_stress(const nbx_fem_t* f, double* strain, double* stress) {
     NBX_HIP(hipMalloc(&ds, bytes));
     NBX_HIP(hipMalloc(&dt, bytes));
     int rc = nbx::fem_strain_stress(S, ds, dt);
-    if (!rc && strain && hipMemcpy(strain, ds, bytes - 8, hipMemcpyDeviceToHost) != hipSuccess)
+    if (!rc && strain && nbx::copy_on(strain, ds, bytes - 8, hipMemcpyDeviceToHost, S.s) != hipSuccess)
         rc = nbx::NBX_ERR_HIP;
-    if (!rc && stress && hipMemcpy(stress, dt, bytes - 8, hipMemcpyDeviceToHost) != hipSuccess)
+    if (!rc && stress && nbx::copy_on(stress, dt, bytes - 8, hipMemcpyDeviceToHost, S.s) != hipSuccess)
         rc = nbx::NBX_ERR_HIP;
     (void)hipFree(ds);
     (void)hipFree(dt);
@@ -397,8 +397,8 @@ int nbx_fem_compliance(nbx_fem_t* f, const double* rho, double penal, double E0,
     double C = 0.0;
     NBX_TRY(nbx::fem_compliance(S, rho ? S.rho : nullptr, penal, E0, Emin, S.ce, S.dc, &C));
     if (compliance) *compliance = C;
-    if (dc) NBX_HIP(hipMemcpy(dc, S.dc, (size_t)S.ne * 8, hipMemcpyDeviceToHost));
-    if (ce) NBX_HIP(hipMemcpy(ce, S.ce, (size_t)S.ne * 8, hipMemcpyDeviceToHost));
+    if (dc) NBX_HIP(nbx::copy_on(dc, S.dc, (size_t)S.ne * 8, hipMemcpyDeviceToHost, S.s));
+    if (ce) NBX_HIP(nbx::copy_on(ce, S.ce, (size_t)S.ne * 8, hipMemcpyDeviceToHost, S.s));
     return nbx::NBX_OK;
 }
 
@@ -420,7 +420,7 @@ int nbx_fem_adjoint(nbx_fem_t* f, const double* rhs, double* lambda, int mode, d
     std::swap(S.F, S.aux);
     std::swap(S.u, S.aux2);
     if (rc > 1) return rc;
-    NBX_HIP(hipMemcpy(lambda, S.aux2, bytes, hipMemcpyDeviceToHost));
+    NBX_HIP(nbx::copy_on(lambda, S.aux2, bytes, hipMemcpyDeviceToHost, S.s));
     return rc;
 }
 
@@ -512,9 +512,9 @@ int nbx_simp_iterate(nbx_simp_t* h, int mode, double tol, uint32_t max_iter,
 int nbx_simp_get(const nbx_simp_t* h, double* x, double* xphys, double* dc, uint32_t* oc_steps) {
     const nbx::FemSys& S = h->fem->sys;
     const size_t bytes = (size_t)S.ne * 8;
-    if (x) NBX_HIP(hipMemcpy(x, h->o.x, bytes, hipMemcpyDeviceToHost));
-    if (xphys) NBX_HIP(hipMemcpy(xphys, h->o.xphys, bytes, hipMemcpyDeviceToHost));
-    if (dc) NBX_HIP(hipMemcpy(dc, h->o.dcf, bytes, hipMemcpyDeviceToHost));
+    if (x) NBX_HIP(nbx::copy_on(x, h->o.x, bytes, hipMemcpyDeviceToHost, h->fem->sys.s));
+    if (xphys) NBX_HIP(nbx::copy_on(xphys, h->o.xphys, bytes, hipMemcpyDeviceToHost, h->fem->sys.s));
+    if (dc) NBX_HIP(nbx::copy_on(dc, h->o.dcf, bytes, hipMemcpyDeviceToHost, h->fem->sys.s));
     if (oc_steps) *oc_steps = h->o.last_oc_steps;
     return nbx::NBX_OK;
 }
@@ -813,8 +813,9 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
         }
         {
             std::vector<double> r0(ne, ft / sqrt(E));   // :275-276
-            if (hipMemcpy(r, r0.data(), (size_t)ne * 8, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemset(dmg, 0, eb) != hipSuccess || hipMemset(S.u, 0, vb - 8) != hipSuccess) {
+            if (nbx::copy_on(r, r0.data(), (size_t)ne * 8, hipMemcpyHostToDevice, S.s) != hipSuccess ||
+                hipMemsetAsync(dmg, 0, eb, S.s) != hipSuccess ||
+                hipMemsetAsync(S.u, 0, vb - 8, S.s) != hipSuccess) {
                 rc = nbx::NBX_ERR_HIP;
                 goto END;
             }
@@ -876,9 +877,9 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
             }
         }
         if (displacement &&
-            hipMemcpy(displacement, S.u, (size_t)N * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            nbx::copy_on(displacement, S.u, (size_t)N * 8, hipMemcpyDeviceToHost, S.s) != hipSuccess)
             rc = nbx::NBX_ERR_HIP;
-        if (!rc && damage && hipMemcpy(damage, dmg, (size_t)ne * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        if (!rc && damage && nbx::copy_on(damage, dmg, (size_t)ne * 8, hipMemcpyDeviceToHost, S.s) != hipSuccess)
             rc = nbx::NBX_ERR_HIP;
     }
 END:

@@ -13,7 +13,7 @@
 
 namespace nbx {
 
-static char g_err[512];
+static thread_local char g_err[512];   // per thread: handles are used from several threads
 void set_last_error(const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);

@@ -254,8 +254,8 @@ int simp_create(Simp& o, FemSys& S, double rmin, double volfrac, double penal, d
     // centroids and areas from the system's mesh
     std::vector<double> nod(2 * (size_t)S.nn);
     std::vector<uint32_t> conn(3 * (size_t)ne);
-    NBX_HIP(hipMemcpy(nod.data(), S.nod, nod.size() * 8, hipMemcpyDeviceToHost));
-    NBX_HIP(hipMemcpy(conn.data(), S.conn, conn.size() * 4, hipMemcpyDeviceToHost));
+    NBX_HIP(copy_on(nod.data(), S.nod, nod.size() * 8, hipMemcpyDeviceToHost, s));
+    NBX_HIP(copy_on(conn.data(), S.conn, conn.size() * 4, hipMemcpyDeviceToHost, s));
     std::vector<double> cx(ne), cy(ne), area(ne);
     for (uint32_t e = 0; e < ne; ++e) {
         const uint32_t a = conn[3 * (size_t)e], b = conn[3 * (size_t)e + 1], c = conn[3 * (size_t)e + 2];
