@@ -229,6 +229,13 @@ class _Params2D(C.Structure):
                 ("revolution_coordinate", C.c_char)]
 
 
+_BC_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double))
+
+
+def _bc_eval(fn, x, out):
+    out[0], out[1] = fn(x[0], x[1])
+
+
 class _Objects:
     """The reference's object API for a Problem: mesh, element type,
     material and boundary conditions (freed on exit)."""
@@ -254,6 +261,12 @@ class _Objects:
                 mask = (C.c_bool * 2)(bool(m[2 * c]), bool(m[2 * c + 1]))
                 val = (C.c_double * 2)(v[2 * c], v[2 * c + 1])
                 L.nb_bcond_push(self.bc, t, w, int(ids[c]), mask, val)
+        self._fns = []              # the callbacks live as long as the bcond
+        for sid, m, fn in p.ns_fn:
+            cb = _BC_FN(lambda x, t, out, fn=fn: _bc_eval(fn, x, out))
+            self._fns.append(cb)
+            mask = (C.c_bool * 2)(bool(m[0]), bool(m[1]))
+            L.nb_bcond_push_function(self.bc, 1, 1, int(sid), mask, C.cast(cb, C.c_void_p))
         self.params = _Params2D(p.thickness, 0.0, b"\0")
 
     def __enter__(self):

@@ -76,6 +76,10 @@ class Problem:
     nv: tuple = field(default_factory=lambda: _empty_bc())
     ds: tuple = field(default_factory=lambda: _empty_bc())
     ns: tuple = field(default_factory=lambda: _empty_bc())
+    # function-valued Neumann segments, applied after ns: (segment id,
+    # (mask_x, mask_y), f) with f(x, y) -> (value_x, value_y), a traction
+    # (nb_bcond_push_function with NB_NEUMANN, NB_BC_ON_SEGMENT)
+    ns_fn: tuple = ()
     name: str = ""
 
     @property

@@ -12,6 +12,7 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import os
 import subprocess
@@ -151,6 +152,8 @@ def ref():
         R.ref_spmv.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32]
         R.ref_testcase_run.restype = I32
         R.ref_testcase_run.argtypes = [C.c_char_p, U32, u32p]
+        R.ref_testcase_run_ex.restype = I32
+        R.ref_testcase_run_ex.argtypes = [C.c_char_p, U32, I32, u32p]
         R.ref_testcase_fetch.argtypes = [f64p, u32p, u32p, u32p, u32p, f64p, f64p, f64p, f64p]
         R.ref_testcase_free.argtypes = []
         _REF = R
@@ -325,6 +328,29 @@ def stress(p, strain_, enabled=None):
     return s
 
 
+def fn_neumann(p):
+    """p with its function-valued Neumann segments (p.ns_fn) appended to the
+    constant ones, and the (ns_fptr, ns_fval) sample arrays of
+    orc_set_bconditions: each function evaluated at its segment's nodes in
+    segment order, at time 0 (set_bconditions.c:87-131)."""
+    ns_fn = getattr(p, "ns_fn", ())
+    if not ns_fn:
+        return p, None, None
+    ids, m, v = p.ns
+    fptr = [-1] * ids.size
+    fval = []
+    for sid, mask, fn in ns_fn:
+        fptr.append(len(fval) // 2)
+        for n in p.sgm_nodes[p.sgm_ptr[sid]:p.sgm_ptr[sid + 1]]:
+            fval += list(fn(float(p.nod[2 * n]), float(p.nod[2 * n + 1])))
+    q = copy.copy(p)
+    q.ns = (np.concatenate([ids, np.array([s for s, _, _ in ns_fn], np.uint32)]),
+            np.concatenate([m, np.array([b for _, mk, _ in ns_fn for b in mk], np.uint8)]),
+            np.concatenate([v, np.zeros(2 * len(ns_fn))]))
+    q.ns_fn = ()
+    return q, np.array(fptr, np.int32), np.array(fval, np.float64)
+
+
 def fem_solve(p, tol=1e-8, max_iter=None, enabled=None):
     """The reference driver flow (static_elasticity2D.c:31-97) on the
     restatement: pattern, assembly, BCs, x=0, PCG(max_iter=N, tol), strain."""
@@ -332,7 +358,8 @@ def fem_solve(p, tol=1e-8, max_iter=None, enabled=None):
     val, F, st = assemble(p, rowptr, col, enabled=enabled)
     if st:
         return None
-    set_bconditions(p, rowptr, col, val, F)
+    q, fptr, fval = fn_neumann(p)
+    set_bconditions(q, rowptr, col, val, F, ns_fptr=fptr, ns_fval=fval)
     u, it, tr, cst = pcg(rowptr, col, val, F, max_iter=max_iter, tol=tol)
     eps = strain(p, u)
     return dict(rowptr=rowptr, col=col, val=val, F=F, u=u, niter=it,

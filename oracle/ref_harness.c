@@ -510,7 +510,62 @@ static int read_model(nb_cfreader_t *cfr, nb_model_t *model)
 	return 0;
 }
 
+/* Kirsch's solution for an infinite plate with a circular hole of radius
+ * a = 0.5 under remote tension tx = 1e4 along x (polar r, theta about the
+ * hole centre): the closed form the reference's plate-with-a-hole test
+ * compares against (utest/.../static_elasticity2D.c:191-206). */
+static void kirsch_stress(double x, double y, double s[3])
+{
+	const double a = 0.5, tx = 1e4;
+	const double r = sqrt(x * x + y * y);
+	const double th = atan2(y, x);
+	const double q2 = (a / r) * (a / r);
+	const double q4 = 1.5 * (q2 * q2);
+	const double c2 = cos(2.0 * th), c4 = cos(4.0 * th);
+	const double s2 = sin(2.0 * th), s4 = sin(4.0 * th);
+	s[0] = tx * (1.0 - q2 * (1.5 * c2 + c4) + q4 * c4);
+	s[1] = tx * (-q2 * (0.5 * c2 - c4) - q4 * c4);
+	s[2] = tx * (-q2 * (0.5 * s2 + s4) + q4 * s4);
+}
+
+/* tractions on the two loaded edges of the test (x-normal, y-normal) */
+static void kirsch_xface(const double *x, double t, double *out)
+{
+	(void)t;
+	double s[3];
+	kirsch_stress(x[0], x[1], s);
+	out[0] = s[0];
+	out[1] = s[2];
+}
+
+static void kirsch_yface(const double *x, double t, double *out)
+{
+	(void)t;
+	double s[3];
+	kirsch_stress(x[0], x[1], s);
+	out[0] = s[2];
+	out[1] = s[1];
+}
+
+static int testcase_run(const char *path, uint32_t N_vtx, int kirsch,
+			uint32_t *sizes);
+
 int ref_testcase_run(const char *path, uint32_t N_vtx, uint32_t *sizes)
+{
+	return testcase_run(path, N_vtx, 0, sizes);
+}
+
+/* kirsch != 0: the plate-with-a-hole test's Neumann tractions on segments
+ * 11 (x-normal edge) and 12 (y-normal edge), pushed after the file's
+ * conditions as the test does (static_elasticity2D.c:208-219, :285-286). */
+int ref_testcase_run_ex(const char *path, uint32_t N_vtx, int kirsch,
+			uint32_t *sizes)
+{
+	return testcase_run(path, N_vtx, kirsch, sizes);
+}
+
+static int testcase_run(const char *path, uint32_t N_vtx, int kirsch,
+			uint32_t *sizes)
 {
 	nb_cfreader_t *cfr = nb_cfreader_create();
 	nb_cfreader_add_line_comment_token(cfr, "#");
@@ -540,6 +595,13 @@ int ref_testcase_run(const char *path, uint32_t N_vtx, uint32_t *sizes)
 	nb_material_set_elasticity_module(mat, E);
 	tcase.E = E;
 	tcase.nu = nu;
+	if (kirsch) {
+		bool mask[2] = {true, true};
+		nb_bcond_push_function(bc, NB_NEUMANN, NB_BC_ON_SEGMENT, 11,
+				       mask, kirsch_xface);
+		nb_bcond_push_function(bc, NB_NEUMANN, NB_BC_ON_SEGMENT, 12,
+				       mask, kirsch_yface);
+	}
 
 	tcase.part = malloc(nb_mesh2D_get_memsize(NB_TRIAN));
 	nb_mesh2D_init(tcase.part, NB_TRIAN);

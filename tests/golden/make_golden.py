@@ -80,13 +80,15 @@ def tokens(path):
     return toks
 
 
-def beam_case(fname, name, N_vtx=1000):
-    """The reference's own FEM unit-test problem (disabled suite), meshed by
-    the reference tessellator; the mesh, BCs and results become a fixture
-    that the oracle and the GPU path replay."""
+def beam_case(fname, name, N_vtx=1000, kirsch=False):
+    """A problem of the reference's own FEM unit tests (disabled suite),
+    meshed by the reference tessellator; the mesh, BCs and results become a
+    fixture that the oracle and the GPU path replay.  kirsch: the
+    plate-with-a-hole test's Kirsch tractions on segments 11 and 12, added
+    after the file's conditions."""
     R = oracle.ref()
     sizes = np.zeros(5, np.uint32)
-    st = R.ref_testcase_run(os.path.join(REF_TESTS, fname).encode(), N_vtx, sizes)
+    st = R.ref_testcase_run_ex(os.path.join(REF_TESTS, fname).encode(), N_vtx, int(kirsch), sizes)
     assert st == 0, st
     nn, ne, nv, ns, tot = (int(s) for s in sizes)
     nod = np.zeros(2 * nn)
@@ -132,6 +134,8 @@ def beam_case(fname, name, N_vtx=1000):
                max_disp=float(np.sqrt((disp.reshape(-1, 2) ** 2).sum(1).max())))
     for k, (ids, m, v) in zip(("dv", "nv", "ds", "ns"), (dv, nvl, ds, nsl)):
         out[k + "_id"], out[k + "_mask"], out[k + "_val"] = ids, m, v
+    if kirsch:
+        out["kirsch_sgm"] = np.array([11, 12], np.uint32)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
     print(f"{name}: nodes={nn} elems={ne} max|u|={out['max_disp']:.9e}")
 
@@ -264,6 +268,9 @@ def main(argv):
     if argv[1:] == ["damage"]:
         damage_cases()
         return
+    if argv[1:] == ["plate"]:
+        beam_case("plate_with_hole.txt", "plate_with_hole", kirsch=True)
+        return
     if argv[1:] == ["files"]:
         file_cases()
         return
@@ -277,6 +284,7 @@ def main(argv):
     full_case(synth.cantilever_traction(128, 64), "traction_128x64", store_full=False,
               keep_x=False)
     beam_case("beam_cantilever.txt", "beam_cantilever")
+    beam_case("plate_with_hole.txt", "plate_with_hole", kirsch=True)
     degenerate_cases()
     cg_cases()
     file_cases()
