@@ -57,11 +57,11 @@ def log(msg):
 
 def pcg_iter_bytes(n_nodes, n_blocks, prec):
     """Algorithmic bytes of one CG iteration as implemented: SpMV kernel
-    (above) + update kernel (FP64: read x,p,g,w,diag, write x,g = 7 vectors;
+    (above) + update kernel (FP64: read g,w,diag, write g = 4 vectors;
     FP32 scaled: read y,p,r,w, write y,r = 6) + direction kernel (FP64: read
-    g,diag,p, write p = 4; FP32: read r,p, write p = 3)."""
+    x,p,g,diag, write x,p = 6; FP32: read r,p, write p = 3)."""
     sv = 8 if prec == 64 else 4
-    nv = 11 if prec == 64 else 9
+    nv = 10 if prec == 64 else 9
     return spmv_bytes(n_nodes, n_blocks, prec) + nv * (2 * n_nodes) * sv
 
 

@@ -209,16 +209,31 @@ __device__ __forceinline__ void put_partials(const f32c (&v)[NV], float* partial
 }
 
 constexpr int FIN = 1024;   // threads of a finalize kernel
+constexpr int FIN_LOADS = 8; // partials a finalize thread loads before it adds
 
-// Sum of np partials per value, in a fixed order; valid in thread 0.
+// Sum of np partials per value, in a fixed order; valid in thread 0.  Each
+// thread adds partials t, t + FIN, t + 2 FIN, ... in that order; the loads of
+// a group of FIN_LOADS are all issued before the first add, so one memory
+// latency covers the group.
 template <int NV>
 __device__ __forceinline__ void fin_sum(const double* partials, unsigned np, double (&tot)[NV]) {
     __shared__ double red[NV][FIN / WAVE];
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
+        const double* pi = partials + (size_t)i * np;
         double s = 0.0;
-        for (unsigned b = threadIdx.x; b < np; b += FIN) s += partials[(size_t)i * np + b];
+        for (unsigned b0 = threadIdx.x; b0 < np; b0 += FIN * FIN_LOADS) {
+            double v[FIN_LOADS];
+#pragma unroll
+            for (int j = 0; j < FIN_LOADS; ++j) {
+                const unsigned b = b0 + (unsigned)j * FIN;
+                v[j] = b < np ? pi[b] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < FIN_LOADS; ++j)
+                if (b0 + (unsigned)j * FIN < np) s += v[j];
+        }
         s = wave_sum(s);
         if (lane == 0) red[i][wid] = s;
     }
@@ -238,10 +253,19 @@ __device__ __forceinline__ void fin_sum(const float* partials, unsigned np, f32c
     const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
+        const float* ps = partials + (size_t)(2 * i) * np;
+        const float* pc = partials + (size_t)(2 * i + 1) * np;
         f32c s{0.f, 0.f};
-        for (unsigned b = threadIdx.x; b < np; b += FIN) {
-            f32c o{partials[(size_t)(2 * i) * np + b], partials[(size_t)(2 * i + 1) * np + b]};
-            s = cadd(s, o);
+        for (unsigned b0 = threadIdx.x; b0 < np; b0 += FIN * FIN_LOADS) {
+            f32c v[FIN_LOADS];
+#pragma unroll
+            for (int j = 0; j < FIN_LOADS; ++j) {
+                const unsigned b = b0 + (unsigned)j * FIN;
+                v[j] = b < np ? f32c{ps[b], pc[b]} : f32c{0.f, 0.f};
+            }
+#pragma unroll
+            for (int j = 0; j < FIN_LOADS; ++j)
+                if (b0 + (unsigned)j * FIN < np) s = cadd(s, v[j]);
         }
         s = wave_sum(s);
         if (lane == 0) red[i][wid] = s;

@@ -2,8 +2,9 @@
 //
 // * Jacobi-preconditioned CG with the reference's recurrences and stop test
 //   (solver_bot/sparse/solvers/cg_precond_jacobi.c:13-90).  Three kernels per
-//   iteration: SpMV + p.w, the x/g update + g.q and g.g, and the direction
-//   update.  Reducing kernels write one partial per workgroup; a one-
+//   iteration: SpMV + p.w, the g update + g.q and g.g, and the x and
+//   direction update (x += alpha p reads the p that the direction update
+//   reads anyway, so the vectors cross HBM 12 times per iteration, not 13).  Reducing kernels write one partial per workgroup; a one-
 //   workgroup finalize kernel sums them in index order and computes alpha,
 //   beta and the stop decision on the GPU (runs are reproducible, and no
 //   device-scope fence or atomic sits in the loop); the host only polls a
@@ -100,31 +101,6 @@ __global__ __launch_bounds__(FIN) void k_fin_A(PcgState* st, const double* parti
     }
 }
 
-// x += alpha p; g += alpha w; q = g / Aii; rz' = g.q; gg' = g.g, then the
-// lagged stop test of cg_precond_jacobi.c:45 (cg_precond_jacobi.c:62-70, :75)
-template <class V>
-__global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ x, V* __restrict__ g,
-                                                 const V* __restrict__ p,
-                                                 const V* __restrict__ w,
-                                                 const V* __restrict__ diag, PcgState* st,
-                                                 double* partials, uint32_t n) {
-    if (st->done) return;
-    const double alpha = st->alpha;
-    double acc[2] = {0.0, 0.0};
-    const Chunk c = my_chunk(n);
-    for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
-        x[u] = axpyu(x[u], alpha, p[u]);
-        const V gv = axpyu(g[u], alpha, w[u]);
-        g[u] = gv;
-        const V q = divu(gv, diag[u]);
-        acc[0] += dotu(gv, q);
-        acc[1] += dotu(gv, gv);
-    }
-    __shared__ double red[2][BLOCK / WAVE];
-    block_sum<2>(acc, red);
-    put_partials<2>(acc, partials);
-}
-
 __global__ __launch_bounds__(FIN) void k_fin_B(PcgState* st, const double* partials, unsigned np) {
     if (st->done) return;
     double tot[2];
@@ -145,18 +121,56 @@ __global__ __launch_bounds__(FIN) void k_fin_B(PcgState* st, const double* parti
     }
 }
 
-// p = -q + beta p   (cg_precond_jacobi.c:72-74)
+// g += alpha w; q = g / Aii; rz' = g.q; gg' = g.g, then the lagged stop test
+// (cg_precond_jacobi.c:63-70, :75); x is left to the direction kernel.
 template <class V>
-__global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ p, const V* __restrict__ g,
-                                                 const V* __restrict__ diag, PcgState* st,
-                                                 uint32_t n) {
+__global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ g, const V* __restrict__ w,
+                                                  const V* __restrict__ diag, PcgState* st,
+                                                  double* partials, uint32_t n) {
     if (st->done) return;
-    const double beta = st->beta;
+    const double alpha = st->alpha;
+    double acc[2] = {0.0, 0.0};
     const Chunk c = my_chunk(n);
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
-        const V q = divu(g[u], diag[u]);
-        p[u] = pupd(q, beta, p[u]);
+        const V gv = axpyu(g[u], alpha, w[u]);
+        g[u] = gv;
+        const V q = divu(gv, diag[u]);
+        acc[0] += dotu(gv, q);
+        acc[1] += dotu(gv, gv);
     }
+    __shared__ double red[2][BLOCK / WAVE];
+    block_sum<2>(acc, red);
+    put_partials<2>(acc, partials);
+}
+
+// x += alpha p; p = -q + beta p  (cg_precond_jacobi.c:62, :72-74).  Once the
+// stop test has fired this kernel no longer runs, and k_pcg_tail applies the
+// last iteration's x update.
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, V* __restrict__ p,
+                                                  const V* __restrict__ g,
+                                                  const V* __restrict__ diag, PcgState* st,
+                                                  uint32_t n) {
+    if (st->done) return;
+    const double alpha = st->alpha, beta = st->beta;
+    const Chunk c = my_chunk(n);
+    for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
+        const V pu = p[u];
+        x[u] = axpyu(x[u], alpha, pu);
+        const V q = divu(g[u], diag[u]);
+        p[u] = pupd(q, beta, pu);
+    }
+}
+
+// the last iteration's x += alpha p (cg_precond_jacobi.c:62), once the loop
+// has stopped; nothing if no iteration ran
+template <class V>
+__global__ __launch_bounds__(BLOCK) void k_pcg_tail(V* __restrict__ x, const V* __restrict__ p,
+                                                    const PcgState* st, uint32_t n) {
+    if (st->k == 0) return;
+    const double alpha = st->alpha;
+    const Chunk c = my_chunk(n);
+    for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) x[u] = axpyu(x[u], alpha, p[u]);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,10 +551,10 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
                                nu);
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
             hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            hipLaunchKernelGGL(k_pcg_B<V>, dim3(G), dim3(BLOCK), 0, s, x, g, p, wv, diag, w.st,
+            hipLaunchKernelGGL(k_pcg_B<V>, dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
                                w.partials, nu);
             hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            hipLaunchKernelGGL(k_pcg_C<V>, dim3(G), dim3(BLOCK), 0, s, p, g, diag, w.st, nu);
+            hipLaunchKernelGGL(k_pcg_C<V>, dim3(G), dim3(BLOCK), 0, s, x, p, g, diag, w.st, nu);
         }
         NBX_HIP(hipGetLastError());
         launched += batch;
@@ -552,6 +566,8 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
         }
         batch = batch < 64 ? batch * 2 : 64;
     }
+    hipLaunchKernelGGL(k_pcg_tail<V>, dim3(G), dim3(BLOCK), 0, s, x, p, w.st, nu);
+    NBX_HIP(hipGetLastError());
     NBX_HIP(hipEventRecord(w.ev[1], s));
     NBX_HIP(hipEventSynchronize(w.ev[1]));
     if (stats) {

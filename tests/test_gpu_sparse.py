@@ -66,6 +66,31 @@ def test_pcg_reference_semantics(case):
     A.destroy()
 
 
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 3, 5])
+def test_pcg_warm_start_cuts(max_iter):
+    """From a nonzero x0, cut after 0..5 iterations: the device's x (its last
+    x += alpha p is applied after the stop test), niter, status and
+    tol_reached follow the oracle's restatement of cg_precond_jacobi.c;
+    with 0 iterations x comes back bit for bit."""
+    import oracle
+    g = load_golden("cant_8x4_jit")
+    A = _sparse(g)
+    rng = np.random.default_rng(7)
+    x0 = rng.standard_normal(g["F"].size)
+    x, st, it, tr = A.solve_cg_jacobi(g["F"], x0=x0, max_iter=max_iter, tol=1e-10)
+    rp, col, val = g["rowptr"], g["col"], g["val"]
+    xr, itr, trr, str_ = oracle.pcg(rp, col, val, g["F"], x0=x0, max_iter=max_iter, tol=1e-10)
+    assert (st, it) == (str_, itr)
+    assert math.isclose(tr, trr, rel_tol=1e-12)
+    if max_iter == 0:
+        assert np.array_equal(x, x0)
+    assert np.linalg.norm(x - xr) <= 1e-12 * np.linalg.norm(xr)
+    # started at the converged solution: no iteration, x unchanged
+    x2, st2, it2, _ = A.solve_cg_jacobi(g["F"], x0=xr, max_iter=max_iter, tol=1e30)
+    assert (st2, it2) == (0, 0) and np.array_equal(x2, xr)
+    A.destroy()
+
+
 def test_pcg_c1_cantilever():
     """C1 (64x32 jittered cantilever, 4,290 DOF): the reference driver's solve."""
     g = load_golden("c1_cantilever_64x32")
