@@ -34,7 +34,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from nbots_amd.probe import spmv_bytes  # noqa: E402  (algorithmic bytes of one SpMV)
+from nbots_amd.probe import spmv_bytes, sym_spmv_bytes  # noqa: E402  (algorithmic bytes of one SpMV)
 
 METRIC = ("DOF·CG-iter/s + time-to-1e-8 residual (FP32+IR vs FP64); "
           "ms per SIMP iteration")
@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_pcg_A measured by two rocprofv3 --pmc passes
 # (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
 # 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md
-PMC_TRAFFIC = {"c3": 599187989}
+PMC_TRAFFIC = {("c3", False): 599187989}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
                                       "right-edge shear traction"),
@@ -55,14 +55,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pcg_iter_bytes(n_nodes, n_blocks, prec):
+def pcg_iter_bytes(n_nodes, n_blocks, prec, sym=False):
     """Algorithmic bytes of one CG iteration as implemented: SpMV kernel
     (above) + update kernel (FP64: read g,w,diag, write g = 4 vectors;
     FP32 scaled: read y,p,r,w, write y,r = 6) + direction kernel (FP64: read
     x,p,g,diag, write x,p = 6; FP32: read r,p, write p = 3)."""
     sv = 8 if prec == 64 else 4
     nv = 10 if prec == 64 else 9
-    return spmv_bytes(n_nodes, n_blocks, prec) + nv * (2 * n_nodes) * sv
+    spmv = sym_spmv_bytes(n_nodes, n_blocks) if sym else spmv_bytes(n_nodes, n_blocks, prec)
+    return spmv + nv * (2 * n_nodes) * sv
 
 
 def cpu_baseline(f, p, seconds, threads):
@@ -230,12 +231,17 @@ def main():
     # dominant kernel: the FP64 SpMV (+ p.w partials), sampled with HIP
     # events on the solver's stream during the timed steps
     spmv_ms = float(np.mean([i["ms_spmv"] for i in infos if i["ms_spmv"] > 0]))
+    sym = infos[-1]["layout"] == 1
     b64 = spmv_bytes(n_nodes, n_valid, 64)
-    achieved = b64 / (spmv_ms * 1e-3) / 1e9
+    bk = sym_spmv_bytes(n_nodes, n_valid) if sym else b64
+    achieved = bk / (spmv_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=PMC_TRAFFIC.get(args.config),
-                    kernel="k_pcg_A<OpS2> (FP64 2x2-block sliced-ELL SpMV + p.w partials)",
-                    bytes_per_launch=b64, us_per_launch=round(spmv_ms * 1e3, 2))
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=PMC_TRAFFIC.get((args.config, sym)),
+                    kernel=("k_pcg_A<OpSym> (FP64 SpMV, symmetric 2x2-block sliced ELL, "
+                            "+ p.w partials)" if sym else
+                            "k_pcg_A<OpS2> (FP64 2x2-block sliced-ELL SpMV + p.w partials)"),
+                    bytes_per_launch=bk, us_per_launch=round(spmv_ms * 1e3, 2))
 
     # mixed precision, timed separately: time to 1e-8 against the FP64 steps
     extra = {}
@@ -261,6 +267,7 @@ def main():
         ms32 = f.spmv_ms(32, 20)
         ms16 = f.spmv_ms(16, 20)
         ms64 = f.spmv_ms(64, 20)
+        ms65 = f.spmv_ms(65, 20) if sym else None
         extra = dict(
             time_to_1e8_fp64_s=round(t_fp64, 4),
             time_to_1e8_mixed_s=round(t_mixed, 4),
@@ -279,12 +286,17 @@ def main():
             mixed_vs_fp64_compliance_rel=abs(c32 - c64) / abs(c64),
             spmv64_us=round(ms64 * 1e3, 2),
             spmv64_gbs=round(b64 / (ms64 * 1e-3) / 1e9, 1),
+            spmv64_sym_us=None if ms65 is None else round(ms65 * 1e3, 2),
+            spmv64_sym_gbs=(None if ms65 is None else
+                            round(sym_spmv_bytes(n_nodes, n_valid) / (ms65 * 1e-3) / 1e9, 1)),
             spmv32_us=round(ms32 * 1e3, 2),
             spmv32_gbs=round(spmv_bytes(n_nodes, n_valid, 32) / (ms32 * 1e-3) / 1e9, 1),
             spmv16_us=round(ms16 * 1e3, 2),
             spmv16_gbs=round(spmv_bytes(n_nodes, n_valid, 16) / (ms16 * 1e-3) / 1e9, 1),
             fp64_iter_us=round(1e6 * t_fp64 / iters[-1], 1),
-            fp64_iter_gbs=round(pcg_iter_bytes(n_nodes, n_valid, 64) * iters[-1] / t_fp64 / 1e9, 1))
+            fp64_iter_gbs=round(pcg_iter_bytes(n_nodes, n_valid, 64, sym) * iters[-1] / t_fp64 / 1e9,
+                                1),
+            fp64_layout="symmetric" if sym else "full")
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: ~{args.cpu_seconds}s of iterations, {args.cpu_threads} threads")

@@ -323,7 +323,9 @@ typedef struct {
 	double ms;            /* device time of the solve                   */
 	int32_t fallback;     /* MIXED CG: sweeps stalled and FP64 PCG
 				 finished; CHEB: direction restarts         */
-	int32_t pad;
+	int32_t layout;       /* FP64 matrix layout of the solve: 0 full
+				 2x2-block rows, 1 symmetric (upper blocks,
+				 lower ones read transposed; same results)  */
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 
@@ -414,7 +416,10 @@ int nbx_fem_adjoint(nbx_fem_t *sys, const double *rhs, double *lambda,
  * of nbx_fem_adjoint for dL/du, dL/drho_e = -E'(rho_e) out_e.             */
 int nbx_fem_elem_cross(nbx_fem_t *sys, const double *a, const double *b,
 		       double *out);
-/* Device time of the last SpMV-dominated kernel sequence (ms). */
+/* Mean device time of reps back-to-back SpMVs on x = 1 (ms).  precision:
+ * 64 the FP64 values in the full 2x2-block layout; 65 the FP64 values in
+ * the symmetric layout the solvers use when K is symmetric (an error if it
+ * is not); 32 / 16 the Jacobi-scaled FP32 / fp16 copies.                  */
 int nbx_fem_spmv_bench(nbx_fem_t *sys, int precision, uint32_t reps,
 		       double *ms_per_spmv);
 

@@ -48,7 +48,8 @@ constexpr int WAVE = 64;          // CDNA wavefront
 constexpr int BLOCK = 256;        // 4 waves per workgroup
 constexpr int SLICE = 64;         // sliced-ELL slice height = one wavefront
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, 32 CUs each
-constexpr int MAX_GRID = 65535;   // workgroups of a reduction kernel (partials)
+constexpr int MAX_GRID = 65535;
+constexpr size_t INFINITY_CACHE_BYTES = 256ull << 20;   // MI355X last-level cache   // workgroups of a reduction kernel (partials)
 
 // A copy ordered on a handle's stream, complete on return (a plain hipMemcpy
 // would also wait for every other handle's work).
@@ -98,6 +99,29 @@ struct Sell2 {
     double* val = nullptr;         // [4 * n_blocks]
     float* val32 = nullptr;        // [4 * n_blocks], Jacobi-scaled FP32 copy
     _Float16* val16 = nullptr;     // [4 * n_blocks], Jacobi-scaled fp16 copy
+};
+
+// Symmetric 2x2-block sliced ELL, for a node matrix whose blocks satisfy
+// K(w, v) = K(v, w)^T bit for bit (what the T3 assembly produces).  Row v
+// stores its blocks (v, w >= v) in the Sell2 layout (upper part, ascending
+// w, diagonal first), and its blocks (v, w < v) as pairs (w, position of the
+// upper block (w, v)) (lower part, ascending w), read transposed.  A row
+// product visits the lower then the upper blocks, i.e. every column in
+// ascending order with the same products as the full layout, so results are
+// identical; the matrix stream shrinks from 36 B per block to 36 B per upper
+// and 8 B per lower block.
+struct Sym2 {
+    uint32_t n_nodes = 0;
+    uint32_t n_slices = 0;
+    size_t n_upper = 0;              // stored upper blocks, padding included
+    size_t n_lower = 0;              // stored lower entries, padding included
+    uint32_t* u_slice_ptr = nullptr; // [n_slices + 1]
+    uint8_t* u_rowlen = nullptr;     // [n_nodes]
+    uint32_t* u_bcol = nullptr;      // [n_upper]
+    double* u_val = nullptr;         // [4 * n_upper], two row planes as Sell2::val
+    uint32_t* l_slice_ptr = nullptr; // [n_slices + 1]
+    uint8_t* l_rowlen = nullptr;     // [n_nodes]
+    uint32_t* l_idx = nullptr;       // [2 * n_lower]: (w, upper position of (w, v))
 };
 
 // index of entry (i, j) of stored block idx in Sell2::val

@@ -210,6 +210,8 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
     nbx::SolveStats st;
     int status = 0;
     memset(info, 0, sizeof *info);
+    NBX_TRY(nbx::mat_prepare_sym(S.K, S.s));
+    info->layout = S.K.has_sym ? 1 : 0;
     if (mode == NBX_SOLVE_REFERENCE) {
         uint32_t it = 0;
         double tr = 0.0;
@@ -449,6 +451,7 @@ int nbx_fem_elem_cross(nbx_fem_t* f, const double* a, const double* b, double* o
 int nbx_fem_spmv_bench(nbx_fem_t* f, int precision, uint32_t reps, double* ms_per_spmv) {
     nbx::FemSys& S = f->sys;
     if (precision == 32 || precision == 16) NBX_TRY(nbx::mat_make_scaled(S.K, S.s, precision));
+    if (precision == 65) NBX_TRY(nbx::mat_prepare_sym(S.K, S.s));
     return nbx::spmv_bench(S.K, precision, reps, S.ws, S.s, ms_per_spmv);
 }
 
@@ -860,6 +863,7 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
                 }
                 uint32_t iters = 0;
                 double err = 0.0;
+                if ((rc = nbx::mat_prepare_sym(S.K, S.s))) goto END;
                 const int st = nbx::pcg_jacobi(S.K, res, du, N, 1e-8, &iters, &err, S.ws, S.s,
                                                nullptr);
                 if (st < 0) {

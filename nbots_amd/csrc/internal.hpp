@@ -37,6 +37,16 @@ struct DevMatrix {
     bool has_val32 = false;
     bool has_val16 = false;
     double lambda_max = 0.0;   // Gershgorin bound of the scaled matrix
+    // symmetric copy of the FP64 values (blocked matrices): its pattern is
+    // built once per pattern; its values are refreshed, and the symmetry of
+    // the values checked, when the solver first needs them after a change
+    Sym2 y;
+    bool sym_pattern = false;  // y's pattern is built (false: not yet, or the
+                               // pattern is not structurally symmetric)
+    bool sym_tried = false;    // y's pattern has been attempted
+    bool sym_checked = false;  // y's values reflect the current values
+    bool has_sym = false;      // ... and the values are symmetric: use y
+    unsigned* sym_flag = nullptr;
 };
 
 // Per-matrix solver scratch (device).  Sized for n scalar rows.
@@ -62,6 +72,7 @@ struct SolveStats {
     double ms = 0.0;           // device time of the solve
     double ms_spmv_avg = 0.0;  // average duration of the dominant SpMV kernel
     int fallback = 0;          // 1 if the FP32 sweeps stalled and FP64 PCG finished
+    int layout = 0;            // FP64 matrix layout used: 0 full, 1 symmetric
 };
 
 // Mixed-precision settings.
@@ -90,6 +101,9 @@ int mat_upload_csr(DevMatrix& m, uint32_t n, const uint32_t* rowptr, const uint3
 int mat_extract_diag(DevMatrix& m, hipStream_t s);
 // The values changed: the scaled copies and their bound are stale.
 void mat_scaled_stale(DevMatrix& m);
+// Build or refresh the symmetric FP64 copy when the matrix allows it
+// (m.has_sym tells); a no-op when it is current.
+int mat_prepare_sym(DevMatrix& m, hipStream_t s);
 // Jacobi-scaled copy D^-1/2 A D^-1/2 with 16- or 32-bit values (and the
 // Gershgorin bound lambda_max); a no-op when current.
 int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits);

@@ -69,6 +69,107 @@ __global__ void k_diag_sell2(Sell2 A, double* __restrict__ diag) {
     diag[2 * v + 1] = d1;
 }
 
+// ---- symmetric copy ---------------------------------------------------------
+// Per row: the number of block columns below the diagonal (lower) and from
+// the diagonal on (upper).  A row whose columns are not strictly ascending
+// sets *bad (the symmetric layout then is not used).
+__global__ void k_sym_count(Sell2 A, uint8_t* __restrict__ ulen, uint8_t* __restrict__ llen,
+                            unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    int lo = 0;
+    int64_t prev = -1;
+    for (int k = 0; k < len; ++k) {
+        const uint32_t w = A.bcol[g + (size_t)k * SLICE];
+        if ((int64_t)w <= prev) *bad = 1u;
+        prev = w;
+        lo += w < v;
+    }
+    ulen[v] = (uint8_t)(len - lo);
+    llen[v] = (uint8_t)lo;
+}
+
+// Upper columns, and for every lower block (v, w) the pair (w, position of
+// the upper block (w, v)); *bad when row w lacks column v.  Padding: upper
+// (column v), lower (v, 0); never read.
+__global__ void k_sym_pattern(Sell2 A, Sym2 Y, unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int lo = Y.l_rowlen[v], up = Y.u_rowlen[v];
+    const uint32_t s = v >> 6;
+    const size_t u = (size_t)Y.u_slice_ptr[s] + (v & 63);
+    const uint32_t uw = (Y.u_slice_ptr[s + 1] - Y.u_slice_ptr[s]) / SLICE;
+    for (uint32_t k = 0; k < uw; ++k)
+        Y.u_bcol[u + (size_t)k * SLICE] =
+            (int)k < up ? A.bcol[g + (size_t)(lo + (int)k) * SLICE] : v;
+    const size_t l = (size_t)Y.l_slice_ptr[s] + (v & 63);
+    const uint32_t lw = (Y.l_slice_ptr[s + 1] - Y.l_slice_ptr[s]) / SLICE;
+    for (uint32_t k = 0; k < lw; ++k) {
+        uint32_t w = v, pos = 0;
+        if ((int)k < lo) {
+            w = A.bcol[g + (size_t)k * SLICE];
+            const size_t gw = (size_t)A.slice_ptr[w >> 6] + (w & 63);
+            const int lenw = A.rowlen[w], low = Y.l_rowlen[w];
+            int t = -1;
+            for (int kk = 0; kk < lenw; ++kk) {
+                if (A.bcol[gw + (size_t)kk * SLICE] == v) {
+                    t = kk;
+                    break;
+                }
+            }
+            if (t < low) {
+                *bad = 1u;
+                t = low;
+            }
+            pos = Y.u_slice_ptr[w >> 6] + (w & 63) + (uint32_t)(t - low) * SLICE;
+        }
+        Y.l_idx[2 * (l + (size_t)k * SLICE)] = w;
+        Y.l_idx[2 * (l + (size_t)k * SLICE) + 1] = pos;
+    }
+}
+
+// Copy the upper blocks' values from the full layout.
+__global__ void k_sym_values(Sell2 A, Sym2 Y) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const size_t u = (size_t)Y.u_slice_ptr[v >> 6] + (v & 63);
+    const int lo = Y.l_rowlen[v], up = Y.u_rowlen[v];
+    for (int k = 0; k < up; ++k) {
+        const size_t gi = g + (size_t)(lo + k) * SLICE, ui = u + (size_t)k * SLICE;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                Y.u_val[ent(Y.n_upper, ui, i, j)] = A.val[ent(A.n_blocks, gi, i, j)];
+    }
+}
+
+// *bad unless every lower block K(v, w) equals the transpose of the stored
+// upper block K(w, v) bit for bit.
+__global__ void k_sym_check(Sell2 A, Sym2 Y, unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const size_t l = (size_t)Y.l_slice_ptr[v >> 6] + (v & 63);
+    const int lo = Y.l_rowlen[v];
+    bool ok = true;
+    for (int k = 0; k < lo; ++k) {
+        const size_t gi = g + (size_t)k * SLICE;
+        const size_t pos = Y.l_idx[2 * (l + (size_t)k * SLICE) + 1];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                ok = ok && __double_as_longlong(A.val[ent(A.n_blocks, gi, i, j)]) ==
+                               __double_as_longlong(Y.u_val[ent(Y.n_upper, pos, j, i)]);
+    }
+    if (!ok) *bad = 1u;
+}
+
 // nb_sparse_get(A, i, i): 0.0 when (i,i) is not stored (sparse.c:164-171)
 __global__ void k_diag_csr(DevCsr A, double* __restrict__ diag) {
     const uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
@@ -240,7 +341,19 @@ void ws_free(Workspace& w) {
     w.n = 0;
 }
 
+static void sym_free(DevMatrix& m) {
+    Sym2& Y = m.y;
+    void* ps[] = {Y.u_slice_ptr, Y.u_rowlen, Y.u_bcol, Y.u_val,
+                  Y.l_slice_ptr, Y.l_rowlen, Y.l_idx,  m.sym_flag};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    Y = Sym2();
+    m.sym_flag = nullptr;
+    m.sym_pattern = m.sym_checked = m.has_sym = false;
+}
+
 void mat_free(DevMatrix& m) {
+    sym_free(m);
     void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32, m.s.val16,
                   m.c.rowptr,    m.c.col,    m.c.val,  m.diag,  m.dis};
     for (void* p : ps)
@@ -378,6 +491,99 @@ void mat_scaled_stale(DevMatrix& m) {
     m.has_scale = false;
     m.has_val32 = false;
     m.has_val16 = false;
+    m.sym_checked = false;
+    m.has_sym = false;
+}
+
+static int read_flag(unsigned* dflag, hipStream_t s, unsigned* out) {
+    NBX_HIP(hipMemcpyAsync(out, dflag, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    return NBX_OK;
+}
+
+// Slice offsets of a layout whose row v holds len[v] blocks; false when
+// they exceed 32 bits.
+static bool slice_offsets(const std::vector<uint8_t>& len, uint32_t ns,
+                          std::vector<uint32_t>& sp) {
+    const uint32_t nn = (uint32_t)len.size();
+    sp.assign(ns + 1, 0);
+    for (uint32_t sl = 0; sl < ns; ++sl) {
+        uint32_t wdt = 0;
+        for (uint32_t v = sl * SLICE; v < nn && v < (sl + 1) * SLICE; ++v)
+            if (len[v] > wdt) wdt = len[v];
+        if ((uint64_t)sp[sl] + (uint64_t)wdt * SLICE > 0xffffffffull) return false;
+        sp[sl + 1] = sp[sl] + wdt * SLICE;
+    }
+    return true;
+}
+
+// The symmetric layout's pattern from the full one.  Leaves m.sym_pattern
+// false (and nothing allocated) when the pattern does not allow it: rows
+// not strictly ascending, a lower block without its transposed partner,
+// offsets beyond 32 bits, or no device memory for the copy.
+static int sym_build_pattern(DevMatrix& m, hipStream_t s) {
+    const Sell2& A = m.s;
+    Sym2& Y = m.y;
+    const uint32_t nn = A.n_nodes, ns = A.n_slices;
+    auto give_up = [&]() {
+        (void)hipGetLastError();
+        sym_free(m);
+        return NBX_OK;
+    };
+    if (hipMalloc(&m.sym_flag, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&Y.u_rowlen, (size_t)nn + 1) != hipSuccess ||
+        hipMalloc(&Y.l_rowlen, (size_t)nn + 1) != hipSuccess)
+        return give_up();
+    NBX_HIP(hipMemsetAsync(m.sym_flag, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_sym_count, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, Y.u_rowlen,
+                       Y.l_rowlen, m.sym_flag);
+    NBX_HIP(hipGetLastError());
+    std::vector<uint8_t> ul(nn), ll(nn);
+    NBX_HIP(hipMemcpyAsync(ul.data(), Y.u_rowlen, nn, hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(ll.data(), Y.l_rowlen, nn, hipMemcpyDeviceToHost, s));
+    unsigned bad = 0;
+    NBX_TRY(read_flag(m.sym_flag, s, &bad));
+    std::vector<uint32_t> usp, lsp;
+    if (bad || !slice_offsets(ul, ns, usp) || !slice_offsets(ll, ns, lsp)) return give_up();
+    Y.n_nodes = nn;
+    Y.n_slices = ns;
+    Y.n_upper = usp[ns];
+    Y.n_lower = lsp[ns];
+    if (hipMalloc(&Y.u_slice_ptr, (ns + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&Y.l_slice_ptr, (ns + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&Y.u_bcol, (Y.n_upper + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&Y.u_val, (4 * Y.n_upper + 4) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&Y.l_idx, (2 * Y.n_lower + 2) * sizeof(uint32_t)) != hipSuccess)
+        return give_up();
+    NBX_HIP(hipMemcpyAsync(Y.u_slice_ptr, usp.data(), (ns + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    NBX_HIP(hipMemcpyAsync(Y.l_slice_ptr, lsp.data(), (ns + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_sym_pattern, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, Y, m.sym_flag);
+    NBX_HIP(hipGetLastError());
+    NBX_TRY(read_flag(m.sym_flag, s, &bad));
+    if (bad) return give_up();
+    m.sym_pattern = true;
+    return NBX_OK;
+}
+
+int mat_prepare_sym(DevMatrix& m, hipStream_t s) {
+    if (!m.blocked || m.n == 0) return NBX_OK;
+    if (!m.sym_tried) {
+        m.sym_tried = true;
+        NBX_TRY(sym_build_pattern(m, s));
+    }
+    if (!m.sym_pattern || m.sym_checked) return NBX_OK;
+    const unsigned g = grid_for(m.s.n_nodes);
+    NBX_HIP(hipMemsetAsync(m.sym_flag, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_sym_values, dim3(g), dim3(BLOCK), 0, s, m.s, m.y);
+    hipLaunchKernelGGL(k_sym_check, dim3(g), dim3(BLOCK), 0, s, m.s, m.y, m.sym_flag);
+    NBX_HIP(hipGetLastError());
+    unsigned bad = 0;
+    NBX_TRY(read_flag(m.sym_flag, s, &bad));
+    m.sym_checked = true;
+    m.has_sym = bad == 0;
+    return NBX_OK;
 }
 
 int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
@@ -437,7 +643,11 @@ int mat_prepare_mixed(DevMatrix& m, hipStream_t s, const MixedOpts& o) {
 
 int spmv(const DevMatrix& m, const double* x, double* y, hipStream_t s) {
     if (m.n == 0) return NBX_OK;
-    if (m.blocked) {
+    if (m.blocked && m.has_sym) {
+        OpSym op{m.y};
+        hipLaunchKernelGGL(k_spmv<OpSym>, dim3(map_grid(m.s.n_nodes)), dim3(BLOCK), 0, s, op,
+                           (const double2*)x, (double2*)y, m.s.n_nodes);
+    } else if (m.blocked) {
         OpS2 op{m.s};
         hipLaunchKernelGGL(k_spmv<OpS2>, dim3(map_grid(m.s.n_nodes)), dim3(BLOCK), 0, s, op,
                            (const double2*)x, (double2*)y, m.s.n_nodes);
@@ -468,6 +678,14 @@ int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStr
         set_last_error("spmv_bench: no scaled %d-bit copy", prec);
         return NBX_ERR_ARG;
     }
+    if (prec == 65 && !m.has_sym) {
+        set_last_error("spmv_bench: the matrix has no symmetric copy");
+        return NBX_ERR_ARG;
+    }
+    if (prec != 64 && prec != 65 && prec != 32 && prec != 16) {
+        set_last_error("spmv_bench: precision %d (64, 65, 32 or 16)", prec);
+        return NBX_ERR_ARG;
+    }
     auto launch = [&]() {
         if (prec == 32)
             hipLaunchKernelGGL(k_spmv_scaled<32>, dim3(G), dim3(BLOCK), 0, s, m.s,
@@ -475,6 +693,9 @@ int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStr
         else if (prec == 16)
             hipLaunchKernelGGL(k_spmv_scaled<16>, dim3(G), dim3(BLOCK), 0, s, m.s,
                                (const float2*)w.v32[0], (float2*)w.v32[1], nu);
+        else if (prec == 65)
+            hipLaunchKernelGGL(k_spmv<OpSym>, dim3(G), dim3(BLOCK), 0, s, OpSym{m.y},
+                               (const double2*)w.v64[0], (double2*)w.v64[1], nu);
         else
             hipLaunchKernelGGL(k_spmv<OpS2>, dim3(G), dim3(BLOCK), 0, s, OpS2{m.s},
                                (const double2*)w.v64[0], (double2*)w.v64[1], nu);

@@ -9,13 +9,57 @@ namespace nbx {
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
+typedef float fv2 __attribute__((ext_vector_type(2)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+typedef uint32_t uv2 __attribute__((ext_vector_type(2)));
 
 // The matrix stream is read once per product: non-temporal loads keep it
 // from evicting the gathered vector out of L2.
 template <class T>
 __device__ __forceinline__ T nt(const T* p) {
     return __builtin_nontemporal_load(p);
+}
+
+// Vector streams that the next kernels do not gather (PCG updates, the
+// Chebyshev recurrences) use non-temporal loads and stores as well: they
+// then leave the L2 and the Infinity Cache to the matrix and to the
+// gathered vector.
+__device__ __forceinline__ double ldn(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double2 ldn(const double2* p) {
+    const dv2 v = __builtin_nontemporal_load(reinterpret_cast<const dv2*>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void stn(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void stn(double2* p, double2 v) {
+    dv2 t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<dv2*>(p));
+}
+__device__ __forceinline__ float2 ldn(const float2* p) {
+    const fv2 v = __builtin_nontemporal_load(reinterpret_cast<const fv2*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stn(float2* p, float2 v) {
+    fv2 t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<fv2*>(p));
+}
+
+template <bool NT, class V>
+__device__ __forceinline__ V ldv(const V* p) {
+    if constexpr (NT)
+        return ldn(p);
+    else
+        return *p;
+}
+template <bool NT, class V>
+__device__ __forceinline__ void stv(V* p, V v) {
+    if constexpr (NT)
+        stn(p, v);
+    else
+        *p = v;
 }
 
 // ---- per-unit vector arithmetic (unit = node with 2 dofs, or scalar row) --
@@ -78,6 +122,83 @@ __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const doubl
                 s0 = s0 + a[k].y * xv[k].y;
                 s1 = s1 + b[k].x * xv[k].x;
                 s1 = s1 + b[k].y * xv[k].y;
+            }
+        }
+    }
+    return make_double2(s0, s1);
+}
+
+// Symmetric layout: the lower blocks (w < v, ascending), each the transpose
+// of the stored upper block (w, v), then the upper blocks (w >= v,
+// ascending); the same products in the same order as row64.  The transposed
+// blocks belong to rows that nearby workgroups of the same XCD stream at
+// about the same time, so their loads are served by L2.  Lower and upper
+// blocks are taken UL and UU at a time; few registers per row keep many
+// waves resident, which hides the dependent loads better than longer
+// batches (4 and 4: 74 VGPRs, 6 waves per SIMD).
+template <int UL, int UU>
+__device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const double2* __restrict__ x) {
+    const dv2* __restrict__ pa = reinterpret_cast<const dv2*>(Y.u_val);
+    const dv2* __restrict__ pb = reinterpret_cast<const dv2*>(Y.u_val + 2 * Y.n_upper);
+    double s0 = 0.0, s1 = 0.0;
+    {
+        const uv2* __restrict__ li = reinterpret_cast<const uv2*>(Y.l_idx);
+        const size_t base = (size_t)Y.l_slice_ptr[v >> 6] + (v & 63);
+        const int len = Y.l_rowlen[v];
+        for (int k0 = 0; k0 < len; k0 += UL) {
+            uv2 e[UL];
+            dv2 a[UL], b[UL];
+            double2 xv[UL];
+#pragma unroll
+            for (int k = 0; k < UL; ++k)
+                if (k0 + k < len) e[k] = nt(&li[base + (size_t)(k0 + k) * SLICE]);
+#pragma unroll
+            for (int k = 0; k < UL; ++k) {
+                if (k0 + k < len) {
+                    a[k] = pa[e[k].y];
+                    b[k] = pb[e[k].y];
+                    xv[k] = x[e[k].x];
+                }
+            }
+            // block (v, w) = [[a.x, b.x], [a.y, b.y]]
+#pragma unroll
+            for (int k = 0; k < UL; ++k) {
+                if (k0 + k < len) {
+                    s0 = s0 + a[k].x * xv[k].x;
+                    s0 = s0 + b[k].x * xv[k].y;
+                    s1 = s1 + a[k].y * xv[k].x;
+                    s1 = s1 + b[k].y * xv[k].y;
+                }
+            }
+        }
+    }
+    {
+        const size_t base = (size_t)Y.u_slice_ptr[v >> 6] + (v & 63);
+        const int len = Y.u_rowlen[v];
+        for (int k0 = 0; k0 < len; k0 += UU) {
+            uint32_t w[UU];
+            dv2 a[UU], b[UU];
+            double2 xv[UU];
+#pragma unroll
+            for (int k = 0; k < UU; ++k) {
+                if (k0 + k < len) {
+                    const size_t i = base + (size_t)(k0 + k) * SLICE;
+                    w[k] = nt(&Y.u_bcol[i]);
+                    a[k] = nt(&pa[i]);
+                    b[k] = nt(&pb[i]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < UU; ++k)
+                if (k0 + k < len) xv[k] = x[w[k]];
+#pragma unroll
+            for (int k = 0; k < UU; ++k) {
+                if (k0 + k < len) {
+                    s0 = s0 + a[k].x * xv[k].x;
+                    s0 = s0 + a[k].y * xv[k].y;
+                    s1 = s1 + b[k].x * xv[k].x;
+                    s1 = s1 + b[k].y * xv[k].y;
+                }
             }
         }
     }
@@ -167,6 +288,14 @@ struct OpS2 {
     using V = double2;
     __device__ __forceinline__ double2 row(uint32_t v, const double2* __restrict__ x) const {
         return row64(A, v, x);
+    }
+};
+
+struct OpSym {
+    Sym2 Y;
+    using V = double2;
+    __device__ __forceinline__ double2 row(uint32_t v, const double2* __restrict__ x) const {
+        return row64s<4, 4>(Y, v, x);
     }
 };
 

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(FIN) void k_fin_B(PcgState* st, const double* parti
 
 // g += alpha w; q = g / Aii; rz' = g.q; gg' = g.g, then the lagged stop test
 // (cg_precond_jacobi.c:63-70, :75); x is left to the direction kernel.
-template <class V>
+template <class V, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ g, const V* __restrict__ w,
                                                   const V* __restrict__ diag, PcgState* st,
                                                   double* partials, uint32_t n) {
@@ -132,9 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ g, const V* __r
     double acc[2] = {0.0, 0.0};
     const Chunk c = my_chunk(n);
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
-        const V gv = axpyu(g[u], alpha, w[u]);
-        g[u] = gv;
-        const V q = divu(gv, diag[u]);
+        const V gv = axpyu(ldv<NT>(&g[u]), alpha, ldv<NT>(&w[u]));
+        stv<NT>(&g[u], gv);
+        const V q = divu(gv, ldv<NT>(&diag[u]));
         acc[0] += dotu(gv, q);
         acc[1] += dotu(gv, gv);
     }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ g, const V* __r
 // x += alpha p; p = -q + beta p  (cg_precond_jacobi.c:62, :72-74).  Once the
 // stop test has fired this kernel no longer runs, and k_pcg_tail applies the
 // last iteration's x update.
-template <class V>
+template <class V, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, V* __restrict__ p,
                                                   const V* __restrict__ g,
                                                   const V* __restrict__ diag, PcgState* st,
@@ -155,10 +155,10 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, V* __restric
     const double alpha = st->alpha, beta = st->beta;
     const Chunk c = my_chunk(n);
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
-        const V pu = p[u];
-        x[u] = axpyu(x[u], alpha, pu);
-        const V q = divu(g[u], diag[u]);
-        p[u] = pupd(q, beta, pu);
+        const V pu = ldv<NT>(&p[u]);
+        stv<NT>(&x[u], axpyu(ldv<NT>(&x[u]), alpha, pu));
+        const V q = divu(ldv<NT>(&g[u]), ldv<NT>(&diag[u]));
+        p[u] = pupd(q, beta, pu);   // gathered by the next SpMV: cached store
     }
 }
 
@@ -525,7 +525,7 @@ static int sampled_ms(Workspace& w, uint32_t n, double* avg) {
 // is g exactly, so the same kernels give conjugate_gradient.c:13-77).
 template <class Op>
 static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typename Op::V* b,
-                   typename Op::V* x, uint32_t max_iter, double tol, Workspace& w,
+                   typename Op::V* x, uint32_t max_iter, double tol, bool stream, Workspace& w,
                    hipStream_t s, SolveStats* stats) {
     using V = typename Op::V;
     V* g = (V*)w.v64[0];
@@ -551,10 +551,19 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
                                nu);
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
             hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            hipLaunchKernelGGL(k_pcg_B<V>, dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
-                               w.partials, nu);
-            hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            hipLaunchKernelGGL(k_pcg_C<V>, dim3(G), dim3(BLOCK), 0, s, x, p, g, diag, w.st, nu);
+            if (stream) {
+                hipLaunchKernelGGL((k_pcg_B<V, true>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
+                                   w.partials, nu);
+                hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+                hipLaunchKernelGGL((k_pcg_C<V, true>), dim3(G), dim3(BLOCK), 0, s, x, p, g, diag,
+                                   w.st, nu);
+            } else {
+                hipLaunchKernelGGL((k_pcg_B<V, false>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag,
+                                   w.st, w.partials, nu);
+                hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+                hipLaunchKernelGGL((k_pcg_C<V, false>), dim3(G), dim3(BLOCK), 0, s, x, p, g, diag,
+                                   w.st, nu);
+            }
         }
         NBX_HIP(hipGetLastError());
         launched += batch;
@@ -606,12 +615,23 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
         if (hipGetLastError() != hipSuccess) return -NBX_ERR_HIP;
         diag = w.v64[5];
     }
-    if (m.blocked)
-        rc = pcg_run(OpS2{m.s}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol, w,
-                     s, stats);
+    // Stream the update kernels' vectors past the caches when the
+    // iteration's working set (matrix + five vectors) exceeds the 256 MiB
+    // Infinity Cache; below that, caching them is faster.
+    const double mat_bytes = !m.blocked ? 12.0 * (double)m.c.nnz
+                             : m.has_sym ? 36.0 * (double)m.y.n_upper + 8.0 * (double)m.y.n_lower
+                                         : 36.0 * (double)m.s.n_blocks;
+    const bool stream = mat_bytes + 40.0 * (double)m.n > (double)INFINITY_CACHE_BYTES;
+    if (m.blocked && m.has_sym)
+        rc = pcg_run(OpSym{m.y}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol,
+                     stream, w, s, stats);
+    else if (m.blocked)
+        rc = pcg_run(OpS2{m.s}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol,
+                     stream, w, s, stats);
     else
-        rc = pcg_run(OpCsr{m.c}, m.c.n, diag, b, x, max_iter, tol, w, s, stats);
+        rc = pcg_run(OpCsr{m.c}, m.c.n, diag, b, x, max_iter, tol, stream, w, s, stats);
     if (rc) return -rc;
+    if (stats) stats->layout = m.blocked && m.has_sym ? 1 : 0;
     if (niter) *niter = w.h_st->k;
     if (tol_reached) *tol_reached = w.h_st->tol_reached;
     return (int)w.h_st->status;
@@ -780,6 +800,7 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
     double2* xv = (double2*)x;
     const unsigned G = red_grid(nu);
     OpS2 op{m.s};
+    OpSym ops{m.y};
     auto precondition = [&](int first) {
         hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv, dis, res,
                            da, z, w.st, w.partials, nu, first, inv_theta);
@@ -812,8 +833,12 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
         for (uint32_t i = 0; i < batch; ++i) {
             const bool samp = first_batch && nsamp < (uint32_t)Workspace::NSAMP;
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
-            hipLaunchKernelGGL(k_pcg_A<OpS2>, dim3(G), dim3(BLOCK), 0, s, op, (const double2*)p, wv,
-                               w.st, w.partials, nu);
+            if (m.has_sym)
+                hipLaunchKernelGGL(k_pcg_A<OpSym>, dim3(G), dim3(BLOCK), 0, s, ops,
+                                   (const double2*)p, wv, w.st, w.partials, nu);
+            else
+                hipLaunchKernelGGL(k_pcg_A<OpS2>, dim3(G), dim3(BLOCK), 0, s, op,
+                                   (const double2*)p, wv, w.st, w.partials, nu);
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
             hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
             precondition(0);
@@ -839,6 +864,7 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
+        stats->layout = m.has_sym ? 1 : 0;
         NBX_TRY(sampled_ms(w, nsamp < w.h_st->k ? nsamp : w.h_st->k, &stats->ms_spmv_avg));
     }
     return w.h_st->status ? NBX_NOT_CONVERGED : NBX_OK;

@@ -514,6 +514,8 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
     if (A->N == 0) return NBX_OK;
     int rc = sync_device(A);
     if (rc) return rc;
+    NBX_TRY(nbx::mat_prepare_sym(A->dev, A->stream));
+    info->layout = A->dev.has_sym ? 1 : 0;
     double* db = A->ws.v64[3];
     double* dx = A->ws.v64[4];
     NBX_HIP(hipMemcpyAsync(db, b, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));

@@ -35,11 +35,11 @@ U32, U64, F64, I32, VP = C.c_uint32, C.c_uint64, C.c_double, C.c_int, C.c_void_p
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_uint32), ("outer", C.c_uint32), ("fp64_iters", C.c_uint32),
                 ("status", C.c_int32), ("rel_res", C.c_double), ("tol_reached", C.c_double),
-                ("ms", C.c_double), ("fallback", C.c_int32), ("pad", C.c_int32),
+                ("ms", C.c_double), ("fallback", C.c_int32), ("layout", C.c_int32),
                 ("ms_spmv", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SolveOpts(C.Structure):

@@ -31,6 +31,18 @@ def spmv_bytes(n_nodes: int, n_blocks: int, prec: int) -> int:
     return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sx
 
 
+def sym_spmv_bytes(n_nodes: int, n_valid: int) -> int:
+    """Algorithmic HBM bytes of one FP64 SpMV in the symmetric layout: per
+    upper block (v, w >= v) its column and 4 values (36 B); per lower block
+    its (column, upper position) pair (8 B; the transposed values are those
+    of upper blocks streamed by nearby workgroups, served by L2); per node
+    two row lengths, the x read and the y write; per 64-node slice two
+    offsets.  With every diagonal block stored, upper = (valid + nodes) / 2
+    and lower = (valid - nodes) / 2 blocks."""
+    up, lo = (n_valid + n_nodes) // 2, (n_valid - n_nodes) // 2
+    return up * 36 + lo * 8 + 2 * n_nodes + 8 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * 8
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nx", type=int, default=2048)
@@ -49,9 +61,10 @@ def main():
     N, nnz, nblk = f.sizes()
     n_nodes, n_valid = N // 2, nnz // 4
     print(f"[probe] {args.nx}x{args.ny} cantilever: {N} DOF, {n_valid} blocks, "
-          f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64)", flush=True)
+          f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64, full layout), "
+          f"{sym_spmv_bytes(n_nodes, n_valid)} (symmetric layout)", flush=True)
     info = f.solve(SOLVE_FP64, tol=0.0, max_iter=args.iters)
-    print(f"[probe] FP64 PCG: {info['iters']} iterations", flush=True)
+    print(f"[probe] FP64 PCG: {info['iters']} iterations, layout {info['layout']}", flush=True)
     f.destroy()
 
     # b = 2 a: an elementwise kernel that reads and writes every byte once

@@ -146,6 +146,7 @@ def test_solve_modes_traction(mode, variant, bits):
     info = f.solve(mode, tol=1e-8,
                    opts=None if variant is None else solve_opts(variant, precond_bits=bits))
     assert info["status"] == 0, info
+    assert info["layout"] == 1      # the assembled K is symmetric bit for bit
     u = f.displacement()
     rp, col, val, F = f.system()
     r = F - oracle.spmv(rp, col, val, u)
@@ -221,7 +222,8 @@ def test_spmv_bench_entry():
     from nbots_amd.fem import Fem
     p = synth.cantilever_traction(64, 32)
     f = Fem.from_problem(p)
-    assert f.spmv_ms(64, 5) > 0 and f.spmv_ms(32, 5) > 0
+    assert f.spmv_ms(64, 5) > 0 and f.spmv_ms(32, 5) > 0 and f.spmv_ms(16, 5) > 0
+    assert f.spmv_ms(65, 5) > 0     # symmetric FP64 layout
     f.destroy()
 
 

@@ -31,6 +31,46 @@ def test_spmv_bit_exact(case):
     A.destroy()
 
 
+@pytest.mark.parametrize("case", SMALL_CASES)
+def test_symmetric_layout_bit_exact(case):
+    """The assembled K is symmetric bit for bit, so a solve builds the
+    symmetric layout (upper blocks, lower ones read transposed) and reports
+    layout 1; nb_sparse_multiply_vector then runs on it and still equals the
+    compiled reference's SpMV bit for bit, and the PCG result matches the
+    reference's as on the full layout."""
+    g = load_golden(case)
+    A = _sparse(g)
+    x, info = A.solve(g["F"], mode=0, tol=float(g["pcg_tol"]))
+    assert info["layout"] == 1
+    assert abs(info["iters"] - int(g["niter"])) <= 2
+    assert np.linalg.norm(x - g["x"]) <= 1e-10 * np.linalg.norm(g["x"])
+    y = A.matvec(np.sin(np.arange(A.N, dtype=np.float64)))
+    assert np.array_equal(y, g["spmv_sin"]), np.abs(y - g["spmv_sin"]).max()
+    A.destroy()
+
+
+def test_symmetric_layout_refused_for_asymmetric_values():
+    """One off-diagonal entry changed on one side: the symmetry check fails,
+    the solve runs on the full layout (layout 0), SpMV stays exact, and after
+    the values are restored the next solve uses the symmetric copy again."""
+    import oracle
+    g = load_golden("cant_8x4_jit")
+    A = _sparse(g)
+    rp, col, val = g["rowptr"], g["col"], g["val"].copy()
+    r = 6
+    j = [k for k in range(rp[r], rp[r + 1]) if col[k] // 2 != r // 2 and val[k] != 0.0][0]
+    val[j] *= 1.0 + 2.0 ** -40
+    A.import_values(val)
+    _, info = A.solve(g["F"], mode=0, tol=1e-8)
+    assert info["layout"] == 0
+    xs = np.cos(np.arange(A.N, dtype=np.float64))
+    assert np.array_equal(A.matvec(xs), oracle.spmv(rp, col, val, xs))
+    A.import_values(g["val"])
+    _, info = A.solve(g["F"], mode=0, tol=1e-8)
+    assert info["layout"] == 1
+    A.destroy()
+
+
 @pytest.mark.parametrize("vpn", [1, 3])
 def test_spmv_scalar_csr_path(vpn):
     """vars_per_node != 2 goes through the scalar CSR kernels."""
