@@ -41,8 +41,9 @@ METRIC = ("DOF·CG-iter/s + time-to-1e-8 residual (FP32+IR vs FP64); "
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_pcg_A measured by two rocprofv3 --pmc passes
 # (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
-# 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md
-PMC_TRAFFIC = {("c3", False): 599187989}
+# 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md;
+# keyed by (config, symmetric layout)
+PMC_TRAFFIC = {("c3", False): 599187989, ("c3", True): 434355412}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
                                       "right-edge shear traction"),
