@@ -173,8 +173,10 @@ def main():
     ap.add_argument("--mixed", default="cheb", choices=["cheb", "restart", "reliable"])
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
-    ap.add_argument("--cheb-degree", type=int, default=None, help="library default 16")
-    ap.add_argument("--cheb-ratio", type=float, default=None, help="library default 3000")
+    ap.add_argument("--cheb-degree", type=int, default=None,
+                    help="library default 16 (SIMP loop: 32)")
+    ap.add_argument("--cheb-ratio", type=float, default=None,
+                    help="library default 3000 (SIMP loop: 3e4)")
     ap.add_argument("--precond-bits", type=int, default=16, choices=[16, 32])
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--seed", type=int, default=20261202)
@@ -309,7 +311,7 @@ def main():
     if not args.no_simp:
         simp = simp_bench(args, solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree,
                                            cheb_ratio=args.cheb_ratio,
-                                           precond_bits=args.precond_bits))
+                                           precond_bits=args.precond_bits, simp=True))
     line = dict(
         metric=METRIC, value=value, unit="DOF*CG-iter/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True,

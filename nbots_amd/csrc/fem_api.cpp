@@ -336,8 +336,7 @@ static int fem_solve_entry(nbx_fem_t* f, int mode, double tol, uint32_t max_iter
     return fem_solve(S, mode, tol, max_iter, mo, info);
 }
 
-void nbx_solve_opts_default(nbx_solve_opts_t* o) {
-    const nbx::MixedOpts d;
+static void opts_from(const nbx::MixedOpts& d, nbx_solve_opts_t* o) {
     o->mixed_variant = (int32_t)d.variant;
     o->cheb_degree = d.cheb_degree;
     o->cheb_ratio = d.cheb_ratio;
@@ -346,6 +345,10 @@ void nbx_solve_opts_default(nbx_solve_opts_t* o) {
     o->precond_bits = d.precond_bits;
     o->reserved = 0;
 }
+
+void nbx_solve_opts_default(nbx_solve_opts_t* o) { opts_from(nbx::MixedOpts(), o); }
+
+void nbx_simp_opts_default(nbx_solve_opts_t* o) { opts_from(nbx::simp_mixed_defaults(), o); }
 
 int nbx_fem_solve(nbx_fem_t* f, int mode, double tol, uint32_t max_iter, double inner_rtol,
                   const double* u_init, nbx_solve_info_t* info) {
@@ -497,7 +500,7 @@ int nbx_simp_iterate(nbx_simp_t* h, int mode, double tol, uint32_t max_iter,
     nbx_solve_info_t tmp;
     if (!info) info = &tmp;
     nbx::MixedOpts mo;
-    NBX_TRY(nbx::mixed_opts(opts, &mo));
+    NBX_TRY(nbx::mixed_opts(opts, &mo, nbx::simp_mixed_defaults()));
     // K(E(xPhys)) with the system's boundary conditions; the solve starts
     // from the previous displacement
     NBX_TRY(nbx::simp_moduli(h->o, S));

@@ -20,10 +20,10 @@ inline MixedOpts mixed_opts(double inner_rtol) {
     return o;
 }
 
-// Fields at 0 keep their defaults.  NBX_ERR_ARG for an unknown variant or
-// a precond_bits other than 0, 16 or 32.
-inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o) {
-    *o = MixedOpts();
+// Fields at 0 keep the defaults in `base`.  NBX_ERR_ARG for an unknown
+// variant or a precond_bits other than 0, 16 or 32.
+inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& base = MixedOpts()) {
+    *o = base;
     if (!p) return NBX_OK;
     if (p->mixed_variant < NBX_MIXED_CG_RESTART || p->mixed_variant > NBX_MIXED_CHEB_FCG) {
         set_last_error("unknown mixed-precision variant %d", (int)p->mixed_variant);
@@ -40,6 +40,17 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o) {
     if (p->reliable_delta > 0.0) o->delta = p->reliable_delta;
     if (p->precond_bits) o->precond_bits = p->precond_bits;
     return NBX_OK;
+}
+
+// Defaults of the SIMP loop's mixed solves: its systems carry void elements
+// (E = Emin = 1e-9 E0), and a wider, higher-degree Chebyshev interval pays
+// there (1.05M-DOF MBB beam, 25 iterations: 259 ms per iteration against
+// 283 ms with the plate defaults of degree 16 on [lmax/3000, lmax]).
+inline MixedOpts simp_mixed_defaults() {
+    MixedOpts o;
+    o.cheb_degree = 32;
+    o.cheb_ratio = 3e4;
+    return o;
 }
 
 }  // namespace nbx

@@ -13,10 +13,14 @@ MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
 
 
 def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,
-               reliable_delta=None, precond_bits=None):
-    """nbx_solve_opts_t with the library's defaults, then the given fields."""
+               reliable_delta=None, precond_bits=None, simp=False):
+    """nbx_solve_opts_t with the library's defaults (the SIMP loop's with
+    simp=True), then the given fields."""
     o = SolveOpts()
-    load().nbx_solve_opts_default(C.byref(o))
+    if simp:
+        load().nbx_simp_opts_default(C.byref(o))
+    else:
+        load().nbx_solve_opts_default(C.byref(o))
     o.mixed_variant = variant
     if cheb_degree is not None:
         o.cheb_degree = cheb_degree

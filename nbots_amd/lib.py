@@ -162,6 +162,7 @@ SIGS = {
     "nbx_sparse_solve_ex": (I32, [VP, f64p, f64p, I32, F64, U32, C.POINTER(SolveOpts),
                                   C.POINTER(SolveInfo)]),
     "nbx_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+    "nbx_simp_opts_default": (None, [C.POINTER(SolveOpts)]),
     "nbx_sparse_spmv_dev": (I32, [VP, VP, VP]),
     "nbx_sparse_is_blocked": (I32, [VP]),
     "nbx_sparse_export_csr": (I32, [VP, u32o, u32o, f64o]),
