@@ -45,7 +45,8 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& 
 // Defaults of the SIMP loop's mixed solves: its systems carry void elements
 // (E = Emin = 1e-9 E0), and a wider, higher-degree Chebyshev interval pays
 // there (1.05M-DOF MBB beam, 25 iterations: 259 ms per iteration against
-// 283 ms with the plate defaults of degree 16 on [lmax/3000, lmax]).
+// 283 ms with degree 16 on [lmax/3000, lmax] and 267 ms with degree 24 on
+// [lmax/1e4, lmax]).
 inline MixedOpts simp_mixed_defaults() {
     MixedOpts o;
     o.cheb_degree = 32;
