@@ -49,6 +49,29 @@ def test_symmetric_layout_bit_exact(case):
     A.destroy()
 
 
+def test_symmetric_layout_spmv_medium():
+    """256x128 cantilever with lognormal moduli (33k nodes, 129 workgroups):
+    the drop-in SpMV on the full layout (before any solve) and on the
+    symmetric layout (after a solve found K symmetric) agree bit for bit, and
+    both equal the oracle's row sums (the reference's order)."""
+    import oracle
+    import nbots_amd as nb
+    from nbots_amd import synth
+    p = synth.cantilever_traction(256, 128)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    x = np.random.default_rng(3).standard_normal(A.N)
+    y_full = A.matvec(x)
+    _, info = A.solve(F, mode=1, tol=1e-6)
+    assert info["layout"] == 1
+    y_sym = A.matvec(x)
+    assert np.array_equal(y_full, y_sym)
+    assert np.array_equal(y_sym, oracle.spmv(rp, col, val, x))
+    A.destroy()
+
+
 def test_symmetric_layout_refused_for_asymmetric_values():
     """One off-diagonal entry changed on one side: the symmetry check fails,
     the solve runs on the full layout (layout 0), SpMV stays exact, and after
