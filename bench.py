@@ -117,14 +117,15 @@ def simp_bench(args, opts):
     run with mixed-precision solves and again with FP64 solves (both to
     relative residual 1e-8, warm-started from the previous displacement)."""
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED
+    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED, solve_opts
     nx, ny = 1024, 512
     p = synth.mbb(nx, ny, rho=0.5)
     out = dict(simp_workload=f"{nx}x{ny} MBB half-beam, {p.N} DOF, {p.N_elem} T3, volfrac 0.5, "
                              "penal 3, filter radius 1.5, OC move 0.2, rho0 = 0.5",
                simp_iterations=args.simp_iters)
     runs = {}
-    for name, mode, o in (("mixed", SOLVE_MIXED, opts), ("fp64", SOLVE_FP64, None)):
+    o64 = solve_opts(true_residual=True, simp=True)
+    for name, mode, o in (("mixed", SOLVE_MIXED, opts), ("fp64", SOLVE_FP64, o64)):
         f = Fem.from_problem(p)
         t0 = time.perf_counter()
         s = Simp(f, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2)
@@ -211,14 +212,18 @@ def main():
 
     # Steps: FP64 Jacobi-PCG solves to 1e-8 (the reference's algorithm,
     # device resident); value = DOF * CG iterations / s over the timed steps.
+    # both solvers stop on the true residual ||F - K u|| / ||F|| <= 1e-8
+    # (restarting from the iterate when their recursively updated residual
+    # has drifted from it)
+    o64 = solve_opts(true_residual=True)
     for w in range(args.warmup):
-        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000)
+        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000, opts=o64)
         log(f"[bench] warmup {w}: FP64 PCG {info['iters']} it, {info['ms']/1e3:.3f}s")
     times, iters, infos = [], [], []
     t_wall0 = time.perf_counter()
     for s in range(args.steps):
         t0 = time.perf_counter()
-        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000)
+        info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000, opts=o64)
         dt = time.perf_counter() - t0
         times.append(dt)
         iters.append(info["iters"])
@@ -228,6 +233,9 @@ def main():
             f"spmv {info['ms_spmv']*1e3:.1f}us")
     t_wall = time.perf_counter() - t_wall0
     u64 = f.displacement()
+    # ||F - K u|| / ||F|| of the solution, evaluated afresh: the solvers stop
+    # on their recursively updated residual, as the reference does
+    true64 = (f.true_residual(True), f.true_residual(False))
     value = N * sum(iters) / t_wall
     ms_step = 1e3 * t_wall / args.steps
 
@@ -252,7 +260,7 @@ def main():
         variant = dict(cheb=MIXED_CHEB_FCG, restart=MIXED_CG_RESTART,
                        reliable=MIXED_CG_RELIABLE)[args.mixed]
         opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
-                          precond_bits=args.precond_bits)
+                          precond_bits=args.precond_bits, true_residual=True)
         f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
         mtimes, minfos = [], []
         for s in range(2):
@@ -263,6 +271,7 @@ def main():
             log(f"[bench] mixed ({args.mixed}) {s}: {info['iters']} it, {info['outer']} inner, "
                 f"{mtimes[-1]:.3f}s, rel_res {info['rel_res']:.2e}, fallback {info['fallback']}")
         u32 = f.displacement()
+        true32 = (f.true_residual(True), f.true_residual(False))
         Fl = f.load_vector()
         c64, c32 = float(Fl @ u64), float(Fl @ u32)
         t_mixed = float(np.median(mtimes))
@@ -285,6 +294,7 @@ def main():
                           "FP32 CG with FP64 reliable updates"),
             mixed_iters=minfos[-1]["iters"], mixed_inner=minfos[-1]["outer"],
             mixed_rel_res=minfos[-1]["rel_res"], mixed_fallback=minfos[-1]["fallback"],
+            mixed_true_rel_res=true32[0], mixed_true_rel_res_fp64_eval=true32[1],
             mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
             mixed_vs_fp64_compliance_rel=abs(c32 - c64) / abs(c64),
             spmv64_us=round(ms64 * 1e3, 2),
@@ -311,7 +321,8 @@ def main():
     if not args.no_simp:
         simp = simp_bench(args, solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree,
                                            cheb_ratio=args.cheb_ratio,
-                                           precond_bits=args.precond_bits, simp=True))
+                                           precond_bits=args.precond_bits, simp=True,
+                                           true_residual=True))
     line = dict(
         metric=METRIC, value=value, unit="DOF*CG-iter/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True,
@@ -319,7 +330,8 @@ def main():
         config=dict(workload=cfg["name"], dof=N, elements=p.N_elem, nnz=nnz,
                     stored_blocks=nblk,
                     solver="FP64 Jacobi-PCG from u = 0 to ||F-Ku||/||F|| <= 1e-8",
-                    iters_per_step=iters, rel_res=[i["rel_res"] for i in infos]),
+                    iters_per_step=iters, rel_res=[i["rel_res"] for i in infos],
+                    true_rel_res=true64[0], true_rel_res_fp64_eval=true64[1]),
         roofline=roofline, cpu_baseline=cpu, setup_s=dict(pattern=round(t_pattern, 3),
                                                           assembly_bc=round(t_assemble, 3)),
         **extra, **simp)

@@ -922,6 +922,51 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(Sell2 K, const double2* __
     put_partials<1>(acc, partials);
 }
 
+// r = F - K u per row with compensated arithmetic (Dot2 of Ogita, Rump and
+// Oishi 2005: each product's rounding error from an FMA, each sum's from
+// TwoSum, the errors summed alongside), so r is accurate to about one
+// rounding of r itself instead of eps * sum |K_ij u_j|; partials of r.r.
+__device__ __forceinline__ void two_sum64(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dot2_add(double a, double x, double& s, double& c) {
+    const double p = a * x;
+    const double pe = fma(a, x, -p);
+    double t, e;
+    two_sum64(s, p, t, e);
+    s = t;
+    c = c + (e + pe);
+}
+__global__ __launch_bounds__(BLOCK) void k_residual_dot2(Sell2 K, const double2* __restrict__ F,
+                                                         const double2* __restrict__ u,
+                                                         double* partials, uint32_t n) {
+    double acc[1] = {0.0};
+    const Chunk c = my_chunk(n);
+    const dv2* __restrict__ pa = reinterpret_cast<const dv2*>(K.val);
+    const dv2* __restrict__ pb = reinterpret_cast<const dv2*>(K.val + 2 * K.n_blocks);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 f = F[i];
+        double s0 = -f.x, c0 = 0.0, s1 = -f.y, c1 = 0.0;
+        size_t idx = (size_t)K.slice_ptr[i >> 6] + (i & 63);
+        const int len = K.rowlen[i];
+        for (int k = 0; k < len; ++k, idx += SLICE) {
+            const double2 x = u[K.bcol[idx]];
+            const dv2 a = pa[idx], b = pb[idx];
+            dot2_add(a.x, x.x, s0, c0);
+            dot2_add(a.y, x.y, s0, c0);
+            dot2_add(b.x, x.x, s1, c1);
+            dot2_add(b.y, x.y, s1, c1);
+        }
+        const double r0 = s0 + c0, r1 = s1 + c1;   // K u - F
+        acc[0] += r0 * r0 + r1 * r1;
+    }
+    __shared__ double red[1][BLOCK / WAVE];
+    block_sum<1>(acc, red);
+    put_partials<1>(acc, partials);
+}
+
 __global__ void k_add_f64(double* __restrict__ x, const double* __restrict__ dx, size_t n) {
     const size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) x[i] += dx[i];
@@ -1004,6 +1049,21 @@ int fem_residual(FemSys& F, double* res, double* norm) {
     double* out = F.ws.v64[5] + F.ws.n;
     hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, F.s, F.K.s, (const double2*)F.F,
                        (const double2*)F.u, (double2*)res, F.ws.partials, nu);
+    hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(FIN), 0, F.s, F.ws.partials, G, out);
+    NBX_HIP(hipGetLastError());
+    double h = 0.0;
+    NBX_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, F.s));
+    NBX_HIP(hipStreamSynchronize(F.s));
+    *norm = sqrt(h);
+    return NBX_OK;
+}
+
+int fem_residual_dot2(FemSys& F, double* norm) {
+    const uint32_t nu = F.K.s.n_nodes;
+    const unsigned G = red_grid(nu);
+    double* out = F.ws.v64[5] + F.ws.n;
+    hipLaunchKernelGGL(k_residual_dot2, dim3(G), dim3(BLOCK), 0, F.s, F.K.s, (const double2*)F.F,
+                       (const double2*)F.u, F.ws.partials, nu);
     hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(FIN), 0, F.s, F.ws.partials, G, out);
     NBX_HIP(hipGetLastError());
     double h = 0.0;

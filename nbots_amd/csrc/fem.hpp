@@ -71,6 +71,8 @@ int fem_elem_cross(FemSys& F, const double* a, const double* b, double* out);
 // quasi-static damage (static_damage2D.c): res = F - K u and ||res||_2;
 // x += dx; strain, damage state r and damage of every element from u
 int fem_residual(FemSys& F, double* res, double* norm);
+// ||F - K u|| with each row evaluated in compensated (Dot2) arithmetic
+int fem_residual_dot2(FemSys& F, double* norm);
 int fem_add(FemSys& F, double* x, const double* dx);
 int fem_damage_update(FemSys& F, double E, double nu, double Gf, double ft, const double* r_prev,
                       double* r, double* dmg, double* strain);

@@ -221,7 +221,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
         info->fp64_iters = it;
         info->tol_reached = tr;
     } else if (mode == NBX_SOLVE_FP64) {
-        status = nbx::solve_fp64(S.K, S.F, S.u, tol, max_iter, S.ws, S.s, &st);
+        status = nbx::solve_fp64(S.K, S.F, S.u, tol, mo.true_residual, max_iter, S.ws, S.s, &st);
         if (status > 1) return status;
         info->iters = st.iters;
         info->fp64_iters = st.fp64_iters;
@@ -343,7 +343,7 @@ static void opts_from(const nbx::MixedOpts& d, nbx_solve_opts_t* o) {
     o->inner_rtol = d.inner_rtol;
     o->reliable_delta = d.delta;
     o->precond_bits = d.precond_bits;
-    o->reserved = 0;
+    o->true_residual = d.true_residual ? 1 : 0;
 }
 
 void nbx_solve_opts_default(nbx_solve_opts_t* o) { opts_from(nbx::MixedOpts(), o); }
@@ -364,6 +364,21 @@ int nbx_fem_solve_ex(nbx_fem_t* f, int mode, double tol, uint32_t max_iter,
 
 int nbx_fem_get_displacement(const nbx_fem_t* f, double* u) {
     NBX_HIP(nbx::copy_on(u, f->sys.u, 2 * (size_t)f->sys.nn * 8, hipMemcpyDeviceToHost, f->sys.s));
+    return nbx::NBX_OK;
+}
+
+int nbx_fem_true_residual(nbx_fem_t* f, int compensated, double* rel_res) {
+    nbx::FemSys& S = f->sys;
+    NBX_TRY(nbx::ws_alloc(S.ws, 2 * S.nn));
+    double r = 0.0, fn = 0.0;
+    if (compensated)
+        NBX_TRY(nbx::fem_residual_dot2(S, &r));
+    else
+        NBX_TRY(nbx::fem_residual(S, S.ws.v64[4], &r));
+    int rc = nbx::NBX_OK;
+    fn = sqrt(nbx::device_norm2(2 * S.nn, S.F, S.ws, S.s, &rc));
+    if (rc) return rc;
+    *rel_res = fn > 0.0 ? r / fn : r;
     return nbx::NBX_OK;
 }
 

@@ -22,6 +22,7 @@ struct PcgState {
     double maxrr;                  // largest r.r since the last reliable update
     double conv_rr;                // r.r at which the true residual should meet tol
     uint32_t updates, pad;
+    double gg0;                    // squared residual norm at the start (fresh b - A x)
 };
 
 // A device matrix: 2-dof blocked sliced ELL when the pattern allows it,
@@ -73,6 +74,7 @@ struct SolveStats {
     double ms_spmv_avg = 0.0;  // average duration of the dominant SpMV kernel
     int fallback = 0;          // 1 if the FP32 sweeps stalled and FP64 PCG finished
     int layout = 0;            // FP64 matrix layout used: 0 full, 1 symmetric
+    double res0 = 0.0;         // ||b - A x|| at the start, computed afresh
 };
 
 // Mixed-precision settings.
@@ -88,6 +90,7 @@ struct MixedOpts {
     uint32_t cheb_degree = 20;  // SpMVs per preconditioner application
     double cheb_ratio = 1e4;    // Chebyshev interval [lambda_max/ratio, lambda_max]
     int precond_bits = 16;     // Chebyshev preconditioner matrix: fp16 or FP32 values
+    bool true_residual = false; // stop on the true residual (restarts), FP64 and CHEB_FCG
 };
 
 int ws_alloc(Workspace& w, uint32_t n);
@@ -126,7 +129,7 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol,
                 SolveStats* stats);
 
 // FP64 Jacobi PCG with a relative tolerance (x0 = x on input).
-int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol,
+int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol, bool true_residual,
                uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats);
 
 int spmv_bench(const DevMatrix& m, int prec, uint32_t reps, Workspace& w, hipStream_t s,

@@ -532,7 +532,8 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
         info->fp64_iters = it;
         info->tol_reached = tr;
     } else if (mode == NBX_SOLVE_FP64) {
-        status = nbx::solve_fp64(A->dev, db, dx, tol, max_iter, A->ws, A->stream, &st);
+        status = nbx::solve_fp64(A->dev, db, dx, tol, mo.true_residual, max_iter, A->ws, A->stream,
+                                 &st);
         if (status > 1) return status;
         info->iters = st.iters;
         info->fp64_iters = st.fp64_iters;

@@ -13,7 +13,7 @@ MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
 
 
 def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,
-               reliable_delta=None, precond_bits=None, simp=False):
+               reliable_delta=None, precond_bits=None, simp=False, true_residual=None):
     """nbx_solve_opts_t with the library's defaults (the SIMP loop's with
     simp=True), then the given fields."""
     o = SolveOpts()
@@ -32,6 +32,8 @@ def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_
         o.reliable_delta = reliable_delta
     if precond_bits is not None:
         o.precond_bits = precond_bits
+    if true_residual is not None:
+        o.true_residual = int(bool(true_residual))
     return o
 
 
@@ -119,6 +121,14 @@ class Fem:
         u = np.zeros(self.N)
         check(self.L.nbx_fem_get_displacement(self.h, u), "nbx_fem_get_displacement")
         return u
+
+    def true_residual(self, compensated=True):
+        """||F - K u|| / ||F|| of the current displacement, on the device:
+        each row in compensated (Dot2) arithmetic, or plain FP64."""
+        r = C.c_double()
+        check(self.L.nbx_fem_true_residual(self.h, int(compensated), C.byref(r)),
+              "nbx_fem_true_residual")
+        return r.value
 
     def system(self):
         """K (scalar CSR, the reference's layout) and F."""

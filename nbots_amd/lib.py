@@ -47,7 +47,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("mixed_variant", C.c_int32), ("cheb_degree", C.c_uint32),
                 ("cheb_ratio", C.c_double), ("inner_rtol", C.c_double),
                 ("reliable_delta", C.c_double), ("precond_bits", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("true_residual", C.c_int32)]
 
 
 class Graph(C.Structure):
@@ -177,6 +177,7 @@ SIGS = {
     "nbx_fem_solve_ex": (I32, [VP, I32, F64, U32, C.POINTER(SolveOpts), f64o,
                                C.POINTER(SolveInfo)]),
     "nbx_fem_get_displacement": (I32, [VP, f64p]),
+    "nbx_fem_true_residual": (I32, [VP, I32, C.POINTER(F64)]),
     "nbx_fem_get_system": (I32, [VP, u32o, u32o, f64o, f64o]),
     "nbx_fem_strain_stress": (I32, [VP, f64p, f64p]),
     "nbx_fem_compliance": (I32, [VP, f64o, F64, F64, F64, C.POINTER(F64), f64o, f64o]),

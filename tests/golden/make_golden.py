@@ -234,33 +234,37 @@ def damage_cases():
     print("damage log:", sum(1 for _ in open(log)), "lines")
 
 
-def c2_case():
-    """The reference's FP64 Jacobi-PCG on the 1024x512 lognormal cantilever
-    (1,051,650 DOF), K and F from the reference's own assembly and boundary
-    conditions, tolerance 1e-8 ||F|| (8 OpenMP threads; the iteration is the
-    same arithmetic as one thread except for the dot-product order).  Kept:
-    every 101st displacement, ||u||, the iteration count, the residual."""
+def c2_case(nx=1024, ny=512, name="c2_reference_solve", step=101):
+    """The reference's FP64 Jacobi-PCG on the nx x ny lognormal cantilever
+    (1024x512: 1,051,650 DOF; 2048x1024: 4,200,450 DOF), K and F from the
+    reference's own assembly and boundary conditions, tolerance 1e-8 ||F||
+    (8 OpenMP threads; the iteration is the same arithmetic as one thread
+    except for the dot-product order).  Kept: every step-th displacement,
+    ||u||, the iteration count, the residual, SHA-256 of K and F."""
     import time
-    p = synth.cantilever_traction(1024, 512)
+    p = synth.cantilever_traction(nx, ny)
     t = time.time()
     rp, col, val, F, st = oracle.ref_assemble(p, apply_bc=True)
     assert st == 0
     tol = 1e-8 * float(np.linalg.norm(F))
     x, it, tr, cst = oracle.ref_pcg(rp, col, val, F, tol=tol, max_iter=p.N, threads=8)
     r = F - oracle.ref_spmv(rp, col, val, x, threads=8)
-    idx = np.arange(0, p.N, 101)
-    np.savez_compressed(os.path.join(OUT, "c2_reference_solve.npz"), idx=idx, u=x[idx],
+    idx = np.arange(0, p.N, step)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), idx=idx, u=x[idx],
                         u_norm=np.linalg.norm(x), niter=it, status=cst, tol=tol,
                         tol_reached=tr, rel_res=np.linalg.norm(r) / np.linalg.norm(F),
                         F_norm=np.linalg.norm(F), K_sha=sha(val), F_sha=sha(F))
-    print(f"c2: {p.N} dof, niter {it}, status {cst}, rel res {np.linalg.norm(r) / np.linalg.norm(F):.3e},"
-          f" {time.time() - t:.0f}s")
+    print(f"{name}: {p.N} dof, niter {it}, status {cst}, "
+          f"rel res {np.linalg.norm(r) / np.linalg.norm(F):.3e}, {time.time() - t:.0f}s")
 
 
 def main(argv):
     oracle.build(ref=True)
     if argv[1:] == ["c2"]:
         c2_case()
+        return
+    if argv[1:] == ["c3"]:
+        c2_case(2048, 1024, "c3_reference_solve", step=401)
         return
     if argv[1:] == ["cg"]:
         cg_cases()
@@ -290,6 +294,7 @@ def main(argv):
     file_cases()
     damage_cases()
     c2_case()
+    c2_case(2048, 1024, "c3_reference_solve", step=401)
 
 
 if __name__ == "__main__":

@@ -10,6 +10,7 @@ Bars (see the module docstrings of the kernels for why they hold):
     (utest/.../static_elasticity2D.c:118).
 """
 import hashlib
+import math
 
 import numpy as np
 import pytest
@@ -158,6 +159,55 @@ def test_solve_modes_traction(mode, variant, bits):
     f.destroy()
 
 
+@pytest.mark.parametrize("tol", [1e-6, 1e-12])
+def test_true_residual_compensated_is_exact(tol):
+    """nbx_fem_true_residual: the compensated (Dot2) evaluation of
+    ||F - K u|| / ||F|| equals the exact rational value for the device's
+    displacement within 1e-12 relative; the plain FP64 evaluation is within
+    its own rounding, about 16 eps * || |K| |u| + |F| || / ||F||."""
+    from fractions import Fraction
+    from nbots_amd.fem import Fem, SOLVE_FP64
+    g = load_golden("traction_16x8")
+    p = problem_from_golden(g)
+    f = Fem.from_problem(p)
+    f.solve(SOLVE_FP64, tol=tol)
+    u = f.displacement()
+    rp, col, val, F = f.system()
+    r2, f2, absku = Fraction(0), Fraction(0), 0.0
+    for i in range(F.size):
+        ks = range(rp[i], rp[i + 1])
+        s = Fraction(F[i]) - sum(Fraction(val[k]) * Fraction(u[col[k]]) for k in ks)
+        r2 += s * s
+        f2 += Fraction(F[i]) ** 2
+        absku += (sum(abs(val[k] * u[col[k]]) for k in ks) + abs(F[i])) ** 2
+    exact = math.sqrt(r2 / f2)
+    assert abs(f.true_residual(True) - exact) <= 1e-12 * exact
+    floor = np.finfo(float).eps * math.sqrt(absku / float(f2)) * 16
+    assert abs(f.true_residual(False) - exact) <= floor
+    f.destroy()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_true_residual_stop(mode):
+    """true_residual = 1: FP64 PCG and the Chebyshev flexible CG return only
+    when ||F - K u|| / ||F||, computed afresh, meets the tolerance; with an
+    unattainable tolerance they give up with NBX_NOT_CONVERGED once a
+    restart stops gaining, after at most 8 passes."""
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, solve_opts
+    p = synth.cantilever_traction(256, 128)
+    f = Fem.from_problem(p)
+    o = solve_opts(true_residual=True)
+    info = f.solve(mode, tol=1e-8, opts=o, u0=np.zeros(p.N))
+    assert info["status"] == 0
+    assert f.true_residual(True) <= 1e-8
+    assert abs(info["rel_res"] - f.true_residual(False)) <= 1e-3 * info["rel_res"]
+    info = f.solve(mode, tol=1e-17, opts=o)
+    assert info["status"] == 1
+    assert f.true_residual(True) <= 1e-8
+    f.destroy()
+
+
 def test_compliance_sensitivities_small():
     """SIMP on the MBB beam: element energies bit-exact with the oracle on
     the same displacement; C = F.u (energy identity); dC/drho against
@@ -227,23 +277,27 @@ def test_spmv_bench_entry():
     f.destroy()
 
 
-def test_c2_fp64_against_reference_solve():
-    """1024x512 lognormal cantilever, right-edge shear traction (1,051,650
-    DOF): the device's K and F equal the compiled reference's (SHA-256), and
-    its FP64 Jacobi-PCG with the reference's stop test (tol 1e-8 ||F||)
-    takes the reference's iteration count within 0.5 % (11,733; the dot
+@pytest.mark.parametrize("name,nx,ny", [("c2_reference_solve", 1024, 512),
+                                        ("c3_reference_solve", 2048, 1024)])
+def test_fp64_against_reference_solve(name, nx, ny):
+    """Lognormal cantilever, right-edge shear traction, 1024x512 (1,051,650
+    DOF) and 2048x1024 (4,200,450 DOF, the bench workload): the device's K
+    and F equal the compiled reference's (SHA-256), and its FP64 Jacobi-PCG
+    with the reference's stop test (tol 1e-8 ||F||), on the symmetric
+    layout, takes the reference's iteration count within 0.5 % (the dot
     products are summed in another order) and reaches the reference's
-    displacement within 1e-7 (every 101st dof and ||u||)."""
+    displacement within 1e-7 (sampled dofs and ||u||)."""
     from nbots_amd import synth
     from nbots_amd.fem import Fem, SOLVE_REFERENCE
-    g = load_golden("c2_reference_solve")
-    p = synth.cantilever_traction(1024, 512)
+    g = load_golden(name)
+    p = synth.cantilever_traction(nx, ny)
     f = Fem.from_problem(p)
     rp, col, val, F = f.system()
     assert sha(val) == str(g["K_sha"]) and sha(F) == str(g["F_sha"])
     del rp, col, val
     info = f.solve(SOLVE_REFERENCE, tol=float(g["tol"]), max_iter=p.N)
     assert info["status"] == int(g["status"]) == 0
+    assert info["layout"] == 1
     n_ref = int(g["niter"])
     assert abs(info["iters"] - n_ref) <= 0.005 * n_ref, (info["iters"], n_ref)
     u = f.displacement()
