@@ -72,6 +72,35 @@ def test_symmetric_layout_spmv_medium():
     A.destroy()
 
 
+@pytest.mark.parametrize("n_nod", [1, 2, 65])
+def test_symmetric_layout_tiny(n_nod):
+    """Chains of 1, 2 and 65 nodes (one and two slices; a single node has no
+    lower block at all): the symmetric layout is built, solves match a dense
+    solve and the SpMV stays exact."""
+    import oracle
+    import nbots_amd as nb
+    adj = [[w for w in (v - 1, v + 1) if 0 <= w < n_nod] for v in range(n_nod)]
+    A = nb.Sparse.from_graph([len(a) for a in adj], adj, 2)
+    rp, col, _ = A.export()
+    rng = np.random.default_rng(n_nod)
+    dense = np.zeros((A.N, A.N))
+    for i in range(A.N):
+        for k in range(rp[i], rp[i + 1]):
+            j = col[k]
+            if j <= i:
+                dense[i, j] = dense[j, i] = rng.uniform(-1, 1) if j < i else 0.0
+    dense += np.diag(np.abs(dense).sum(1) + 1.0)
+    val = np.array([dense[i, col[k]] for i in range(A.N) for k in range(rp[i], rp[i + 1])])
+    A.import_values(val)
+    b = rng.standard_normal(A.N)
+    x, info = A.solve(b, mode=1, tol=1e-12)
+    assert info["layout"] == 1
+    assert np.linalg.norm(x - np.linalg.solve(dense, b)) <= 1e-10 * np.linalg.norm(x)
+    xs = rng.standard_normal(A.N)
+    assert np.array_equal(A.matvec(xs), oracle.spmv(rp, col, val, xs))
+    A.destroy()
+
+
 def test_symmetric_layout_refused_for_asymmetric_values():
     """One off-diagonal entry changed on one side: the symmetry check fails,
     the solve runs on the full layout (layout 0), SpMV stays exact, and after
