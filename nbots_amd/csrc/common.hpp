@@ -48,8 +48,8 @@ constexpr int WAVE = 64;          // CDNA wavefront
 constexpr int BLOCK = 256;        // 4 waves per workgroup
 constexpr int SLICE = 64;         // sliced-ELL slice height = one wavefront
 constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, 32 CUs each
-constexpr int MAX_GRID = 65535;
-constexpr size_t INFINITY_CACHE_BYTES = 256ull << 20;   // MI355X last-level cache   // workgroups of a reduction kernel (partials)
+constexpr int MAX_GRID = 65535;   // workgroups of a reduction kernel (partials)
+constexpr size_t INFINITY_CACHE_BYTES = 256ull << 20;   // MI355X last-level cache
 
 // A copy ordered on a handle's stream, complete on return (a plain hipMemcpy
 // would also wait for every other handle's work).

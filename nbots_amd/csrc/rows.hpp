@@ -241,6 +241,7 @@ __device__ __forceinline__ float2 row32(const Sell2& A, uint32_t v, const float2
 
 // fp16 copy (Jacobi-scaled values widened to FP32, float2 vectors): 12 B
 // per block instead of 20 B.
+template <bool NT = true>
 __device__ __forceinline__ float2 row16(const Sell2& A, uint32_t v, const float2* __restrict__ x) {
     const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const int len = A.rowlen[v];
@@ -254,8 +255,13 @@ __device__ __forceinline__ float2 row16(const Sell2& A, uint32_t v, const float2
         for (int k = 0; k < UNROLL; ++k) {
             if (k0 + k < len) {
                 const size_t i = base + (size_t)(k0 + k) * SLICE;
-                w[k] = nt(&A.bcol[i]);
-                a[k] = nt(&val[i]);
+                if constexpr (NT) {
+                    w[k] = nt(&A.bcol[i]);
+                    a[k] = nt(&val[i]);
+                } else {
+                    w[k] = A.bcol[i];
+                    a[k] = val[i];
+                }
             }
         }
 #pragma unroll
