@@ -308,8 +308,8 @@ enum {
 };
 typedef struct {
 	int32_t mixed_variant;   /* NBX_MIXED_*                               */
-	uint32_t cheb_degree;    /* SpMVs per preconditioner application (20) */
-	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (1e4) */
+	uint32_t cheb_degree;    /* SpMVs per preconditioner application (30) */
+	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (3e4) */
 	double inner_rtol;       /* CG_RESTART inner tolerance (1e-4)         */
 	double reliable_delta;   /* CG_RELIABLE update threshold (0.1)        */
 	int32_t precond_bits;    /* CHEB_FCG matrix values: 16 (default) or 32 */
