@@ -61,7 +61,8 @@ struct Workspace {
     PcgState* st = nullptr;    // device state (2 slots: outer, inner)
     PcgState* h_st = nullptr;  // pinned host mirror
     hipEvent_t ev[4] = {};
-    static constexpr int NSAMP = 8;     // sampled SpMV launches per solve
+    static constexpr int NSAMP = 32;    // sampled SpMV launches per solve, one every
+    static constexpr int SAMPLE_EVERY = 64;   // SAMPLE_EVERY iterations (PCG)
     hipEvent_t sev[2 * NSAMP] = {};
 };
 

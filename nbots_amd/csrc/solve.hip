@@ -547,7 +547,8 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
     uint32_t launched = 0, batch = 4, nsamp = 0;
     for (;;) {
         for (uint32_t i = 0; i < batch; ++i) {
-            const bool samp = launched == 0 && nsamp < (uint32_t)Workspace::NSAMP;
+            const bool samp = (launched + i) % Workspace::SAMPLE_EVERY == 0 &&
+                              nsamp < (uint32_t)Workspace::NSAMP;
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, p, wv, w.st, w.partials,
                                nu);
@@ -588,8 +589,9 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
         stats->iters = w.h_st->k;
         stats->fp64_iters = w.h_st->k;
         stats->res0 = sqrt(w.h_st->gg0);
-        // samples of iterations that did run (k > sample index)
-        NBX_TRY(sampled_ms(w, nsamp < w.h_st->k ? nsamp : w.h_st->k, &stats->ms_spmv_avg));
+        // samples of iterations that did run (iteration j * SAMPLE_EVERY < k)
+        const uint32_t ran = (w.h_st->k + Workspace::SAMPLE_EVERY - 1) / Workspace::SAMPLE_EVERY;
+        NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));
     }
     return NBX_OK;
 }
@@ -887,10 +889,10 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     NBX_HIP(hipGetLastError());
     NBX_TRY(poll_state(w, s));
     uint32_t batch = 4, nsamp = 0, launched = 0;
-    bool first_batch = true;
+    constexpr uint32_t every = 8;   // sample one outer iteration in eight
     while (!w.h_st->done) {
         for (uint32_t i = 0; i < batch; ++i) {
-            const bool samp = first_batch && nsamp < (uint32_t)Workspace::NSAMP;
+            const bool samp = (launched + i) % every == 0 && nsamp < (uint32_t)Workspace::NSAMP;
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             if (m.has_sym)
                 hipLaunchKernelGGL(k_pcg_A<OpSym>, dim3(G), dim3(BLOCK), 0, s, ops,
@@ -905,7 +907,6 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         NBX_HIP(hipGetLastError());
         launched += batch;
         NBX_TRY(poll_state(w, s));
-        first_batch = false;
         if (launched > max_iter + 8u && !w.h_st->done) {
             set_last_error("flexible CG did not report completion");
             return NBX_ERR_BREAKDOWN;
@@ -925,7 +926,8 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         stats->fallback = (int)w.h_st->breakdown;
         stats->layout = m.has_sym ? 1 : 0;
         stats->res0 = sqrt(w.h_st->gg0);
-        NBX_TRY(sampled_ms(w, nsamp < w.h_st->k ? nsamp : w.h_st->k, &stats->ms_spmv_avg));
+        const uint32_t ran = (w.h_st->k + every - 1) / every;
+        NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));
     }
     return w.h_st->status ? NBX_NOT_CONVERGED : NBX_OK;
 }
