@@ -138,11 +138,18 @@ class Sparse:
             raise RuntimeError(f"GPU solve failed: {last_error()}")
         return x, st, it.value, tr.value
 
-    def solve(self, b, mode=SOLVE_FP64, tol=1e-8, max_iter=None, x0=None, inner_rtol=0.0):
+    def solve(self, b, mode=SOLVE_FP64, tol=1e-8, max_iter=None, x0=None, inner_rtol=0.0,
+              opts=None):
+        """nbx_sparse_solve, or nbx_sparse_solve_ex when opts (a SolveOpts,
+        nbots_amd.fem.solve_opts(...)) is given."""
         x = np.zeros(self.N) if x0 is None else np.array(x0, np.float64)
         info = SolveInfo()
-        rc = self.L.nbx_sparse_solve(self.h, np.ascontiguousarray(b, np.float64), x, mode, tol,
-                                     10 * self.N if max_iter is None else max_iter,
-                                     inner_rtol, C.byref(info))
+        mi = 10 * self.N if max_iter is None else max_iter
+        bb = np.ascontiguousarray(b, np.float64)
+        if opts is not None:
+            rc = self.L.nbx_sparse_solve_ex(self.h, bb, x, mode, tol, mi, C.byref(opts),
+                                            C.byref(info))
+        else:
+            rc = self.L.nbx_sparse_solve(self.h, bb, x, mode, tol, mi, inner_rtol, C.byref(info))
         check(rc, "nbx_sparse_solve", ok=(0, 1))
         return x, info.as_dict()

@@ -101,6 +101,27 @@ def test_symmetric_layout_tiny(n_nod):
     A.destroy()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_sparse_true_residual_stop(mode):
+    """nbx_sparse_solve_ex with true_residual = 1 (FP64 PCG and the mixed
+    Chebyshev flexible CG): the returned x meets the tolerance on the true
+    residual, checked with the oracle's reference-order SpMV."""
+    import oracle
+    import nbots_amd as nb
+    from nbots_amd import synth
+    from nbots_amd.fem import solve_opts
+    p = synth.cantilever_traction(128, 64)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    x, info = A.solve(F, mode=mode, tol=1e-10, opts=solve_opts(true_residual=True))
+    assert info["status"] == 0 and info["layout"] == 1
+    r = F - oracle.spmv(rp, col, val, x)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(F)
+    A.destroy()
+
+
 def test_symmetric_layout_refused_for_asymmetric_values():
     """One off-diagonal entry changed on one side: the symmetry check fails,
     the solve runs on the full layout (layout 0), SpMV stays exact, and after
