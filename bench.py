@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
     ap.add_argument("--cheb-degree", type=int, default=None,
-                    help="library default 30 (SIMP loop: 32)")
+                    help="library default 40 (SIMP loop: 32)")
     ap.add_argument("--cheb-ratio", type=float, default=None,
                     help="library default 3e4 (SIMP loop: 3e4)")
     ap.add_argument("--precond-bits", type=int, default=16, choices=[16, 32])

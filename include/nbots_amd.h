@@ -308,7 +308,7 @@ enum {
 };
 typedef struct {
 	int32_t mixed_variant;   /* NBX_MIXED_*                               */
-	uint32_t cheb_degree;    /* SpMVs per preconditioner application (30) */
+	uint32_t cheb_degree;    /* SpMVs per preconditioner application (40) */
 	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (3e4) */
 	double inner_rtol;       /* CG_RESTART inner tolerance (1e-4)         */
 	double reliable_delta;   /* CG_RELIABLE update threshold (0.1)        */

@@ -88,7 +88,7 @@ struct MixedOpts {
     MixedVariant variant = CHEB_FCG;
     double inner_rtol = 1e-4;
     double delta = 0.1;        // reliable update when r.r < delta^2 max r.r
-    uint32_t cheb_degree = 30;  // SpMVs per preconditioner application
+    uint32_t cheb_degree = 40;  // SpMVs per preconditioner application
     double cheb_ratio = 3e4;    // Chebyshev interval [lambda_max/ratio, lambda_max]
     int precond_bits = 16;     // Chebyshev preconditioner matrix: fp16 or FP32 values
     bool true_residual = false; // stop on the true residual (restarts), FP64 and CHEB_FCG

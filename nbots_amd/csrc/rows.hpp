@@ -241,12 +241,49 @@ __device__ __forceinline__ float2 row32(const Sell2& A, uint32_t v, const float2
 
 // fp16 copy (Jacobi-scaled values widened to FP32, float2 vectors): 12 B
 // per block instead of 20 B.
-template <bool NT = true>
 __device__ __forceinline__ float2 row16(const Sell2& A, uint32_t v, const float2* __restrict__ x) {
     const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const int len = A.rowlen[v];
     const hv4* __restrict__ val = reinterpret_cast<const hv4*>(A.val16);
     float s0 = 0.f, s1 = 0.f;
+    for (int k0 = 0; k0 < len; k0 += UNROLL) {
+        uint32_t w[UNROLL];
+        hv4 a[UNROLL];
+        float2 xv[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (k0 + k < len) {
+                const size_t i = base + (size_t)(k0 + k) * SLICE;
+                w[k] = nt(&A.bcol[i]);
+                a[k] = nt(&val[i]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k)
+            if (k0 + k < len) xv[k] = x[w[k]];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (k0 + k < len) {
+                s0 = s0 + (float)a[k].x * xv[k].x;
+                s0 = s0 + (float)a[k].y * xv[k].y;
+                s1 = s1 + (float)a[k].z * xv[k].x;
+                s1 = s1 + (float)a[k].w * xv[k].y;
+            }
+        }
+    }
+    return make_float2(s0, s1);
+}
+
+// The same row with FP64 accumulation: the fp16 values and FP32 vector
+// entries are exact in FP64, so each product is exact and only the sum
+// rounds; stiffness rows cancel heavily, and an FP32 sum loses most of a
+// small result.
+template <bool NT = true>
+__device__ __forceinline__ double2 row16d(const Sell2& A, uint32_t v, const float2* __restrict__ x) {
+    const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    const hv4* __restrict__ val = reinterpret_cast<const hv4*>(A.val16);
+    double s0 = 0.0, s1 = 0.0;
     for (int k0 = 0; k0 < len; k0 += UNROLL) {
         uint32_t w[UNROLL];
         hv4 a[UNROLL];
@@ -270,14 +307,14 @@ __device__ __forceinline__ float2 row16(const Sell2& A, uint32_t v, const float2
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             if (k0 + k < len) {
-                s0 = s0 + (float)a[k].x * xv[k].x;
-                s0 = s0 + (float)a[k].y * xv[k].y;
-                s1 = s1 + (float)a[k].z * xv[k].x;
-                s1 = s1 + (float)a[k].w * xv[k].y;
+                s0 = s0 + (double)a[k].x * (double)xv[k].x;
+                s0 = s0 + (double)a[k].y * (double)xv[k].y;
+                s1 = s1 + (double)a[k].z * (double)xv[k].x;
+                s1 = s1 + (double)a[k].w * (double)xv[k].y;
             }
         }
     }
-    return make_float2(s0, s1);
+    return make_double2(s0, s1);
 }
 
 // y = A x with the scaled copy of BITS-bit values

@@ -423,7 +423,7 @@ __global__ __launch_bounds__(FIN) void k_fin_fcg_update(PcgState* st, const doub
 }
 
 // one Chebyshev step: res -= Kt d; d' = c1 d + c2 res; z += d'
-template <int BITS, bool NT = true>   // NT: non-temporal matrix loads
+template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __restrict__ din,
                                                      float2* __restrict__ dout,
                                                      float2* __restrict__ res,
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
     if (st->done) return;
     const Chunk c = my_chunk(n);
     for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
-        const float2 t = BITS == 16 ? row16<NT>(K, (uint32_t)i, din) : row32(K, (uint32_t)i, din);
+        const float2 t = row_scaled<BITS>(K, (uint32_t)i, din);
         float2 rs = res[i];
         rs.x = rs.x - t.x;
         rs.y = rs.y - t.y;
@@ -444,6 +444,33 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
         res[i] = rs;
         dout[i] = dn;
         z[i] = zv;
+    }
+}
+
+// The step on the fp16 copy, with the row sum and the recurrence in FP64
+// (vectors stored in FP32).  The FP32 sum loses most of the small result of
+// a cancelling stiffness row; with it the outer iteration count jumps up
+// and down with the degree (4.2M DOF, degree 28-40: 1006, 940, 1118, 996,
+// 711), with FP64 sums it falls steadily (1006, 943, 882, 787, 709) at the
+// same cost per step.  NT: non-temporal matrix loads.
+template <bool NT>
+__global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __restrict__ din,
+                                                       float2* __restrict__ dout,
+                                                       float2* __restrict__ res,
+                                                       float2* __restrict__ z, PcgState* st,
+                                                       uint32_t n, double c1, double c2) {
+    if (st->done) return;
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 t = row16d<NT>(K, (uint32_t)i, din);
+        const float2 r0 = res[i];
+        const double rx = (double)r0.x - t.x, ry = (double)r0.y - t.y;
+        const float2 di = din[i];
+        const double dx = c1 * (double)di.x + c2 * rx, dy = c1 * (double)di.y + c2 * ry;
+        const float2 zv = z[i];
+        res[i] = make_float2((float)rx, (float)ry);
+        dout[i] = make_float2((float)dx, (float)dy);
+        z[i] = make_float2((float)((double)zv.x + dx), (float)((double)zv.y + dy));
     }
 }
 
@@ -832,11 +859,14 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     const double lmax = m.lambda_max, lmin = lmax / opt.cheb_ratio;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     std::vector<float> c1(deg), c2(deg);
+    std::vector<double> c1d(deg), c2d(deg);
     double rho = 1.0 / sigma;
     for (uint32_t k = 1; k < deg; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
-        c1[k] = (float)(rn * rho);
-        c2[k] = (float)(2.0 * rn / delta);
+        c1d[k] = rn * rho;
+        c2d[k] = 2.0 * rn / delta;
+        c1[k] = (float)c1d[k];
+        c2[k] = (float)c2d[k];
         rho = rn;
     }
     const float inv_theta = (float)(1.0 / theta);
@@ -867,11 +897,11 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         float2* dout = db;
         for (uint32_t k = 1; k < deg; ++k) {
             if (opt.precond_bits == 16 && cheb_cached)
-                hipLaunchKernelGGL((k_cheb_step<16, false>), dim3(G), dim3(BLOCK), 0, s, m.s, din, dout,
-                                   res, z, w.st, nu, c1[k], c2[k]);
+                hipLaunchKernelGGL(k_cheb_step64<false>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout,
+                                   res, z, w.st, nu, c1d[k], c2d[k]);
             else if (opt.precond_bits == 16)
-                hipLaunchKernelGGL(k_cheb_step<16>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
-                                   w.st, nu, c1[k], c2[k]);
+                hipLaunchKernelGGL(k_cheb_step64<true>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout,
+                                   res, z, w.st, nu, c1d[k], c2d[k]);
             else
                 hipLaunchKernelGGL(k_cheb_step<32>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
                                    w.st, nu, c1[k], c2[k]);

@@ -59,7 +59,7 @@ def test_solve_options_checked_before_device_use():
 def test_solve_options_defaults():
     from nbots_amd.fem import solve_opts, MIXED_CHEB_FCG
     o = solve_opts()
-    assert (o.mixed_variant, o.cheb_degree, o.cheb_ratio, o.precond_bits) == (MIXED_CHEB_FCG, 30,
+    assert (o.mixed_variant, o.cheb_degree, o.cheb_ratio, o.precond_bits) == (MIXED_CHEB_FCG, 40,
                                                                                3e4, 16)
 
 
