@@ -175,9 +175,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
     ap.add_argument("--cheb-degree", type=int, default=None,
-                    help="library default 40 (SIMP loop: 32)")
+                    help="library default 40")
     ap.add_argument("--cheb-ratio", type=float, default=None,
-                    help="library default 3e4 (SIMP loop: 3e4)")
+                    help="library default 3e4")
     ap.add_argument("--precond-bits", type=int, default=16, choices=[16, 32])
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--seed", type=int, default=20261202)

@@ -320,8 +320,9 @@ typedef struct {
 } nbx_solve_opts_t;
 /* Fills *opts with the defaults above. */
 void nbx_solve_opts_default(nbx_solve_opts_t *opts);
-/* Fills *opts with the SIMP loop's defaults (nbx_simp_iterate): the same,
- * except the Chebyshev interval, degree 32 on [lambda_max/3e4, lambda_max]. */
+/* Fills *opts with the SIMP loop's defaults (nbx_simp_iterate); at present
+ * the same as nbx_solve_opts_default (degree 40 on [lambda_max/3e4,
+ * lambda_max] is the best point swept for both).                        */
 void nbx_simp_opts_default(nbx_solve_opts_t *opts);
 
 typedef struct {

@@ -43,14 +43,14 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& 
     return NBX_OK;
 }
 
-// Defaults of the SIMP loop's mixed solves: its systems carry void elements
-// (E = Emin = 1e-9 E0), and a wider, higher-degree Chebyshev interval pays
-// there (1.05M-DOF MBB beam, 25 iterations: 259 ms per iteration against
-// 283 ms with degree 16 on [lmax/3000, lmax] and 267 ms with degree 24 on
-// [lmax/1e4, lmax]).
+// Defaults of the SIMP loop's mixed solves.  Its systems carry void
+// elements (E = Emin = 1e-9 E0); the plate defaults (degree 40 on
+// [lmax/3e4, lmax]) are also the best point swept there (1.05M-DOF MBB
+// beam, 25 iterations: 261 ms per iteration; degree 32: 265, degree 48 on
+// [lmax/1e5, lmax]: 263, degree 24: 271).
 inline MixedOpts simp_mixed_defaults() {
     MixedOpts o;
-    o.cheb_degree = 32;
+    o.cheb_degree = 40;
     o.cheb_ratio = 3e4;
     return o;
 }
