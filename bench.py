@@ -70,8 +70,10 @@ def pcg_iter_bytes(n_nodes, n_blocks, prec, sym=False):
 def cpu_baseline(f, p, seconds, threads):
     """The reference's nb_sparse_solve_CG_precond_Jacobi (compiled from its
     own sources into oracle/_ref) on the same K and F, cut after about
-    `seconds` of work (10 calibration iterations decide the count); falls
-    back to this repo's C restatement of it."""
+    `seconds` of work (10 calibration iterations decide the count), on
+    `threads` OpenMP threads and again on 1 (the reference FEM driver's own
+    setting, static_elasticity2D.c:88-90) for a quarter of the time; falls
+    back to this repo's C restatement of it (1 thread)."""
     import ctypes as C
     import oracle
     rp, col, val, F = f.system()
@@ -85,30 +87,38 @@ def cpu_baseline(f, p, seconds, threads):
     if R is not None:
         h = R.ref_sparse_import(p.N, rp, col, val)
 
-        def run(n):
+        def run(n, nt):
             x = np.zeros(p.N)
             t0 = time.perf_counter()
-            R.ref_pcg_handle(h, F, x, n, 0.0, threads, C.byref(it), C.byref(tr))
+            R.ref_pcg_handle(h, F, x, n, 0.0, nt, C.byref(it), C.byref(tr))
             return time.perf_counter() - t0
     else:
         threads = 1
 
-        def run(n):
+        def run(n, nt):
             x = np.zeros(p.N)
             t0 = time.perf_counter()
             oracle.lib().orc_pcg_jacobi(p.N, rp, col, val, F, x, n, 0.0, C.byref(it), C.byref(tr))
             return time.perf_counter() - t0
-    t_cal = run(10)
-    iters = int(max(10, min(20000, seconds / max(t_cal / 10, 1e-6))))
-    dt = run(iters)
+
+    def sample(secs, nt):
+        t_cal = run(10, nt)
+        n = int(max(10, min(20000, secs / max(t_cal / 10, 1e-6))))
+        dt = run(n, nt)
+        return it.value, dt
+
+    n_it, dt = sample(seconds, threads)
+    one = None
+    if R is not None and threads > 1:
+        n1, dt1 = sample(seconds / 4, 1)
+        one = dict(value=p.N * n1 / dt1, iters=n1, seconds=dt1)
     if R is not None:
         R.ref_sparse_free(h)
-    n_it = it.value
     return dict(value=p.N * n_it / dt, unit="DOF*CG-iter/s", cores=threads, kind=kind,
                 sample=f"{n_it} Jacobi-PCG iterations on the same {p.N}-DOF system "
                        f"(reference build gcc -O2, {threads} OpenMP threads), {dt:.2f} s; "
                        f"time to 1e-8 extrapolated from the FP64 GPU iteration count",
-                seconds=dt, iters=n_it)
+                seconds=dt, iters=n_it, one_thread=one)
 
 
 def simp_bench(args, opts):
@@ -316,6 +326,10 @@ def main():
         cpu = cpu_baseline(f, p, args.cpu_seconds, args.cpu_threads)
         log(f"[bench] CPU: {cpu['value']:.3e} DOF*it/s ({cpu['kind']}, {cpu['seconds']:.1f}s)")
         cpu["time_to_1e8_fp64_extrapolated_s"] = round(iters[-1] * cpu["seconds"] / cpu["iters"], 1)
+        if cpu["one_thread"]:
+            o1 = cpu["one_thread"]
+            o1["time_to_1e8_fp64_extrapolated_s"] = round(iters[-1] * o1["seconds"] / o1["iters"], 1)
+            log(f"[bench] CPU, 1 thread: {o1['value']:.3e} DOF*it/s")
     f.destroy()
     simp = {}
     if not args.no_simp:
