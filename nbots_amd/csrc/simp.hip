@@ -39,14 +39,6 @@ __device__ __forceinline__ double filt_row(const DevFilter& H, uint32_t e,
     return s;
 }
 
-// out_e = (H x)_e / Hs_e      (density filter)
-__global__ void k_filter_density(DevFilter H, const double* __restrict__ x,
-                                 double* __restrict__ out) {
-    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= H.ne) return;
-    out[e] = filt_row<false>(H, e, x) / H.Hs[e];
-}
-
 // out_j = (H (d ./ Hs))_j     (sensitivity filter; H is symmetric)
 __global__ void k_filter_sens(DevFilter H, const double* __restrict__ d,
                               double* __restrict__ out) {
