@@ -453,7 +453,14 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
 // and down with the degree (4.2M DOF, degree 28-40: 1006, 940, 1118, 996,
 // 711), with FP64 sums it falls steadily (1006, 943, 882, 787, 709) at the
 // same cost per step.  NT: non-temporal matrix loads.
-template <bool NT>
+// ZM says how z = d_0 + d_1 + ... takes this step's direction:
+//   Z_SKIP  not yet (the next step adds it with its own),
+//   Z_PAIR  z += d_in + d_out (the previous step's and this one's),
+//   Z_ONE   z += d_out.
+// Adding the directions in pairs reads and writes z on half the steps
+// (16 B per node less on every other step) and rounds z once per pair.
+enum { Z_SKIP = 0, Z_PAIR = 1, Z_ONE = 2 };
+template <bool NT, int ZM>
 __global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __restrict__ din,
                                                        float2* __restrict__ dout,
                                                        float2* __restrict__ res,
@@ -467,11 +474,32 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __
         const double rx = (double)r0.x - t.x, ry = (double)r0.y - t.y;
         const float2 di = din[i];
         const double dx = c1 * (double)di.x + c2 * rx, dy = c1 * (double)di.y + c2 * ry;
-        const float2 zv = z[i];
         res[i] = make_float2((float)rx, (float)ry);
         dout[i] = make_float2((float)dx, (float)dy);
-        z[i] = make_float2((float)((double)zv.x + dx), (float)((double)zv.y + dy));
+        if constexpr (ZM != Z_SKIP) {
+            const float2 zv = z[i];
+            const double ax = ZM == Z_PAIR ? (double)di.x + dx : dx;
+            const double ay = ZM == Z_PAIR ? (double)di.y + dy : dy;
+            z[i] = make_float2((float)((double)zv.x + ax), (float)((double)zv.y + ay));
+        }
     }
+}
+
+// Step k of 1..last (z = d_0 on entry): even steps add d_{k-1} + d_k, odd
+// steps wait for the next one, except the last, which adds d_k alone.
+template <bool NT>
+static void launch_cheb_step64(uint32_t k, uint32_t last, unsigned G, hipStream_t s, const Sell2& K,
+                               const float2* din, float2* dout, float2* res, float2* z,
+                               PcgState* st, uint32_t n, double c1, double c2) {
+    if (k % 2 == 0)
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_PAIR>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+                           res, z, st, n, c1, c2);
+    else if (k == last)
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_ONE>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+                           res, z, st, n, c1, c2);
+    else
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_SKIP>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+                           res, z, st, n, c1, c2);
 }
 
 // z64 = dis z; partials z64.w (Polak-Ribiere numerator) and r.z64
@@ -897,11 +925,11 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         float2* dout = db;
         for (uint32_t k = 1; k < deg; ++k) {
             if (opt.precond_bits == 16 && cheb_cached)
-                hipLaunchKernelGGL(k_cheb_step64<false>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout,
-                                   res, z, w.st, nu, c1d[k], c2d[k]);
+                launch_cheb_step64<false>(k, deg - 1, G, s, m.s, din, dout, res, z, w.st, nu, c1d[k],
+                                          c2d[k]);
             else if (opt.precond_bits == 16)
-                hipLaunchKernelGGL(k_cheb_step64<true>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout,
-                                   res, z, w.st, nu, c1d[k], c2d[k]);
+                launch_cheb_step64<true>(k, deg - 1, G, s, m.s, din, dout, res, z, w.st, nu, c1d[k],
+                                         c2d[k]);
             else
                 hipLaunchKernelGGL(k_cheb_step<32>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
                                    w.st, nu, c1[k], c2[k]);
