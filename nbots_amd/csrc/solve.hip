@@ -92,32 +92,39 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_A(Op op, const typename Op::V* __
     put_partials<1>(acc, partials);
 }
 
+// The finalize kernels read the state they need together with the partials
+// and test `done` afterwards: one memory round trip instead of three in a
+// row (state, partials, state).  The partials buffer holds 4 x MAX_GRID
+// values, so the reads stay in it whatever the state says; a finished
+// solve ignores the sums.
 __global__ __launch_bounds__(FIN) void k_fin_A(PcgState* st, const double* partials, unsigned np) {
-    if (st->done) return;
+    const uint32_t done = st->done;
+    const double rz = st->rz;
     double tot[1];
     fin_sum<1>(partials, np, tot);
+    if (done) return;
     if (threadIdx.x == 0) {
         st->pw = tot[0];
-        st->alpha = st->rz / tot[0];
+        st->alpha = rz / tot[0];
     }
 }
 
 __global__ __launch_bounds__(FIN) void k_fin_B(PcgState* st, const double* partials, unsigned np) {
-    if (st->done) return;
+    const uint32_t done = st->done, k = st->k + 1, max_iter = st->max_iter;
+    const double rz = st->rz, gg_prev = st->gg_cur, tol2 = st->tol2;
     double tot[2];
     fin_sum<2>(partials, np, tot);
+    if (done) return;
     if (threadIdx.x == 0) {
-        st->beta = tot[0] / st->rz;
+        st->beta = tot[0] / rz;
         st->rz = tot[0];
-        const double gg_prev = st->gg_cur;
         st->gg_prev = gg_prev;
         st->gg_cur = tot[1];
-        const uint32_t k = st->k + 1;
         st->k = k;
-        if (!(gg_prev > st->tol2 && k < st->max_iter)) {
+        if (!(gg_prev > tol2 && k < max_iter)) {
             st->done = 1;
             st->tol_reached = sqrt(gg_prev);
-            st->status = gg_prev > st->tol2 ? 1u : 0u;
+            st->status = gg_prev > tol2 ? 1u : 0u;
         }
     }
 }
@@ -403,19 +410,21 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_update(Sell2 K, const double2* __
 
 __global__ __launch_bounds__(FIN) void k_fin_fcg_update(PcgState* st, const double* partials,
                                                         unsigned np, int first) {
-    if (st->done) return;
+    const uint32_t done = st->done, k = first ? st->k : st->k + 1, max_iter = st->max_iter;
+    const double tol2 = st->tol2;
     double tot[1];
     fin_sum<1>(partials, np, tot);
+    if (done) return;
     if (threadIdx.x == 0) {
         const double gg = tot[0];
-        if (!first) st->k += 1;
+        st->k = k;
         if (first) st->gg0 = gg;
         st->gg_cur = gg;
         st->tol_reached = sqrt(gg);
-        if (!(gg > st->tol2)) {
+        if (!(gg > tol2)) {
             st->done = 1;
             st->status = 0;
-        } else if (st->k >= st->max_iter || !isfinite(gg)) {
+        } else if (k >= max_iter || !isfinite(gg)) {
             st->done = 1;
             st->status = 1;
         }
@@ -525,12 +534,14 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_dots(const float2* __restrict__ z
 
 __global__ __launch_bounds__(FIN) void k_fin_fcg_dots(PcgState* st, const double* partials,
                                                       unsigned np, int first) {
-    if (st->done) return;
+    const uint32_t done = st->done;
+    const double alpha = st->alpha, rz_old = st->rz;
     double tot[2];
     fin_sum<2>(partials, np, tot);
+    if (done) return;
     if (threadIdx.x == 0) {
         const double zw = tot[0], rz = tot[1];
-        double beta = first ? 0.0 : -st->alpha * zw / st->rz;
+        double beta = first ? 0.0 : -alpha * zw / rz_old;
         if (!(rz > 0.0) || !isfinite(rz) || !isfinite(beta)) {
             // the preconditioned residual lost positivity: restart the
             // direction (p = z) and count the event
