@@ -103,21 +103,24 @@ struct Sell2 {
 
 // Symmetric 2x2-block sliced ELL, for a node matrix whose blocks satisfy
 // K(w, v) = K(v, w)^T bit for bit (what the T3 assembly produces).  Row v
-// stores its blocks (v, w >= v) in the Sell2 layout (upper part, ascending
-// w, diagonal first), and its blocks (v, w < v) as pairs (w, position of the
-// upper block (w, v)) (lower part, ascending w), read transposed.  A row
-// product visits the lower then the upper blocks, i.e. every column in
-// ascending order with the same products as the full layout, so results are
-// identical; the matrix stream shrinks from 36 B per block to 36 B per upper
-// and 8 B per lower block.
+// stores its diagonal block (v, v) as three values (it is symmetric itself),
+// its blocks (v, w > v) in the Sell2 layout (upper part, ascending w), and
+// its blocks (v, w < v) as pairs (w, position of the upper block (w, v))
+// (lower part, ascending w), read transposed.  A row product visits the
+// lower blocks, the diagonal block, then the upper blocks, i.e. every column
+// in ascending order with the same products as the full layout, so results
+// are identical; the matrix stream shrinks from 36 B per block to 24 B per
+// diagonal, 36 B per upper and 8 B per lower block.
 struct Sym2 {
     uint32_t n_nodes = 0;
     uint32_t n_slices = 0;
     size_t n_upper = 0;              // stored upper blocks, padding included
     size_t n_lower = 0;              // stored lower entries, padding included
+    double* d_val = nullptr;         // diagonal blocks: [2 * n_nodes] (K00, K01) per
+                                     // node, then [n_nodes] K11 (K10 == K01)
     uint32_t* u_slice_ptr = nullptr; // [n_slices + 1]
     uint8_t* u_rowlen = nullptr;     // [n_nodes]
-    uint32_t* u_bcol = nullptr;      // [n_upper]
+    uint32_t* u_bcol = nullptr;      // [n_upper], block columns w > v
     double* u_val = nullptr;         // [4 * n_upper], two row planes as Sell2::val
     uint32_t* l_slice_ptr = nullptr; // [n_slices + 1]
     uint8_t* l_rowlen = nullptr;     // [n_nodes]

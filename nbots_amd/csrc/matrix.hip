@@ -70,30 +70,33 @@ __global__ void k_diag_sell2(Sell2 A, double* __restrict__ diag) {
 }
 
 // ---- symmetric copy ---------------------------------------------------------
-// Per row: the number of block columns below the diagonal (lower) and from
-// the diagonal on (upper).  A row whose columns are not strictly ascending
-// sets *bad (the symmetric layout then is not used).
+// Per row: the number of block columns below the diagonal (lower) and above
+// it (upper).  A row whose columns are not strictly ascending, or that lacks
+// its diagonal block, sets *bad (the symmetric layout then is not used).
 __global__ void k_sym_count(Sell2 A, uint8_t* __restrict__ ulen, uint8_t* __restrict__ llen,
                             unsigned* bad) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     if (v >= A.n_nodes) return;
     const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const int len = A.rowlen[v];
-    int lo = 0;
+    int lo = 0, dg = 0;
     int64_t prev = -1;
     for (int k = 0; k < len; ++k) {
         const uint32_t w = A.bcol[g + (size_t)k * SLICE];
         if ((int64_t)w <= prev) *bad = 1u;
         prev = w;
         lo += w < v;
+        dg += w == v;
     }
-    ulen[v] = (uint8_t)(len - lo);
+    if (dg != 1) *bad = 1u;
+    ulen[v] = (uint8_t)(len - lo - dg);
     llen[v] = (uint8_t)lo;
 }
 
 // Upper columns, and for every lower block (v, w) the pair (w, position of
-// the upper block (w, v)); *bad when row w lacks column v.  Padding: upper
-// (column v), lower (v, 0); never read.
+// the upper block (w, v)); *bad when row w lacks column v.  Row v of the full
+// layout holds lo lower blocks, the diagonal block, then the upper ones.
+// Padding: upper (column v), lower (v, 0); never read.
 __global__ void k_sym_pattern(Sell2 A, Sym2 Y, unsigned* bad) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     if (v >= A.n_nodes) return;
@@ -104,7 +107,7 @@ __global__ void k_sym_pattern(Sell2 A, Sym2 Y, unsigned* bad) {
     const uint32_t uw = (Y.u_slice_ptr[s + 1] - Y.u_slice_ptr[s]) / SLICE;
     for (uint32_t k = 0; k < uw; ++k)
         Y.u_bcol[u + (size_t)k * SLICE] =
-            (int)k < up ? A.bcol[g + (size_t)(lo + (int)k) * SLICE] : v;
+            (int)k < up ? A.bcol[g + (size_t)(lo + 1 + (int)k) * SLICE] : v;
     const size_t l = (size_t)Y.l_slice_ptr[s] + (v & 63);
     const uint32_t lw = (Y.l_slice_ptr[s + 1] - Y.l_slice_ptr[s]) / SLICE;
     for (uint32_t k = 0; k < lw; ++k) {
@@ -120,26 +123,30 @@ __global__ void k_sym_pattern(Sell2 A, Sym2 Y, unsigned* bad) {
                     break;
                 }
             }
-            if (t < low) {
+            if (t <= low) {
                 *bad = 1u;
-                t = low;
+                t = low + 1;
             }
-            pos = Y.u_slice_ptr[w >> 6] + (w & 63) + (uint32_t)(t - low) * SLICE;
+            pos = Y.u_slice_ptr[w >> 6] + (w & 63) + (uint32_t)(t - low - 1) * SLICE;
         }
         Y.l_idx[2 * (l + (size_t)k * SLICE)] = w;
         Y.l_idx[2 * (l + (size_t)k * SLICE) + 1] = pos;
     }
 }
 
-// Copy the upper blocks' values from the full layout.
+// Copy the diagonal and upper blocks' values from the full layout.
 __global__ void k_sym_values(Sell2 A, Sym2 Y) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     if (v >= A.n_nodes) return;
     const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const size_t u = (size_t)Y.u_slice_ptr[v >> 6] + (v & 63);
     const int lo = Y.l_rowlen[v], up = Y.u_rowlen[v];
+    const size_t gd = g + (size_t)lo * SLICE;
+    Y.d_val[2 * (size_t)v] = A.val[ent(A.n_blocks, gd, 0, 0)];
+    Y.d_val[2 * (size_t)v + 1] = A.val[ent(A.n_blocks, gd, 0, 1)];
+    Y.d_val[2 * (size_t)Y.n_nodes + v] = A.val[ent(A.n_blocks, gd, 1, 1)];
     for (int k = 0; k < up; ++k) {
-        const size_t gi = g + (size_t)(lo + k) * SLICE, ui = u + (size_t)k * SLICE;
+        const size_t gi = g + (size_t)(lo + 1 + k) * SLICE, ui = u + (size_t)k * SLICE;
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -149,14 +156,16 @@ __global__ void k_sym_values(Sell2 A, Sym2 Y) {
 }
 
 // *bad unless every lower block K(v, w) equals the transpose of the stored
-// upper block K(w, v) bit for bit.
+// upper block K(w, v), and the diagonal block its own transpose, bit for bit.
 __global__ void k_sym_check(Sell2 A, Sym2 Y, unsigned* bad) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     if (v >= A.n_nodes) return;
     const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const size_t l = (size_t)Y.l_slice_ptr[v >> 6] + (v & 63);
     const int lo = Y.l_rowlen[v];
-    bool ok = true;
+    const size_t gd = g + (size_t)lo * SLICE;
+    bool ok = __double_as_longlong(A.val[ent(A.n_blocks, gd, 0, 1)]) ==
+              __double_as_longlong(A.val[ent(A.n_blocks, gd, 1, 0)]);
     for (int k = 0; k < lo; ++k) {
         const size_t gi = g + (size_t)k * SLICE;
         const size_t pos = Y.l_idx[2 * (l + (size_t)k * SLICE) + 1];
@@ -343,8 +352,8 @@ void ws_free(Workspace& w) {
 
 static void sym_free(DevMatrix& m) {
     Sym2& Y = m.y;
-    void* ps[] = {Y.u_slice_ptr, Y.u_rowlen, Y.u_bcol, Y.u_val,
-                  Y.l_slice_ptr, Y.l_rowlen, Y.l_idx,  m.sym_flag};
+    void* ps[] = {Y.d_val,       Y.u_slice_ptr, Y.u_rowlen, Y.u_bcol, Y.u_val,
+                  Y.l_slice_ptr, Y.l_rowlen,    Y.l_idx,    m.sym_flag};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     Y = Sym2();
@@ -553,6 +562,7 @@ static int sym_build_pattern(DevMatrix& m, hipStream_t s) {
         hipMalloc(&Y.l_slice_ptr, (ns + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&Y.u_bcol, (Y.n_upper + 1) * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&Y.u_val, (4 * Y.n_upper + 4) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&Y.d_val, (3 * (size_t)nn + 2) * sizeof(double)) != hipSuccess ||
         hipMalloc(&Y.l_idx, (2 * Y.n_lower + 2) * sizeof(uint32_t)) != hipSuccess)
         return give_up();
     NBX_HIP(hipMemcpyAsync(Y.u_slice_ptr, usp.data(), (ns + 1) * sizeof(uint32_t),

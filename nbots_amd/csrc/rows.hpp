@@ -129,8 +129,10 @@ __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const doubl
 }
 
 // Symmetric layout: the lower blocks (w < v, ascending), each the transpose
-// of the stored upper block (w, v), then the upper blocks (w >= v,
-// ascending); the same products in the same order as row64.  The transposed
+// of the stored upper block (w, v), the diagonal block (three stored values,
+// read with x_v in whole-wave coalesced loads issued first), then the upper
+// blocks (w > v, ascending); the same products in the same order as row64.
+// The transposed
 // blocks belong to rows that nearby workgroups of the same XCD stream at
 // about the same time, so their loads are served by L2.  Lower and upper
 // blocks are taken UL and UU at a time; few registers per row keep many
@@ -140,6 +142,9 @@ template <int UL, int UU>
 __device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const double2* __restrict__ x) {
     const dv2* __restrict__ pa = reinterpret_cast<const dv2*>(Y.u_val);
     const dv2* __restrict__ pb = reinterpret_cast<const dv2*>(Y.u_val + 2 * Y.n_upper);
+    const dv2 d01 = nt(reinterpret_cast<const dv2*>(Y.d_val) + v);
+    const double d11 = nt(Y.d_val + 2 * (size_t)Y.n_nodes + v);
+    const double2 xd = x[v];
     double s0 = 0.0, s1 = 0.0;
     {
         const uv2* __restrict__ li = reinterpret_cast<const uv2*>(Y.l_idx);
@@ -172,6 +177,11 @@ __device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const doubl
             }
         }
     }
+    // block (v, v) = [[d01.x, d01.y], [d01.y, d11]]
+    s0 = s0 + d01.x * xd.x;
+    s0 = s0 + d01.y * xd.y;
+    s1 = s1 + d01.y * xd.x;
+    s1 = s1 + d11 * xd.y;
     {
         const size_t base = (size_t)Y.u_slice_ptr[v >> 6] + (v & 63);
         const int len = Y.u_rowlen[v];

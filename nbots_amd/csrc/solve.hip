@@ -690,7 +690,8 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
     // iteration's working set (matrix + five vectors) exceeds the 256 MiB
     // Infinity Cache; below that, caching them is faster.
     const double mat_bytes = !m.blocked ? 12.0 * (double)m.c.nnz
-                             : m.has_sym ? 36.0 * (double)m.y.n_upper + 8.0 * (double)m.y.n_lower
+                             : m.has_sym ? 24.0 * (double)m.y.n_nodes + 36.0 * (double)m.y.n_upper +
+                                               8.0 * (double)m.y.n_lower
                                          : 36.0 * (double)m.s.n_blocks;
     const bool stream = mat_bytes + 40.0 * (double)m.n > (double)INFINITY_CACHE_BYTES;
     if (m.blocked && m.has_sym)

@@ -33,14 +33,15 @@ def spmv_bytes(n_nodes: int, n_blocks: int, prec: int) -> int:
 
 def sym_spmv_bytes(n_nodes: int, n_valid: int) -> int:
     """Algorithmic HBM bytes of one FP64 SpMV in the symmetric layout: per
-    upper block (v, w >= v) its column and 4 values (36 B); per lower block
-    its (column, upper position) pair (8 B; the transposed values are those
-    of upper blocks streamed by nearby workgroups, served by L2); per node
-    two row lengths, the x read and the y write; per 64-node slice two
-    offsets.  With every diagonal block stored, upper = (valid + nodes) / 2
-    and lower = (valid - nodes) / 2 blocks."""
-    up, lo = (n_valid + n_nodes) // 2, (n_valid - n_nodes) // 2
-    return up * 36 + lo * 8 + 2 * n_nodes + 8 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * 8
+    node its diagonal block's 3 values (24 B), two row lengths, the x read
+    and the y write; per upper block (v, w > v) its column and 4 values
+    (36 B); per lower block its (column, upper position) pair (8 B; the
+    transposed values are those of upper blocks streamed by nearby
+    workgroups, served by L2); per 64-node slice two offsets.  With every
+    diagonal block stored, upper = lower = (valid - nodes) / 2 blocks."""
+    up = lo = (n_valid - n_nodes) // 2
+    return (n_nodes * 24 + up * 36 + lo * 8 + 2 * n_nodes + 8 * ((n_nodes + 63) // 64)
+            + 2 * (2 * n_nodes) * 8)
 
 
 def main():
