@@ -338,9 +338,14 @@ typedef struct {
 	double ms;            /* device time of the solve                   */
 	int32_t fallback;     /* MIXED CG: sweeps stalled and FP64 PCG
 				 finished; CHEB: direction restarts         */
-	int32_t layout;       /* FP64 matrix layout of the solve: 0 full
-				 2x2-block rows, 1 symmetric (upper blocks,
-				 lower ones read transposed; same results)  */
+	int32_t layout;       /* bit 0: the FP64 products ran on the
+				 symmetric layout (diagonal blocks, upper
+				 blocks, lower ones read transposed; same
+				 results), else on full 2x2-block rows;
+				 bit 1: the fp16 Chebyshev preconditioner
+				 read the copy without diagonal blocks
+				 (each scaled diagonal entry is 1 in fp16),
+				 else the full fp16 copy                    */
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 

@@ -101,6 +101,22 @@ struct Sell2 {
     _Float16* val16 = nullptr;     // [4 * n_blocks], Jacobi-scaled fp16 copy
 };
 
+// The fp16 Jacobi-scaled copy D^-1/2 K D^-1/2 without its diagonal blocks,
+// for the Chebyshev preconditioner.  Each diagonal entry of the scaled
+// matrix rounds to 1.0 in fp16 (the build checks it), so a diagonal block
+// is [[1, od.x], [od.y, 1]] and costs 4 B per node instead of 12 B of
+// column and values in the ELL; the off-diagonal blocks keep the Sell2 form.
+struct Off16 {
+    uint32_t n_nodes = 0;
+    uint32_t n_slices = 0;
+    size_t n_blocks = 0;             // stored off-diagonal blocks, padding included
+    uint32_t* slice_ptr = nullptr;   // [n_slices + 1]
+    uint8_t* rowlen = nullptr;       // [n_nodes]
+    uint32_t* bcol = nullptr;        // [n_blocks]
+    _Float16* val = nullptr;         // [4 * n_blocks], 8 B per block as Sell2::val16
+    _Float16* od = nullptr;          // [2 * n_nodes]: (K01, K10) scaled, per node
+};
+
 // Symmetric 2x2-block sliced ELL, for a node matrix whose blocks satisfy
 // K(w, v) = K(v, w)^T bit for bit (what the T3 assembly produces).  Row v
 // stores its diagonal block (v, v) as three values (it is symmetric itself),

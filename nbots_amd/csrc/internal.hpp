@@ -48,6 +48,13 @@ struct DevMatrix {
     bool sym_checked = false;  // y's values reflect the current values
     bool has_sym = false;      // ... and the values are symmetric: use y
     unsigned* sym_flag = nullptr;
+    // fp16 scaled copy without the diagonal blocks (the Chebyshev
+    // preconditioner's matrix): pattern once per pattern, values with the scale
+    Off16 o;
+    bool off_tried = false;    // o's pattern has been attempted
+    bool off_pattern = false;  // o's pattern is built
+    bool has_off16 = false;    // o's values reflect the current scale
+    unsigned* off_flag = nullptr;
 };
 
 // Per-matrix solver scratch (device).  Sized for n scalar rows.
@@ -111,6 +118,9 @@ int mat_prepare_sym(DevMatrix& m, hipStream_t s);
 // Jacobi-scaled copy D^-1/2 A D^-1/2 with 16- or 32-bit values (and the
 // Gershgorin bound lambda_max); a no-op when current.
 int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits);
+// fp16 scaled copy without the diagonal blocks (m.has_off16 tells whether
+// the matrix allows it); a no-op when current.
+int mat_make_off16(DevMatrix& m, hipStream_t s);
 // the scaled copy a mixed-precision solve with these options reads
 int mat_prepare_mixed(DevMatrix& m, hipStream_t s, const MixedOpts& o);
 

@@ -233,23 +233,82 @@ __global__ void k_scale32(Sell2 A, const double* __restrict__ dis) {
     }
 }
 
+// Stored block idx (row v, column w) of D^-1/2 K D^-1/2 in fp16.
+__device__ __forceinline__ hv4 scaled16(const Sell2& A, size_t idx, uint32_t w, double r0,
+                                        double r1, const double* __restrict__ dis) {
+    const double c0 = dis[2 * w], c1 = dis[2 * w + 1];
+    const double* a = A.val + ent(A.n_blocks, idx, 0, 0);
+    const double* b = A.val + ent(A.n_blocks, idx, 1, 0);
+    hv4 o;
+    o.x = (_Float16)(float)(a[0] * r0 * c0);
+    o.y = (_Float16)(float)(a[1] * r0 * c1);
+    o.z = (_Float16)(float)(b[0] * r1 * c0);
+    o.w = (_Float16)(float)(b[1] * r1 * c1);
+    return o;
+}
+
 __global__ void k_scale16(Sell2 A, const double* __restrict__ dis) {
     const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
     if (v >= A.n_nodes) return;
     size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
     const int len = A.rowlen[v];
     const double r0 = dis[2 * v], r1 = dis[2 * v + 1];
+    for (int k = 0; k < len; ++k, idx += SLICE)
+        reinterpret_cast<hv4*>(A.val16)[idx] = scaled16(A, idx, A.bcol[idx], r0, r1, dis);
+}
+
+// ---- fp16 scaled copy without the diagonal blocks ------------------------
+// Per row: the number of off-diagonal blocks; *bad unless the row holds
+// exactly one diagonal block.
+__global__ void k_off_count(Sell2 A, uint8_t* __restrict__ olen, unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    int dg = 0;
+    for (int k = 0; k < len; ++k) dg += A.bcol[g + (size_t)k * SLICE] == v;
+    if (dg != 1) *bad = 1u;
+    olen[v] = (uint8_t)(dg == 1 ? len - 1 : 0);
+}
+
+// Off-diagonal block columns in row order.  Padding: column v, never read.
+__global__ void k_off_pattern(Sell2 A, Off16 O) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    const size_t g = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const size_t o = (size_t)O.slice_ptr[v >> 6] + (v & 63);
+    const uint32_t ow = (O.slice_ptr[(v >> 6) + 1] - O.slice_ptr[v >> 6]) / SLICE;
+    const int len = A.rowlen[v];
+    uint32_t k2 = 0;
+    for (int k = 0; k < len; ++k) {
+        const uint32_t w = A.bcol[g + (size_t)k * SLICE];
+        if (w != v && k2 < ow) O.bcol[o + (size_t)k2++ * SLICE] = w;
+    }
+    for (; k2 < ow; ++k2) O.bcol[o + (size_t)k2 * SLICE] = v;
+}
+
+// The same fp16 values as k_scale16: off-diagonal blocks into O's ELL, the
+// diagonal block's (K01, K10) into O.od; *bad unless its K00 and K11 are 1.
+__global__ void k_scale16_off(Sell2 A, Off16 O, const double* __restrict__ dis, unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= A.n_nodes) return;
+    size_t idx = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    size_t o = (size_t)O.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v], olen = O.rowlen[v];
+    const double r0 = dis[2 * v], r1 = dis[2 * v + 1];
+    int k2 = 0;
     for (int k = 0; k < len; ++k, idx += SLICE) {
         const uint32_t w = A.bcol[idx];
-        const double c0 = dis[2 * w], c1 = dis[2 * w + 1];
-        const double* a = A.val + ent(A.n_blocks, idx, 0, 0);
-        const double* b = A.val + ent(A.n_blocks, idx, 1, 0);
-        hv4 o;
-        o.x = (_Float16)(float)(a[0] * r0 * c0);
-        o.y = (_Float16)(float)(a[1] * r0 * c1);
-        o.z = (_Float16)(float)(b[0] * r1 * c0);
-        o.w = (_Float16)(float)(b[1] * r1 * c1);
-        reinterpret_cast<hv4*>(A.val16)[idx] = o;
+        const hv4 h = scaled16(A, idx, w, r0, r1, dis);
+        if (w == v) {
+            if (!(h.x == (_Float16)1.0f && h.w == (_Float16)1.0f)) *bad = 1u;
+            O.od[2 * (size_t)v] = h.y;
+            O.od[2 * (size_t)v + 1] = h.z;
+        } else if (k2 < olen) {
+            reinterpret_cast<hv4*>(O.val)[o] = h;
+            o += SLICE;
+            ++k2;
+        }
     }
 }
 
@@ -361,8 +420,19 @@ static void sym_free(DevMatrix& m) {
     m.sym_pattern = m.sym_checked = m.has_sym = false;
 }
 
+static void off_free(DevMatrix& m) {
+    Off16& O = m.o;
+    void* ps[] = {O.slice_ptr, O.rowlen, O.bcol, O.val, O.od, m.off_flag};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    O = Off16();
+    m.off_flag = nullptr;
+    m.off_pattern = m.has_off16 = false;
+}
+
 void mat_free(DevMatrix& m) {
     sym_free(m);
+    off_free(m);
     void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32, m.s.val16,
                   m.c.rowptr,    m.c.col,    m.c.val,  m.diag,  m.dis};
     for (void* p : ps)
@@ -500,6 +570,7 @@ void mat_scaled_stale(DevMatrix& m) {
     m.has_scale = false;
     m.has_val32 = false;
     m.has_val16 = false;
+    m.has_off16 = false;
     m.sym_checked = false;
     m.has_sym = false;
 }
@@ -596,6 +667,38 @@ int mat_prepare_sym(DevMatrix& m, hipStream_t s) {
     return NBX_OK;
 }
 
+// dis = 1/sqrt(diag) and the Gershgorin bound of the scaled matrix, once per
+// set of values.
+static int ensure_scale(DevMatrix& m, hipStream_t s) {
+    if (m.has_scale) return NBX_OK;
+    if (!m.dis) NBX_HIP(hipMalloc(&m.dis, (size_t)m.n * sizeof(double)));
+    const unsigned gg = grid_for(m.s.n_nodes);
+    unsigned* bad = nullptr;
+    double *part = nullptr, *dmax = nullptr;
+    NBX_HIP(hipMalloc(&bad, sizeof(unsigned)));
+    NBX_HIP(hipMalloc(&part, (size_t)gg * sizeof(double)));
+    NBX_HIP(hipMalloc(&dmax, sizeof(double)));
+    NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
+    // Gershgorin bound of the scaled matrix (Chebyshev preconditioner)
+    hipLaunchKernelGGL(k_gersh, dim3(gg), dim3(BLOCK), 0, s, m.s, m.dis, part);
+    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(FIN), 0, s, part, gg, dmax);
+    NBX_HIP(hipGetLastError());
+    unsigned hbad = 0;
+    NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipMemcpyAsync(&m.lambda_max, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
+    NBX_HIP(hipStreamSynchronize(s));
+    (void)hipFree(bad);
+    (void)hipFree(part);
+    (void)hipFree(dmax);
+    if (hbad) {
+        set_last_error("matrix diagonal has %u non-positive entries", hbad);
+        return NBX_ERR_ARG;
+    }
+    m.has_scale = true;
+    return NBX_OK;
+}
+
 int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
     if (!m.blocked) {
         set_last_error("mixed precision needs a 2-dof node matrix");
@@ -605,33 +708,7 @@ int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
         set_last_error("scaled copy of %d-bit values (16 or 32)", bits);
         return NBX_ERR_ARG;
     }
-    if (!m.has_scale) {
-        if (!m.dis) NBX_HIP(hipMalloc(&m.dis, (size_t)m.n * sizeof(double)));
-        const unsigned gg = grid_for(m.s.n_nodes);
-        unsigned* bad = nullptr;
-        double *part = nullptr, *dmax = nullptr;
-        NBX_HIP(hipMalloc(&bad, sizeof(unsigned)));
-        NBX_HIP(hipMalloc(&part, (size_t)gg * sizeof(double)));
-        NBX_HIP(hipMalloc(&dmax, sizeof(double)));
-        NBX_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), s));
-        hipLaunchKernelGGL(k_dis, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, m.diag, m.dis, m.n, bad);
-        // Gershgorin bound of the scaled matrix (Chebyshev preconditioner)
-        hipLaunchKernelGGL(k_gersh, dim3(gg), dim3(BLOCK), 0, s, m.s, m.dis, part);
-        hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(FIN), 0, s, part, gg, dmax);
-        NBX_HIP(hipGetLastError());
-        unsigned hbad = 0;
-        NBX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        NBX_HIP(hipMemcpyAsync(&m.lambda_max, dmax, sizeof(double), hipMemcpyDeviceToHost, s));
-        NBX_HIP(hipStreamSynchronize(s));
-        (void)hipFree(bad);
-        (void)hipFree(part);
-        (void)hipFree(dmax);
-        if (hbad) {
-            set_last_error("matrix diagonal has %u non-positive entries", hbad);
-            return NBX_ERR_ARG;
-        }
-        m.has_scale = true;
-    }
+    NBX_TRY(ensure_scale(m, s));
     if (bits == 32 && !m.has_val32) {
         if (!m.s.val32) NBX_HIP(hipMalloc(&m.s.val32, (4 * m.s.n_blocks + 4) * sizeof(float)));
         hipLaunchKernelGGL(k_scale32, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.dis);
@@ -647,7 +724,71 @@ int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
     return NBX_OK;
 }
 
+// O's pattern from the full one.  Leaves m.off_pattern false (and nothing
+// allocated) when a row lacks its diagonal block, the offsets exceed 32
+// bits, or there is no device memory for it.
+static int off_build_pattern(DevMatrix& m, hipStream_t s) {
+    const Sell2& A = m.s;
+    Off16& O = m.o;
+    const uint32_t nn = A.n_nodes, ns = A.n_slices;
+    auto give_up = [&]() {
+        (void)hipGetLastError();
+        off_free(m);
+        return NBX_OK;
+    };
+    if (hipMalloc(&m.off_flag, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&O.rowlen, (size_t)nn + 1) != hipSuccess)
+        return give_up();
+    NBX_HIP(hipMemsetAsync(m.off_flag, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_off_count, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, O.rowlen, m.off_flag);
+    NBX_HIP(hipGetLastError());
+    std::vector<uint8_t> ol(nn);
+    NBX_HIP(hipMemcpyAsync(ol.data(), O.rowlen, nn, hipMemcpyDeviceToHost, s));
+    unsigned bad = 0;
+    NBX_TRY(read_flag(m.off_flag, s, &bad));
+    std::vector<uint32_t> osp;
+    if (bad || !slice_offsets(ol, ns, osp)) return give_up();
+    O.n_nodes = nn;
+    O.n_slices = ns;
+    O.n_blocks = osp[ns];
+    if (hipMalloc(&O.slice_ptr, (ns + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&O.bcol, (O.n_blocks + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&O.val, (4 * O.n_blocks + 4) * sizeof(_Float16)) != hipSuccess ||
+        hipMalloc(&O.od, (2 * (size_t)nn + 2) * sizeof(_Float16)) != hipSuccess)
+        return give_up();
+    NBX_HIP(hipMemcpyAsync(O.slice_ptr, osp.data(), (ns + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_off_pattern, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, O);
+    NBX_HIP(hipGetLastError());
+    m.off_pattern = true;
+    return NBX_OK;
+}
+
+int mat_make_off16(DevMatrix& m, hipStream_t s) {
+    if (!m.blocked || m.n == 0) return NBX_OK;
+    NBX_TRY(ensure_scale(m, s));
+    if (!m.off_tried) {
+        m.off_tried = true;
+        NBX_TRY(off_build_pattern(m, s));
+    }
+    if (!m.off_pattern || m.has_off16) return NBX_OK;
+    NBX_HIP(hipMemsetAsync(m.off_flag, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_scale16_off, dim3(grid_for(m.s.n_nodes)), dim3(BLOCK), 0, s, m.s, m.o,
+                       m.dis, m.off_flag);
+    NBX_HIP(hipGetLastError());
+    unsigned bad = 0;
+    NBX_TRY(read_flag(m.off_flag, s, &bad));
+    m.has_off16 = bad == 0;
+    return NBX_OK;
+}
+
 int mat_prepare_mixed(DevMatrix& m, hipStream_t s, const MixedOpts& o) {
+    // the fp16 Chebyshev preconditioner reads the copy without diagonal
+    // blocks when the matrix allows it, the full fp16 copy otherwise
+    if (m.blocked && o.variant == CHEB_FCG && o.precond_bits == 16) {
+        NBX_TRY(mat_make_off16(m, s));
+        if (m.has_off16) return NBX_OK;
+    }
     return mat_make_scaled(m, s, o.variant == CHEB_FCG ? o.precond_bits : 32);
 }
 

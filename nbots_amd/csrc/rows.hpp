@@ -11,6 +11,7 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
 typedef float fv2 __attribute__((ext_vector_type(2)));
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hv2 __attribute__((ext_vector_type(2)));
 typedef uint32_t uv2 __attribute__((ext_vector_type(2)));
 
 // The matrix stream is read once per product: non-temporal loads keep it
@@ -325,6 +326,67 @@ __device__ __forceinline__ double2 row16d(const Sell2& A, uint32_t v, const floa
         }
     }
     return make_double2(s0, s1);
+}
+
+// The same row on the copy without diagonal blocks: the diagonal block
+// [[1, od.x], [od.y, 1]] times the row's own entry xo (which the caller has
+// loaded), then the off-diagonal blocks.
+template <bool NT = true>
+__device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const float2* __restrict__ x,
+                                           float2 xo) {
+    const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = A.rowlen[v];
+    const hv4* __restrict__ val = reinterpret_cast<const hv4*>(A.val);
+    hv2 od;
+    if constexpr (NT)
+        od = nt(reinterpret_cast<const hv2*>(A.od) + v);
+    else
+        od = reinterpret_cast<const hv2*>(A.od)[v];
+    double s0 = (double)xo.x + (double)od.x * (double)xo.y;
+    double s1 = (double)od.y * (double)xo.x + (double)xo.y;
+    for (int k0 = 0; k0 < len; k0 += UNROLL) {
+        uint32_t w[UNROLL];
+        hv4 a[UNROLL];
+        float2 xv[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (k0 + k < len) {
+                const size_t i = base + (size_t)(k0 + k) * SLICE;
+                if constexpr (NT) {
+                    w[k] = nt(&A.bcol[i]);
+                    a[k] = nt(&val[i]);
+                } else {
+                    w[k] = A.bcol[i];
+                    a[k] = val[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k)
+            if (k0 + k < len) xv[k] = x[w[k]];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            if (k0 + k < len) {
+                s0 = s0 + (double)a[k].x * (double)xv[k].x;
+                s0 = s0 + (double)a[k].y * (double)xv[k].y;
+                s1 = s1 + (double)a[k].z * (double)xv[k].x;
+                s1 = s1 + (double)a[k].w * (double)xv[k].y;
+            }
+        }
+    }
+    return make_double2(s0, s1);
+}
+
+// the Chebyshev step's row on either fp16 layout
+template <bool NT>
+__device__ __forceinline__ double2 cheb_row(const Sell2& A, uint32_t v, const float2* __restrict__ x,
+                                            float2) {
+    return row16d<NT>(A, v, x);
+}
+template <bool NT>
+__device__ __forceinline__ double2 cheb_row(const Off16& A, uint32_t v, const float2* __restrict__ x,
+                                            float2 xo) {
+    return row16od<NT>(A, v, x, xo);
 }
 
 // y = A x with the scaled copy of BITS-bit values

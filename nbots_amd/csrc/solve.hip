@@ -468,9 +468,10 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
 //   Z_ONE   z += d_out.
 // Adding the directions in pairs reads and writes z on half the steps
 // (16 B per node less on every other step) and rounds z once per pair.
+// M: the fp16 matrix, Sell2 (full copy) or Off16 (without diagonal blocks).
 enum { Z_SKIP = 0, Z_PAIR = 1, Z_ONE = 2 };
-template <bool NT, int ZM>
-__global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __restrict__ din,
+template <bool NT, int ZM, class M>
+__global__ __launch_bounds__(BLOCK) void k_cheb_step64(M K, const float2* __restrict__ din,
                                                        float2* __restrict__ dout,
                                                        float2* __restrict__ res,
                                                        float2* __restrict__ z, PcgState* st,
@@ -478,10 +479,10 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __
     if (st->done) return;
     const Chunk c = my_chunk(n);
     for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
-        const double2 t = row16d<NT>(K, (uint32_t)i, din);
+        const float2 di = din[i];
+        const double2 t = cheb_row<NT>(K, (uint32_t)i, din, di);
         const float2 r0 = res[i];
         const double rx = (double)r0.x - t.x, ry = (double)r0.y - t.y;
-        const float2 di = din[i];
         const double dx = c1 * (double)di.x + c2 * rx, dy = c1 * (double)di.y + c2 * ry;
         res[i] = make_float2((float)rx, (float)ry);
         dout[i] = make_float2((float)dx, (float)dy);
@@ -496,18 +497,18 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step64(Sell2 K, const float2* __
 
 // Step k of 1..last (z = d_0 on entry): even steps add d_{k-1} + d_k, odd
 // steps wait for the next one, except the last, which adds d_k alone.
-template <bool NT>
-static void launch_cheb_step64(uint32_t k, uint32_t last, unsigned G, hipStream_t s, const Sell2& K,
+template <bool NT, class M>
+static void launch_cheb_step64(uint32_t k, uint32_t last, unsigned G, hipStream_t s, const M& K,
                                const float2* din, float2* dout, float2* res, float2* z,
                                PcgState* st, uint32_t n, double c1, double c2) {
     if (k % 2 == 0)
-        hipLaunchKernelGGL((k_cheb_step64<NT, Z_PAIR>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_PAIR, M>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
                            res, z, st, n, c1, c2);
     else if (k == last)
-        hipLaunchKernelGGL((k_cheb_step64<NT, Z_ONE>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_ONE, M>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
                            res, z, st, n, c1, c2);
     else
-        hipLaunchKernelGGL((k_cheb_step64<NT, Z_SKIP>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
+        hipLaunchKernelGGL((k_cheb_step64<NT, Z_SKIP, M>), dim3(G), dim3(BLOCK), 0, s, K, din, dout,
                            res, z, st, n, c1, c2);
 }
 
@@ -927,8 +928,10 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     // read by each of the degree - 1 steps in a row: when it fits in the
     // Infinity Cache, cached loads keep it there between steps (at 4.2M DOF,
     // 243 MB: 1.48 -> 1.32 s per solve); otherwise the matrix streams.
-    const bool cheb_cached =
-        12.0 * (double)m.s.n_blocks + 33.0 * (double)nu <= (double)INFINITY_CACHE_BYTES;
+    const bool off16 = opt.precond_bits == 16 && m.has_off16;
+    const double cheb_mat = off16 ? 12.0 * (double)m.o.n_blocks + 4.0 * (double)nu
+                                  : 12.0 * (double)m.s.n_blocks;
+    const bool cheb_cached = cheb_mat + 33.0 * (double)nu <= (double)INFINITY_CACHE_BYTES;
     auto precondition = [&](int first) {
         hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv, dis, res,
                            da, z, w.st, w.partials, nu, first, inv_theta);
@@ -936,7 +939,13 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         float2* din = da;
         float2* dout = db;
         for (uint32_t k = 1; k < deg; ++k) {
-            if (opt.precond_bits == 16 && cheb_cached)
+            if (off16 && cheb_cached)
+                launch_cheb_step64<false>(k, deg - 1, G, s, m.o, din, dout, res, z, w.st, nu, c1d[k],
+                                          c2d[k]);
+            else if (off16)
+                launch_cheb_step64<true>(k, deg - 1, G, s, m.o, din, dout, res, z, w.st, nu, c1d[k],
+                                         c2d[k]);
+            else if (opt.precond_bits == 16 && cheb_cached)
                 launch_cheb_step64<false>(k, deg - 1, G, s, m.s, din, dout, res, z, w.st, nu, c1d[k],
                                           c2d[k]);
             else if (opt.precond_bits == 16)
@@ -994,7 +1003,7 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
-        stats->layout = m.has_sym ? 1 : 0;
+        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0);
         stats->res0 = sqrt(w.h_st->gg0);
         const uint32_t ran = (w.h_st->k + every - 1) / every;
         NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));
@@ -1038,7 +1047,7 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
 int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, const MixedOpts& opt,
                 uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats) {
     const bool fp16 = opt.variant == CHEB_FCG && opt.precond_bits == 16;
-    if (!m.blocked || !m.has_scale || !(fp16 ? m.has_val16 : m.has_val32)) {
+    if (!m.blocked || !m.has_scale || !(fp16 ? m.has_off16 || m.has_val16 : m.has_val32)) {
         set_last_error("solve_mixed: matrix has no scaled %d-bit copy", fp16 ? 16 : 32);
         return NBX_ERR_ARG;
     }
