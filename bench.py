@@ -302,7 +302,8 @@ def main():
                           "FP32 CG sweeps (inner rtol 1e-4) in FP64 iterative refinement"
                           if variant == MIXED_CG_RESTART else
                           "FP32 CG with FP64 reliable updates"),
-            mixed_precond_matrix=("fp16 copy without diagonal blocks"
+            mixed_precond_matrix=(("fp16 copy without diagonal blocks"
+                                   + (", 16-bit column deltas" if minfos[-1]["layout"] & 4 else ""))
                                   if minfos[-1]["layout"] & 2 else "full scaled copy"),
             mixed_iters=minfos[-1]["iters"], mixed_inner=minfos[-1]["outer"],
             mixed_rel_res=minfos[-1]["rel_res"], mixed_fallback=minfos[-1]["fallback"],

@@ -345,7 +345,9 @@ typedef struct {
 				 bit 1: the fp16 Chebyshev preconditioner
 				 read the copy without diagonal blocks
 				 (each scaled diagonal entry is 1 in fp16),
-				 else the full fp16 copy                    */
+				 else the full fp16 copy; bit 2: ... with
+				 its block columns as packed 16-bit deltas
+				 (every |w - v| < 32768)                    */
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 

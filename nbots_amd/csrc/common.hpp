@@ -115,6 +115,13 @@ struct Off16 {
     uint32_t* bcol = nullptr;        // [n_blocks]
     _Float16* val = nullptr;         // [4 * n_blocks], 8 B per block as Sell2::val16
     _Float16* od = nullptr;          // [2 * n_nodes]: (K01, K10) scaled, per node
+    // the same columns as row-relative 16-bit deltas, two blocks per word:
+    // pair j of row v (blocks 2j, 2j+1) at pslice_ptr[v/64] + v%64 + 64 j,
+    // low half (w_2j - v), high half (w_2j+1 - v); built when every delta
+    // fits in int16
+    size_t n_pairs = 0;              // stored words, padding included
+    uint32_t* pslice_ptr = nullptr;  // [n_slices + 1]
+    uint32_t* pcol = nullptr;        // [n_pairs]
 };
 
 // Symmetric 2x2-block sliced ELL, for a node matrix whose blocks satisfy

@@ -231,7 +231,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
         status = nbx::solve_mixed(S.K, S.F, S.u, tol, mo, max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
-        info->layout |= st.layout & 2;   // the preconditioner's fp16 copy
+        info->layout |= st.layout & 6;   // the preconditioner's fp16 copy
         info->iters = st.iters;
         info->outer = st.outer;
         info->fp64_iters = st.fp64_iters;

@@ -54,6 +54,7 @@ struct DevMatrix {
     bool off_tried = false;    // o's pattern has been attempted
     bool off_pattern = false;  // o's pattern is built
     bool has_off16 = false;    // o's values reflect the current scale
+    bool off_packed = false;   // o.pcol holds the columns as 16-bit deltas
     unsigned* off_flag = nullptr;
 };
 

@@ -287,6 +287,30 @@ __global__ void k_off_pattern(Sell2 A, Off16 O) {
     for (; k2 < ow; ++k2) O.bcol[o + (size_t)k2 * SLICE] = v;
 }
 
+// O's column deltas w - v, two blocks per word (padding: 0); *bad when one
+// does not fit in int16.
+__global__ void k_off_pack(Off16 O, unsigned* bad) {
+    const uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+    if (v >= O.n_nodes) return;
+    const uint32_t sl = v >> 6;
+    const size_t o = (size_t)O.slice_ptr[sl] + (v & 63);
+    const size_t pb = (size_t)O.pslice_ptr[sl] + (v & 63);
+    const uint32_t pwid = (O.pslice_ptr[sl + 1] - O.pslice_ptr[sl]) / SLICE;
+    const int len = O.rowlen[v];
+    for (uint32_t j = 0; j < pwid; ++j) {
+        uint32_t word = 0;
+        for (int h = 0; h < 2; ++h) {
+            const int k = 2 * (int)j + h;
+            if (k < len) {
+                const int64_t d = (int64_t)O.bcol[o + (size_t)k * SLICE] - (int64_t)v;
+                if (d < -32768 || d > 32767) *bad = 1u;
+                word |= (uint32_t)(uint16_t)(int16_t)d << (16 * h);
+            }
+        }
+        O.pcol[pb + (size_t)j * SLICE] = word;
+    }
+}
+
 // The same fp16 values as k_scale16: off-diagonal blocks into O's ELL, the
 // diagonal block's (K01, K10) into O.od; *bad unless its K00 and K11 are 1.
 __global__ void k_scale16_off(Sell2 A, Off16 O, const double* __restrict__ dis, unsigned* bad) {
@@ -422,12 +446,12 @@ static void sym_free(DevMatrix& m) {
 
 static void off_free(DevMatrix& m) {
     Off16& O = m.o;
-    void* ps[] = {O.slice_ptr, O.rowlen, O.bcol, O.val, O.od, m.off_flag};
+    void* ps[] = {O.slice_ptr, O.rowlen, O.bcol, O.val, O.od, O.pslice_ptr, O.pcol, m.off_flag};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     O = Off16();
     m.off_flag = nullptr;
-    m.off_pattern = m.has_off16 = false;
+    m.off_pattern = m.has_off16 = m.off_packed = false;
 }
 
 void mat_free(DevMatrix& m) {
@@ -761,6 +785,24 @@ static int off_build_pattern(DevMatrix& m, hipStream_t s) {
     hipLaunchKernelGGL(k_off_pattern, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, O);
     NBX_HIP(hipGetLastError());
     m.off_pattern = true;
+    // the columns as packed 16-bit deltas, when every one fits (otherwise,
+    // or without memory for them, the rows read O.bcol)
+    std::vector<uint32_t> psp(ns + 1, 0);
+    for (uint32_t sl = 0; sl < ns; ++sl)
+        psp[sl + 1] = psp[sl] + ((osp[sl + 1] - osp[sl]) / SLICE + 1) / 2 * SLICE;
+    O.n_pairs = psp[ns];
+    if (hipMalloc(&O.pslice_ptr, (ns + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&O.pcol, (O.n_pairs + 1) * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        return NBX_OK;
+    }
+    NBX_HIP(hipMemcpyAsync(O.pslice_ptr, psp.data(), (ns + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s));
+    NBX_HIP(hipMemsetAsync(m.off_flag, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_off_pack, dim3(grid_for(nn)), dim3(BLOCK), 0, s, O, m.off_flag);
+    NBX_HIP(hipGetLastError());
+    NBX_TRY(read_flag(m.off_flag, s, &bad));
+    m.off_packed = bad == 0;
     return NBX_OK;
 }
 

@@ -331,7 +331,9 @@ __device__ __forceinline__ double2 row16d(const Sell2& A, uint32_t v, const floa
 // The same row on the copy without diagonal blocks: the diagonal block
 // [[1, od.x], [od.y, 1]] times the row's own entry xo (which the caller has
 // loaded), then the off-diagonal blocks.
-template <bool NT = true>
+// P: read the columns as packed 16-bit deltas (A.pcol), one 32-bit load per
+// two blocks, instead of A.bcol.
+template <bool NT, bool P>
 __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const float2* __restrict__ x,
                                            float2 xo) {
     const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
@@ -348,15 +350,27 @@ __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const flo
         uint32_t w[UNROLL];
         hv4 a[UNROLL];
         float2 xv[UNROLL];
+        if constexpr (P) {
+            const size_t pb = (size_t)A.pslice_ptr[v >> 6] + (v & 63) + (size_t)(k0 / 2) * SLICE;
+            uint32_t pw[UNROLL / 2];
+#pragma unroll
+            for (int j = 0; j < UNROLL / 2; ++j)
+                if (k0 + 2 * j < len) pw[j] = NT ? nt(&A.pcol[pb + (size_t)j * SLICE])
+                                                 : A.pcol[pb + (size_t)j * SLICE];
+#pragma unroll
+            for (int k = 0; k < UNROLL; ++k)
+                if (k0 + k < len)
+                    w[k] = v + (uint32_t)(int32_t)(int16_t)(uint16_t)(pw[k / 2] >> (16 * (k & 1)));
+        }
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             if (k0 + k < len) {
                 const size_t i = base + (size_t)(k0 + k) * SLICE;
                 if constexpr (NT) {
-                    w[k] = nt(&A.bcol[i]);
+                    if constexpr (!P) w[k] = nt(&A.bcol[i]);
                     a[k] = nt(&val[i]);
                 } else {
-                    w[k] = A.bcol[i];
+                    if constexpr (!P) w[k] = A.bcol[i];
                     a[k] = val[i];
                 }
             }
@@ -386,7 +400,16 @@ __device__ __forceinline__ double2 cheb_row(const Sell2& A, uint32_t v, const fl
 template <bool NT>
 __device__ __forceinline__ double2 cheb_row(const Off16& A, uint32_t v, const float2* __restrict__ x,
                                             float2 xo) {
-    return row16od<NT>(A, v, x, xo);
+    return row16od<NT, false>(A, v, x, xo);
+}
+// the copy without diagonal blocks read through its packed column deltas
+struct Off16P {
+    Off16 O;
+};
+template <bool NT>
+__device__ __forceinline__ double2 cheb_row(const Off16P& A, uint32_t v, const float2* __restrict__ x,
+                                            float2 xo) {
+    return row16od<NT, true>(A.O, v, x, xo);
 }
 
 // y = A x with the scaled copy of BITS-bit values

@@ -929,8 +929,11 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     // Infinity Cache, cached loads keep it there between steps (at 4.2M DOF,
     // 243 MB: 1.48 -> 1.32 s per solve); otherwise the matrix streams.
     const bool off16 = opt.precond_bits == 16 && m.has_off16;
-    const double cheb_mat = off16 ? 12.0 * (double)m.o.n_blocks + 4.0 * (double)nu
-                                  : 12.0 * (double)m.s.n_blocks;
+    const bool packed = off16 && m.off_packed;
+    const double cheb_mat = packed  ? 8.0 * (double)m.o.n_blocks + 4.0 * (double)m.o.n_pairs +
+                                          4.0 * (double)nu
+                            : off16 ? 12.0 * (double)m.o.n_blocks + 4.0 * (double)nu
+                                    : 12.0 * (double)m.s.n_blocks;
     const bool cheb_cached = cheb_mat + 33.0 * (double)nu <= (double)INFINITY_CACHE_BYTES;
     auto precondition = [&](int first) {
         hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv, dis, res,
@@ -939,18 +942,20 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         float2* din = da;
         float2* dout = db;
         for (uint32_t k = 1; k < deg; ++k) {
-            if (off16 && cheb_cached)
-                launch_cheb_step64<false>(k, deg - 1, G, s, m.o, din, dout, res, z, w.st, nu, c1d[k],
-                                          c2d[k]);
+            auto step = [&](const auto& K) {
+                if (cheb_cached)
+                    launch_cheb_step64<false>(k, deg - 1, G, s, K, din, dout, res, z, w.st, nu,
+                                              c1d[k], c2d[k]);
+                else
+                    launch_cheb_step64<true>(k, deg - 1, G, s, K, din, dout, res, z, w.st, nu,
+                                             c1d[k], c2d[k]);
+            };
+            if (packed)
+                step(Off16P{m.o});
             else if (off16)
-                launch_cheb_step64<true>(k, deg - 1, G, s, m.o, din, dout, res, z, w.st, nu, c1d[k],
-                                         c2d[k]);
-            else if (opt.precond_bits == 16 && cheb_cached)
-                launch_cheb_step64<false>(k, deg - 1, G, s, m.s, din, dout, res, z, w.st, nu, c1d[k],
-                                          c2d[k]);
+                step(m.o);
             else if (opt.precond_bits == 16)
-                launch_cheb_step64<true>(k, deg - 1, G, s, m.s, din, dout, res, z, w.st, nu, c1d[k],
-                                         c2d[k]);
+                step(m.s);
             else
                 hipLaunchKernelGGL(k_cheb_step<32>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
                                    w.st, nu, c1[k], c2[k]);
@@ -1003,7 +1008,7 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
-        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0);
+        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0) | (packed ? 4 : 0);
         stats->res0 = sqrt(w.h_st->gg0);
         const uint32_t ran = (w.h_st->k + every - 1) / every;
         NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));

@@ -544,7 +544,7 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
         status = nbx::solve_mixed(A->dev, db, dx, tol, mo, max_iter,
                                   A->ws, A->stream, &st);
         if (status > 1) return status;
-        info->layout |= st.layout & 2;   // the preconditioner's fp16 copy
+        info->layout |= st.layout & 6;   // the preconditioner's fp16 copy
         info->iters = st.iters;
         info->outer = st.outer;
         info->fp64_iters = st.fp64_iters;

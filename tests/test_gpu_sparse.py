@@ -116,7 +116,7 @@ def test_sparse_true_residual_stop(mode):
     oracle.set_bconditions(p, rp, col, val, F)
     A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
     x, info = A.solve(F, mode=mode, tol=1e-10, opts=solve_opts(true_residual=True))
-    assert info["status"] == 0 and info["layout"] == (3 if mode == 2 else 1)
+    assert info["status"] == 0 and info["layout"] == (7 if mode == 2 else 1)
     r = F - oracle.spmv(rp, col, val, x)
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(F)
     A.destroy()
