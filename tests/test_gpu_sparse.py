@@ -122,17 +122,24 @@ def test_sparse_true_residual_stop(mode):
     A.destroy()
 
 
-def test_symmetric_layout_refused_for_asymmetric_values():
-    """One off-diagonal entry changed on one side: the symmetry check fails,
-    the solve runs on the full layout (layout 0), SpMV stays exact, and after
-    the values are restored the next solve uses the symmetric copy again."""
+@pytest.mark.parametrize("where", ["off_diagonal_block", "diagonal_block"])
+def test_symmetric_layout_refused_for_asymmetric_values(where):
+    """One entry changed on one side, in an off-diagonal block or in the
+    x-y coupling of a diagonal block (which the symmetric layout stores
+    once): the symmetry check fails, the solve runs on the full layout
+    (layout 0), SpMV stays exact, and after the values are restored the next
+    solve uses the symmetric copy again."""
     import oracle
     g = load_golden("cant_8x4_jit")
     A = _sparse(g)
     rp, col, val = g["rowptr"], g["col"], g["val"].copy()
     r = 6
-    j = [k for k in range(rp[r], rp[r + 1]) if col[k] // 2 != r // 2 and val[k] != 0.0][0]
-    val[j] *= 1.0 + 2.0 ** -40
+    if where == "off_diagonal_block":
+        j = [k for k in range(rp[r], rp[r + 1]) if col[k] // 2 != r // 2 and val[k] != 0.0][0]
+        val[j] *= 1.0 + 2.0 ** -40
+    else:
+        j = [k for k in range(rp[r], rp[r + 1]) if col[k] == r + 1][0]
+        val[j] = np.nextafter(val[j], np.inf)
     A.import_values(val)
     _, info = A.solve(g["F"], mode=0, tol=1e-8)
     assert info["layout"] == 0
