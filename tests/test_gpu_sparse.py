@@ -315,3 +315,44 @@ def test_plain_cg_edge_cases():
     x, st, it, tr = A.solve_cg(np.zeros(1), max_iter=10, tol=1e-8)
     assert (st, it, tr) == (0, 0, 0.0) and x[0] == 0.0
     A.destroy()
+
+
+def test_mixed_solve_wide_node_numbering():
+    """Nodes numbered at random (512x128 cantilever, 66,177 nodes): block
+    columns lie up to 66k rows from the diagonal, beyond the 16-bit deltas,
+    so the fp16 preconditioner reads its 32-bit columns (layout 3, not 7).
+    The mixed solve still meets the tolerance on the oracle's residual and
+    agrees with the solve in the original numbering."""
+    import scipy.sparse as sp
+    import oracle
+    import nbots_amd as nb
+    from nbots_amd import synth
+    from nbots_amd.fem import solve_opts
+    p = synth.cantilever_traction(512, 128)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    N = p.N
+    perm = np.random.default_rng(7).permutation(p.N_nod)        # old node -> new node
+    dof = (2 * perm[:, None] + np.arange(2)).ravel()            # old dof -> new dof
+    K = sp.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(N, N)).tocoo()
+    Kp = sp.csr_matrix((K.data, (dof[K.row], dof[K.col])), shape=(N, N))
+    Kp.sort_indices()
+    rpp, colp, valp = Kp.indptr.astype(np.uint32), Kp.indices.astype(np.uint32), Kp.data.copy()
+    assert valp.size == val.size
+    rows = np.repeat(np.arange(N), np.diff(Kp.indptr))
+    assert np.abs(colp.astype(np.int64) // 2 - rows // 2).max() >= 32768
+    Fp = np.empty(N)
+    Fp[dof] = F
+    opts = solve_opts(true_residual=True)
+    A = nb.Sparse.from_csr(rpp, colp, valp, vars_per_node=2)
+    x, info = A.solve(Fp, mode=nb.SOLVE_MIXED, tol=1e-8, opts=opts)
+    assert info["status"] == 0 and info["layout"] == 3, info
+    r = Fp - oracle.spmv(rpp, colp, valp, x)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Fp)
+    A.destroy()
+    A0 = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    x0, info0 = A0.solve(F, mode=nb.SOLVE_MIXED, tol=1e-8, opts=opts)
+    assert info0["status"] == 0 and info0["layout"] == 7, info0
+    assert np.linalg.norm(x[dof] - x0) <= 1e-6 * np.linalg.norm(x0)
+    A0.destroy()
