@@ -139,12 +139,22 @@ __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const doubl
 // blocks are taken UL and UU at a time; few registers per row keep many
 // waves resident, which hides the dependent loads better than longer
 // batches (4 and 4: 74 VGPRs, 6 waves per SIMD).
-template <int UL, int UU>
+// NT: non-temporal loads of the matrix stream (when the working set exceeds
+// the Infinity Cache); otherwise ordinary loads, so that the matrix stays
+// cached from one product to the next.
+template <bool NT, class T>
+__device__ __forceinline__ T ldm(const T* p) {
+    if constexpr (NT)
+        return nt(p);
+    else
+        return *p;
+}
+template <int UL, int UU, bool NT = true>
 __device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const double2* __restrict__ x) {
     const dv2* __restrict__ pa = reinterpret_cast<const dv2*>(Y.u_val);
     const dv2* __restrict__ pb = reinterpret_cast<const dv2*>(Y.u_val + 2 * Y.n_upper);
-    const dv2 d01 = nt(reinterpret_cast<const dv2*>(Y.d_val) + v);
-    const double d11 = nt(Y.d_val + 2 * (size_t)Y.n_nodes + v);
+    const dv2 d01 = ldm<NT>(reinterpret_cast<const dv2*>(Y.d_val) + v);
+    const double d11 = ldm<NT>(Y.d_val + 2 * (size_t)Y.n_nodes + v);
     const double2 xd = x[v];
     double s0 = 0.0, s1 = 0.0;
     {
@@ -157,7 +167,7 @@ __device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const doubl
             double2 xv[UL];
 #pragma unroll
             for (int k = 0; k < UL; ++k)
-                if (k0 + k < len) e[k] = nt(&li[base + (size_t)(k0 + k) * SLICE]);
+                if (k0 + k < len) e[k] = ldm<NT>(&li[base + (size_t)(k0 + k) * SLICE]);
 #pragma unroll
             for (int k = 0; k < UL; ++k) {
                 if (k0 + k < len) {
@@ -194,9 +204,9 @@ __device__ __forceinline__ double2 row64s(const Sym2& Y, uint32_t v, const doubl
             for (int k = 0; k < UU; ++k) {
                 if (k0 + k < len) {
                     const size_t i = base + (size_t)(k0 + k) * SLICE;
-                    w[k] = nt(&Y.u_bcol[i]);
-                    a[k] = nt(&pa[i]);
-                    b[k] = nt(&pb[i]);
+                    w[k] = ldm<NT>(&Y.u_bcol[i]);
+                    a[k] = ldm<NT>(&pa[i]);
+                    b[k] = ldm<NT>(&pb[i]);
                 }
             }
 #pragma unroll
@@ -434,6 +444,14 @@ struct OpSym {
     using V = double2;
     __device__ __forceinline__ double2 row(uint32_t v, const double2* __restrict__ x) const {
         return row64s<4, 4>(Y, v, x);
+    }
+};
+// the same with cached matrix loads (working set within the Infinity Cache)
+struct OpSymC {
+    Sym2 Y;
+    using V = double2;
+    __device__ __forceinline__ double2 row(uint32_t v, const double2* __restrict__ x) const {
+        return row64s<4, 4, false>(Y, v, x);
     }
 };
 

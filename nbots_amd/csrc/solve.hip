@@ -695,8 +695,11 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
                                                8.0 * (double)m.y.n_lower
                                          : 36.0 * (double)m.s.n_blocks;
     const bool stream = mat_bytes + 40.0 * (double)m.n > (double)INFINITY_CACHE_BYTES;
-    if (m.blocked && m.has_sym)
+    if (m.blocked && m.has_sym && stream)
         rc = pcg_run(OpSym{m.y}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol,
+                     stream, w, s, stats);
+    else if (m.blocked && m.has_sym)
+        rc = pcg_run(OpSymC{m.y}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol,
                      stream, w, s, stats);
     else if (m.blocked)
         rc = pcg_run(OpS2{m.s}, m.s.n_nodes, diag, (const double2*)b, (double2*)x, max_iter, tol,
