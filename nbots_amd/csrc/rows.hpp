@@ -133,12 +133,13 @@ __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const doubl
 // of the stored upper block (w, v), the diagonal block (three stored values,
 // read with x_v in whole-wave coalesced loads issued first), then the upper
 // blocks (w > v, ascending); the same products in the same order as row64.
-// The transposed
-// blocks belong to rows that nearby workgroups of the same XCD stream at
-// about the same time, so their loads are served by L2.  Lower and upper
-// blocks are taken UL and UU at a time; few registers per row keep many
-// waves resident, which hides the dependent loads better than longer
-// batches (4 and 4: 74 VGPRs, 6 waves per SIMD).
+// The transposed blocks belong to rows that nearby workgroups of the same
+// XCD stream at about the same time, so their loads are served by L2.
+// Lower and upper blocks are taken UL and UU at a time; few registers per
+// row keep many waves resident, which hides the dependent loads better than
+// longer batches (4 and 4: 84 VGPRs, 5 waves per SIMD; issuing the diagonal
+// loads after the lower blocks gives 74 and 6 waves, but exposes their
+// latency, which costs more where the matrix is read from cache).
 // NT: non-temporal loads of the matrix stream (when the working set exceeds
 // the Infinity Cache); otherwise ordinary loads, so that the matrix stays
 // cached from one product to the next.
