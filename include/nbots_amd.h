@@ -9,11 +9,13 @@
  *     reference's.  The heavy work (SpMV, Jacobi-PCG, assembly, boundary
  *     conditions, strain) runs on the GPU.
  *
- *  2. nbx_* extensions that the reference lacks and the north-star workloads
- *     need: a device-resident FEM system (per-element modulus, reassembly
- *     without re-uploading the mesh), a relative tolerance, mixed precision
- *     (FP32 CG inside FP64 iterative refinement), SIMP compliance and
- *     sensitivities, the adjoint solve, density filter and OC update.
+ *  2. nbx_* extensions that the reference lacks and the solve, mixed
+ *     precision and topology-optimisation workloads need: a device-resident
+ *     FEM system (per-element modulus, reassembly without re-uploading the
+ *     mesh), a relative tolerance, mixed precision (an FP64 flexible CG with
+ *     an fp16/FP32 Chebyshev preconditioner, or FP32 CG inside FP64
+ *     iterative refinement), SIMP compliance and sensitivities, the adjoint
+ *     solve, density filter and OC update.
  *     These never call exit(); they return an nbx status code and
  *     nbx_last_error() describes the failure.
  *

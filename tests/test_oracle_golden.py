@@ -60,7 +60,7 @@ def test_digests(case):
         assert sha(res["stress"]) == str(g["stress_sha"])
 
 
-def test_c1_matches_survey_sizes():
+def test_c1_cantilever_sizes():
     """64x32 grid: 2,145 nodes, 4,096 T3, 4,290 DOF, nnz = 4*(nodes + 2*edges) = 58,500."""
     g = load_golden("c1_cantilever_64x32")
     p = problem_from_golden(g)
