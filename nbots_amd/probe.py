@@ -9,6 +9,9 @@ matrix once and gathers x; the copy reads and writes exactly --copy-mib MiB,
 so the ratio of the copy's counter reading to its size calibrates the
 counter for the SpMV.  The probe prints the algorithmic bytes of one SpMV
 launch for comparison.  No timing is taken here.
+
+--mixed runs --iters outer iterations of the default mixed-precision solve
+instead, for the counters of its Chebyshev steps (k_cheb_step64).
 """
 from __future__ import annotations
 
@@ -44,6 +47,18 @@ def sym_spmv_bytes(n_nodes: int, n_valid: int) -> int:
             + 2 * (2 * n_nodes) * 8)
 
 
+def cheb_step_bytes(n_nodes: int, n_valid: int, adds_z: bool) -> int:
+    """Algorithmic bytes of one Chebyshev step on the fp16 copy without
+    diagonal blocks: per off-diagonal block 8 B of values and a 2-B packed
+    column; per node the diagonal pair (4 B), the row length (1 B), the d
+    gather (8 B), the residual read and write (16 B), the new d (8 B) and,
+    on the steps that add to z, its read and write (16 B); per 64-node slice
+    two offsets."""
+    off = n_valid - n_nodes
+    per_node = 4 + 1 + 8 + 16 + 8 + (16 if adds_z else 0)
+    return off * 10 + n_nodes * per_node + 8 * ((n_nodes + 63) // 64)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nx", type=int, default=2048)
@@ -51,11 +66,12 @@ def main():
     ap.add_argument("--iters", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--mixed", action="store_true")
     args = ap.parse_args()
 
     import torch
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, SOLVE_FP64
+    from nbots_amd.fem import Fem, SOLVE_FP64, SOLVE_MIXED
 
     p = synth.cantilever_traction(args.nx, args.ny, seed=args.seed)
     f = Fem.from_problem(p)
@@ -64,8 +80,14 @@ def main():
     print(f"[probe] {args.nx}x{args.ny} cantilever: {N} DOF, {n_valid} blocks, "
           f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64, full layout), "
           f"{sym_spmv_bytes(n_nodes, n_valid)} (symmetric layout)", flush=True)
-    info = f.solve(SOLVE_FP64, tol=0.0, max_iter=args.iters)
-    print(f"[probe] FP64 PCG: {info['iters']} iterations, layout {info['layout']}", flush=True)
+    if args.mixed:
+        print(f"[probe] Chebyshev step algorithmic bytes {cheb_step_bytes(n_nodes, n_valid, True)} "
+              f"(adding to z), {cheb_step_bytes(n_nodes, n_valid, False)} (not)", flush=True)
+        info = f.solve(SOLVE_MIXED, tol=0.0, max_iter=args.iters)
+        print(f"[probe] mixed: {info['iters']} outer iterations, layout {info['layout']}", flush=True)
+    else:
+        info = f.solve(SOLVE_FP64, tol=0.0, max_iter=args.iters)
+        print(f"[probe] FP64 PCG: {info['iters']} iterations, layout {info['layout']}", flush=True)
     f.destroy()
 
     # b = 2 a: an elementwise kernel that reads and writes every byte once
