@@ -93,6 +93,9 @@ __device__ __forceinline__ double2 subu(double2 a, double2 b) {
 // at a time with every independent load (column, values) issued before the
 // dependent gathers of x, so a wavefront keeps many requests in flight.
 constexpr int UNROLL = 8;
+// blocks per batch of a row without its diagonal block (6 covers an interior
+// row of a T3 mesh; 8 would not fit in the registers of 8 waves per SIMD)
+constexpr int UNROLL_OD = 6;
 
 __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const double2* __restrict__ x) {
     const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
@@ -357,24 +360,24 @@ __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const flo
         od = reinterpret_cast<const hv2*>(A.od)[v];
     double s0 = (double)xo.x + (double)od.x * (double)xo.y;
     double s1 = (double)od.y * (double)xo.x + (double)xo.y;
-    for (int k0 = 0; k0 < len; k0 += UNROLL) {
-        uint32_t w[UNROLL];
-        hv4 a[UNROLL];
-        float2 xv[UNROLL];
+    for (int k0 = 0; k0 < len; k0 += UNROLL_OD) {
+        uint32_t w[UNROLL_OD];
+        hv4 a[UNROLL_OD];
+        float2 xv[UNROLL_OD];
         if constexpr (P) {
             const size_t pb = (size_t)A.pslice_ptr[v >> 6] + (v & 63) + (size_t)(k0 / 2) * SLICE;
-            uint32_t pw[UNROLL / 2];
+            uint32_t pw[UNROLL_OD / 2];
 #pragma unroll
-            for (int j = 0; j < UNROLL / 2; ++j)
+            for (int j = 0; j < UNROLL_OD / 2; ++j)
                 if (k0 + 2 * j < len) pw[j] = NT ? nt(&A.pcol[pb + (size_t)j * SLICE])
                                                  : A.pcol[pb + (size_t)j * SLICE];
 #pragma unroll
-            for (int k = 0; k < UNROLL; ++k)
+            for (int k = 0; k < UNROLL_OD; ++k)
                 if (k0 + k < len)
                     w[k] = v + (uint32_t)(int32_t)(int16_t)(uint16_t)(pw[k / 2] >> (16 * (k & 1)));
         }
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
+        for (int k = 0; k < UNROLL_OD; ++k) {
             if (k0 + k < len) {
                 const size_t i = base + (size_t)(k0 + k) * SLICE;
                 if constexpr (NT) {
@@ -387,10 +390,10 @@ __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const flo
             }
         }
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k)
+        for (int k = 0; k < UNROLL_OD; ++k)
             if (k0 + k < len) xv[k] = x[w[k]];
 #pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
+        for (int k = 0; k < UNROLL_OD; ++k) {
             if (k0 + k < len) {
                 s0 = s0 + (double)a[k].x * (double)xv[k].x;
                 s0 = s0 + (double)a[k].y * (double)xv[k].y;

@@ -471,7 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_cheb_step(Sell2 K, const float2* __re
 // M: the fp16 matrix, Sell2 (full copy) or Off16 (without diagonal blocks).
 enum { Z_SKIP = 0, Z_PAIR = 1, Z_ONE = 2 };
 template <bool NT, int ZM, class M>
-__global__ __launch_bounds__(BLOCK) void k_cheb_step64(M K, const float2* __restrict__ din,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cheb_step64(M K, const float2* __restrict__ din,
                                                        float2* __restrict__ dout,
                                                        float2* __restrict__ res,
                                                        float2* __restrict__ z, PcgState* st,
