@@ -140,9 +140,11 @@ __device__ __forceinline__ double2 row64(const Sell2& A, uint32_t v, const doubl
 // XCD stream at about the same time, so their loads are served by L2.
 // Lower and upper blocks are taken UL and UU at a time; few registers per
 // row keep many waves resident, which hides the dependent loads better than
-// longer batches (4 and 4: 84 VGPRs, 5 waves per SIMD; issuing the diagonal
-// loads after the lower blocks gives 74 and 6 waves, but exposes their
-// latency, which costs more where the matrix is read from cache).
+// longer batches.  An interior row of a T3 mesh has 3 lower and 3 upper
+// blocks: the streamed product takes them 3 and 3 (70 VGPRs, 7 waves per
+// SIMD), the cached one 4 and 4 (84 VGPRs, 5 waves), which measured faster
+// where the matrix is read from cache.  Issuing the diagonal loads after the
+// lower blocks instead saves registers but exposes their latency.
 // NT: non-temporal loads of the matrix stream (when the working set exceeds
 // the Infinity Cache); otherwise ordinary loads, so that the matrix stays
 // cached from one product to the next.
@@ -447,7 +449,7 @@ struct OpSym {
     Sym2 Y;
     using V = double2;
     __device__ __forceinline__ double2 row(uint32_t v, const double2* __restrict__ x) const {
-        return row64s<4, 4>(Y, v, x);
+        return row64s<3, 3>(Y, v, x);
     }
 };
 // the same with cached matrix loads (working set within the Infinity Cache)
