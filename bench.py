@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
 # 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md;
 # keyed by (config, symmetric layout)
-PMC_TRAFFIC = {("c3", False): 599187989, ("c3", True): 407530226}
+PMC_TRAFFIC = {("c3", False): 599187989, ("c3", True): 411256645}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
                                       "right-edge shear traction"),
