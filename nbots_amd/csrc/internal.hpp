@@ -23,7 +23,7 @@ struct PcgState {
     double conv_rr;                // r.r at which the true residual should meet tol
     uint32_t updates, pad;
     double gg0;                    // squared residual norm at the start (fresh b - A x)
-    double alpha_prev;             // alpha of the iteration before (paired x updates)
+    double alpha_hist[4];          // alpha of iteration j at [j % 4] (grouped x updates)
 };
 
 // A device matrix: 2-dof blocked sliced ELL when the pattern allows it,
@@ -62,7 +62,7 @@ struct DevMatrix {
 // Per-matrix solver scratch (device).  Sized for n scalar rows.
 struct Workspace {
     uint32_t n = 0;
-    double* v64[7] = {};       // FP64 vectors
+    double* v64[9] = {};       // FP64 vectors
     float* v32[4] = {};        // FP32 vectors
     double* partials = nullptr;
     float* partials32 = nullptr;

@@ -98,15 +98,15 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_A(Op op, const typename Op::V* __
 // values, so the reads stay in it whatever the state says; a finished
 // solve ignores the sums.
 __global__ __launch_bounds__(FIN) void k_fin_A(PcgState* st, const double* partials, unsigned np) {
-    const uint32_t done = st->done;
-    const double rz = st->rz, alpha_prev = st->alpha;
+    const uint32_t done = st->done, j = st->k;   // this is iteration j
+    const double rz = st->rz;
     double tot[1];
     fin_sum<1>(partials, np, tot);
     if (done) return;
     if (threadIdx.x == 0) {
         st->pw = tot[0];
-        st->alpha_prev = alpha_prev;
         st->alpha = rz / tot[0];
+        st->alpha_hist[j & 3u] = st->alpha;
     }
 }
 
@@ -152,30 +152,48 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_B(V* __restrict__ g, const V* __r
     put_partials<2>(acc, partials);
 }
 
-// p_next = -q + beta p  (cg_precond_jacobi.c:72-74) into the other of two
-// direction buffers, and x += alpha p (:62) for two iterations at a time:
-// iteration j reads p_j from buffer j % 2 and writes p_{j+1} into the other
-// one, which still holds p_{j-1}.  An even iteration leaves x alone; an odd
-// one applies x = (x + alpha_{j-1} p_{j-1}) + alpha_j p_j, the same two
-// roundings in the same order as one update per iteration, so x is
-// bit-identical, and x is read and written every other iteration only.
+// The four direction buffers of the FP64 PCG: p_j lives in p[j % 4].
+template <class V>
+struct PBufs {
+    V* p[4];
+};
+
+// p_{j+1} = -q + beta p_j  (cg_precond_jacobi.c:72-74) into the next of four
+// direction buffers, and x += alpha p (:62) for four iterations at a time:
+// iteration j reads p_j from buffer j % 4 and writes p_{j+1} into buffer
+// (j + 1) % 4, which still holds p_{j-3}.  Iterations j % 4 = 0, 1, 2 leave
+// x alone; iteration j % 4 = 3 (FLUSH) applies
+// x = (((x + a_{j-3} p_{j-3}) + a_{j-2} p_{j-2}) + a_{j-1} p_{j-1}) + a_j p_j,
+// the same roundings in the same order as one update per iteration, so x is
+// bit-identical, and x is read and written every fourth iteration only.
 // Once the stop test has fired this kernel no longer runs, and k_pcg_tail
 // applies the updates still pending.
-template <class V, bool NT, bool PAIR>
-__global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, const V* __restrict__ p,
-                                                  V* __restrict__ p_next,
+template <class V, bool NT, bool FLUSH>
+__global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, PBufs<V> P, uint32_t j,
                                                   const V* __restrict__ g,
                                                   const V* __restrict__ diag, PcgState* st,
                                                   uint32_t n) {
     if (st->done) return;
     const double alpha = st->alpha, beta = st->beta;
-    const double alpha_prev = PAIR ? st->alpha_prev : 0.0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if constexpr (FLUSH) {
+        a0 = st->alpha_hist[0];
+        a1 = st->alpha_hist[1];
+        a2 = st->alpha_hist[2];
+    }
+    const V* __restrict__ p = P.p[j & 3u];
+    V* __restrict__ p_next = P.p[(j + 1) & 3u];
     const Chunk c = my_chunk(n);
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
         const V pu = ldv<NT>(&p[u]);
-        if constexpr (PAIR) {
-            const V pp = ldv<NT>(&p_next[u]);   // p_{j-1}, read before p_{j+1} replaces it
-            stv<NT>(&x[u], axpyu(axpyu(ldv<NT>(&x[u]), alpha_prev, pp), alpha, pu));
+        if constexpr (FLUSH) {
+            // j % 4 = 3: p_{j-3} is in p_next (read before p_{j+1} replaces
+            // it), p_{j-2} in P.p[1], p_{j-1} in P.p[2]
+            V xv = ldv<NT>(&x[u]);
+            xv = axpyu(xv, a0, ldv<NT>(&p_next[u]));
+            xv = axpyu(xv, a1, ldv<NT>(&P.p[1][u]));
+            xv = axpyu(xv, a2, ldv<NT>(&P.p[2][u]));
+            stv<NT>(&x[u], axpyu(xv, alpha, pu));
         }
         const V q = divu(ldv<NT>(&g[u]), ldv<NT>(&diag[u]));
         p_next[u] = pupd(q, beta, pu);   // gathered by the next SpMV: cached store
@@ -183,24 +201,20 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_C(V* __restrict__ x, const V* __r
 }
 
 // The x updates still pending when the loop has stopped (cg_precond_jacobi.c:
-// 62): iteration J = k - 1 did not reach k_pcg_C, so alpha_J p_J is pending,
-// and so is alpha_{J-1} p_{J-1} when J is odd (iteration J - 1 was even).
-// p_j is in p_buf[j % 2].  Nothing if no iteration ran.
+// 62): iteration J = k - 1 did not reach k_pcg_C, and the updates of the
+// iterations 4 floor(J / 4) .. J are pending, applied in ascending order.
+// Nothing if no iteration ran.
 template <class V>
-__global__ __launch_bounds__(BLOCK) void k_pcg_tail(V* __restrict__ x, const V* __restrict__ p0,
-                                                    const V* __restrict__ p1, const PcgState* st,
-                                                    uint32_t n) {
+__global__ __launch_bounds__(BLOCK) void k_pcg_tail(V* __restrict__ x, PBufs<V> P,
+                                                    const PcgState* st, uint32_t n) {
     const uint32_t k = st->k;
     if (k == 0) return;
-    const uint32_t J = k - 1;
-    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
-    const V* __restrict__ pj = (J & 1u) ? p1 : p0;
-    const V* __restrict__ pprev = (J & 1u) ? p0 : p1;
+    const uint32_t last = (k - 1) & 3u;   // buffers 0 .. last hold p of the pending ones
     const Chunk c = my_chunk(n);
     for (size_t u = c.beg + threadIdx.x; u < c.end; u += BLOCK) {
         V xv = x[u];
-        if (J & 1u) xv = axpyu(xv, alpha_prev, pprev[u]);
-        x[u] = axpyu(xv, alpha, pj[u]);
+        for (uint32_t i = 0; i <= last; ++i) xv = axpyu(xv, st->alpha_hist[i], P.p[i][u]);
+        x[u] = xv;
     }
 }
 
@@ -623,7 +637,7 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
     V* g = (V*)w.v64[0];
     V* p = (V*)w.v64[1];
     V* wv = (V*)w.v64[2];
-    V* pbuf[2] = {p, (V*)w.v64[6]};   // p_j lives in pbuf[j % 2] (k_pcg_C)
+    const PBufs<V> pbuf{{p, (V*)w.v64[6], (V*)w.v64[7], (V*)w.v64[8]}};   // p_j in [j % 4]
     const V* diag = (const V*)diag_in;
     PcgState h{};
     h.tol2 = tol * tol;
@@ -643,8 +657,8 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             // iteration j = launched + i (kernels after the stop return at once)
             const uint32_t j = launched + i;
-            V* pj = pbuf[j & 1u];
-            V* pn = pbuf[(j + 1) & 1u];
+            const V* pj = pbuf.p[j & 3u];
+            const bool flush = (j & 3u) == 3u;
             hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, pj, wv, w.st, w.partials,
                                nu);
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
@@ -653,22 +667,22 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
                 hipLaunchKernelGGL((k_pcg_B<V, true>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
                                    w.partials, nu);
                 hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-                if (j & 1u)
-                    hipLaunchKernelGGL((k_pcg_C<V, true, true>), dim3(G), dim3(BLOCK), 0, s, x, pj, pn,
+                if (flush)
+                    hipLaunchKernelGGL((k_pcg_C<V, true, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf, j,
                                        g, diag, w.st, nu);
                 else
-                    hipLaunchKernelGGL((k_pcg_C<V, true, false>), dim3(G), dim3(BLOCK), 0, s, x, pj, pn,
-                                       g, diag, w.st, nu);
+                    hipLaunchKernelGGL((k_pcg_C<V, true, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
+                                       j, g, diag, w.st, nu);
             } else {
                 hipLaunchKernelGGL((k_pcg_B<V, false>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag,
                                    w.st, w.partials, nu);
                 hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-                if (j & 1u)
-                    hipLaunchKernelGGL((k_pcg_C<V, false, true>), dim3(G), dim3(BLOCK), 0, s, x, pj,
-                                       pn, g, diag, w.st, nu);
+                if (flush)
+                    hipLaunchKernelGGL((k_pcg_C<V, false, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
+                                       j, g, diag, w.st, nu);
                 else
-                    hipLaunchKernelGGL((k_pcg_C<V, false, false>), dim3(G), dim3(BLOCK), 0, s, x, pj,
-                                       pn, g, diag, w.st, nu);
+                    hipLaunchKernelGGL((k_pcg_C<V, false, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
+                                       j, g, diag, w.st, nu);
             }
         }
         NBX_HIP(hipGetLastError());
@@ -681,7 +695,7 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
         }
         batch = batch < 64 ? batch * 2 : 64;
     }
-    hipLaunchKernelGGL(k_pcg_tail<V>, dim3(G), dim3(BLOCK), 0, s, x, pbuf[0], pbuf[1], w.st, nu);
+    hipLaunchKernelGGL(k_pcg_tail<V>, dim3(G), dim3(BLOCK), 0, s, x, pbuf, w.st, nu);
     NBX_HIP(hipGetLastError());
     NBX_HIP(hipEventRecord(w.ev[1], s));
     NBX_HIP(hipEventSynchronize(w.ev[1]));
