@@ -60,11 +60,11 @@ def pcg_iter_bytes(n_nodes, n_blocks, prec, sym=False):
     """Algorithmic bytes of one CG iteration as implemented: SpMV kernel
     (above) + update kernel (FP64: read g,w,diag, write g = 4 vectors;
     FP32 scaled: read y,p,r,w, write y,r = 6) + direction kernel (FP64: read
-    p,g,diag and write the next p = 4 vectors, and on every other iteration
-    also read the previous p and read and write x = 7; FP32: read r,p,
+    p,g,diag and write the next p = 4 vectors, and on every fourth iteration
+    also read the three previous p and read and write x = 9; FP32: read r,p,
     write p = 3)."""
     sv = 8 if prec == 64 else 4
-    nv = 9.5 if prec == 64 else 9
+    nv = 9.25 if prec == 64 else 9
     spmv = sym_spmv_bytes(n_nodes, n_blocks) if sym else spmv_bytes(n_nodes, n_blocks, prec)
     return spmv + nv * (2 * n_nodes) * sv
 
