@@ -186,12 +186,14 @@ def test_pcg_reference_semantics(case):
     A.destroy()
 
 
-@pytest.mark.parametrize("max_iter", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("max_iter", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_pcg_warm_start_cuts(max_iter):
-    """From a nonzero x0, cut after 0..5 iterations: the device's x (its last
-    x += alpha p is applied after the stop test), niter, status and
-    tol_reached follow the oracle's restatement of cg_precond_jacobi.c;
-    with 0 iterations x comes back bit for bit."""
+    """From a nonzero x0, cut after 0..9 iterations: the device's x, niter,
+    status and tol_reached follow the oracle's restatement of
+    cg_precond_jacobi.c; with 0 iterations x comes back bit for bit.  The
+    device applies x += alpha p four iterations at a time and the pending
+    one to four terms after the stop test, so these cuts cover every
+    remainder, before and after a group has been applied."""
     import oracle
     g = load_golden("cant_8x4_jit")
     A = _sparse(g)
