@@ -29,6 +29,7 @@ STREAM_JITTER = 1
 STREAM_MODULUS = 2
 STREAM_DENSITY = 3
 STREAM_PROBE = 4
+STREAM_PROJECTION = 16      # streams 16.. : test projections w_k, one per k
 DEFAULT_SEED = 20261202
 
 
@@ -203,6 +204,16 @@ def mbb(nx: int, ny: int, seed: int = DEFAULT_SEED, rho=None,
                    dv=_bc([1], [0, 1], [0.0, 0.0]),
                    nv=_bc([3], [0, 1], [0.0, -1.0]),
                    name=f"mbb_{nx}x{ny}")
+
+
+def projections(n: int, k: int = 8, seed: int = DEFAULT_SEED) -> np.ndarray:
+    """k vectors of length n with entries U[-1, 1), row i from stream
+    STREAM_PROJECTION + i.  With a solution's norm, F.u and its samples, the
+    values w_i.u pin the whole vector: a difference e changes each w_i.u by
+    a random amount of standard deviation ||e|| / sqrt(3), the same scale as
+    w_i.u itself has for u, so |w_i.(u - v)| / ||u|| measures ||u - v|| / ||u||
+    along random directions."""
+    return np.stack([2.0 * uniform(seed, STREAM_PROJECTION + i, n) - 1.0 for i in range(k)])
 
 
 def elem_centroids(p: Problem):
