@@ -42,11 +42,9 @@ typedef struct {
 	double **wij; /* NULL for equal weights on edges */
 } nb_graph_t;
 
-/* headers/nb/graph_bot/graph.h:33-37 */
-uint32_t nb_graph_get_memsize(void);
-void nb_graph_init(nb_graph_t *graph);
-void nb_graph_finish(nb_graph_t *graph);
-void nb_graph_clear(nb_graph_t *graph);
+/* The graph's functions (graph.h:33-37) belong to libnbots: this library
+ * only reads the struct (nb_sparse_create).  libnbx_objects.so defines them
+ * for callers without libnbots. */
 
 /* ======================================================================== */
 /* Sparse matrix (headers/nb/solver_bot/sparse/sparse.h:9-78)                */
@@ -138,76 +136,68 @@ typedef enum { NB_DIRICHLET, NB_NEUMANN } nb_bcond_id;               /* bcond.h:
 typedef enum { NB_BC_ON_POINT, NB_BC_ON_SEGMENT } nb_bcond_where;    /* bcond.h:12-15 */
 typedef struct nb_bcond_s nb_bcond_t;                                /* bcond.h:17 */
 
-/* headers/nb/geometric_bot/mesh/mesh2D.h:16-25 (triangle meshes only)    */
-nb_mesh2D_t *nb_mesh2D_create(nb_mesh2D_type type);
-void nb_mesh2D_destroy(nb_mesh2D_t *mesh);
-uint32_t nb_mesh2D_get_N_invtx(const nb_mesh2D_t *mesh);          /* mesh2D.h:30 */
-uint32_t nb_mesh2D_get_N_insgm(const nb_mesh2D_t *mesh);          /* mesh2D.h:31 */
-uint32_t nb_mesh2D_get_N_nodes(const nb_mesh2D_t *mesh);          /* mesh2D.h:32 */
-uint32_t nb_mesh2D_get_N_elems(const nb_mesh2D_t *mesh);          /* mesh2D.h:34 */
-double nb_mesh2D_node_get_x(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:36 */
-double nb_mesh2D_node_get_y(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:37 */
-uint32_t nb_mesh2D_elem_get_adj(const nb_mesh2D_t *mesh, uint32_t elem_id,
-				uint8_t node_id);                 /* mesh2D.h:72 */
-uint32_t nb_mesh2D_get_invtx(const nb_mesh2D_t *mesh, uint32_t id); /* mesh2D.h:79 */
-uint32_t nb_mesh2D_insgm_get_N_nodes(const nb_mesh2D_t *mesh, uint32_t id); /* :80 */
-uint32_t nb_mesh2D_insgm_get_node(const nb_mesh2D_t *mesh, uint32_t sgm_id,
-				  uint32_t node_id);              /* mesh2D.h:82 */
-/* nbx extension: the reference fills triangle meshes only through its
- * Delaunay tessellator (nb_mesh2D_load_from_tessellator2D, mesh2D.h:93);
- * this sets one from arrays: nodes nod[2*N_nod], triangles adj[3*N_elem]
- * (counter-clockwise), input vertices vtx[N_vtx] (model vertex -> node),
- * input segments as node lists nod_x_sgm[sgm_ptr[s]..sgm_ptr[s+1]). */
-int nbx_mesh2D_set_trg(nb_mesh2D_t *mesh, uint32_t N_nod, const double *nod,
-		       uint32_t N_elem, const uint32_t *adj,
-		       uint32_t N_vtx, const uint32_t *vtx,
-		       uint32_t N_sgm, const uint32_t *sgm_ptr,
-		       const uint32_t *nod_x_sgm);
+typedef struct nb_bcond_iter_s nb_bcond_iter_t;                      /* bcond_iter.h:9 */
+typedef struct nb_fem_implicit_s nb_fem_implicit_t;          /* static_damage2D.h:13 */
 
-/* headers/nb/pde_bot/finite_element/element.h:15-26 */
-nb_fem_elem_t *nb_fem_elem_create(nb_elem_id type);
-void nb_fem_elem_destroy(nb_fem_elem_t *elemtype);
+/* ------------------------------------------------------------------------ */
+/* Reference objects this library READS and does not define.               */
+/*                                                                          */
+/* The FEM drivers below take a mesh, an element type, a material, boundary */
+/* conditions and (damage) implicit-scheme parameters that the caller built */
+/* with libnbots.  This library reads them only through the accessors       */
+/* listed here, which it imports: in a drop-in link (-lnbots_amd -lnbots)   */
+/* they resolve to libnbots' own definitions, so the objects keep the       */
+/* reference's layout end to end.  Callers without libnbots load            */
+/* libnbx_objects.so (include/nbx_objects.h), which defines the same        */
+/* functions on objects of its own.                                         */
+/* NBX_IMPORTS_BEGIN */
+/* headers/nb/geometric_bot/mesh/mesh2D.h:30-82 */
+uint32_t nb_mesh2D_get_N_invtx(const nb_mesh2D_t *mesh);
+uint32_t nb_mesh2D_get_N_insgm(const nb_mesh2D_t *mesh);
+uint32_t nb_mesh2D_get_N_nodes(const nb_mesh2D_t *mesh);
+uint32_t nb_mesh2D_get_N_elems(const nb_mesh2D_t *mesh);
+double nb_mesh2D_node_get_x(const nb_mesh2D_t *mesh, uint32_t id);
+double nb_mesh2D_node_get_y(const nb_mesh2D_t *mesh, uint32_t id);
+uint32_t nb_mesh2D_elem_get_N_adj(const nb_mesh2D_t *mesh, uint32_t id);
+uint32_t nb_mesh2D_elem_get_adj(const nb_mesh2D_t *mesh, uint32_t elem_id,
+				uint8_t adj_id);
+uint32_t nb_mesh2D_get_invtx(const nb_mesh2D_t *mesh, uint32_t id);
+uint32_t nb_mesh2D_insgm_get_N_nodes(const nb_mesh2D_t *mesh, uint32_t id);
+uint32_t nb_mesh2D_insgm_get_node(const nb_mesh2D_t *mesh, uint32_t sgm_id,
+				  uint32_t node_id);
+double nb_mesh2D_insgm_get_length(const nb_mesh2D_t *mesh, uint32_t sgm_id);
+double nb_mesh2D_insgm_subsgm_get_length(const nb_mesh2D_t *mesh,
+					 uint32_t sgm_id, uint32_t subsgm_id);
+/* headers/nb/pde_bot/finite_element/element.h:17-18 */
 uint8_t nb_fem_elem_get_N_gpoints(const nb_fem_elem_t *const elemtype);
 uint8_t nb_fem_elem_get_N_nodes(const nb_fem_elem_t *const elemtype);
-
-/* headers/nb/pde_bot/material.h */
-nb_material_t *nb_material_create(void);
-void nb_material_destroy(nb_material_t *mat);
-void nb_material_set_poisson_module(nb_material_t *mat, double pmodule);
-void nb_material_set_elasticity_module(nb_material_t *mat, double emodule);
-void nb_material_set_density(nb_material_t *mat, double density);
+/* headers/nb/pde_bot/material.h:22-26 */
 double nb_material_get_poisson_module(const nb_material_t *const mat);
 double nb_material_get_elasticity_module(const nb_material_t *const mat);
 double nb_material_get_density(const nb_material_t *const mat);
-nb_material_t *nb_material_clone(nb_material_t *mat);             /* :7 */
-/* material.h:13-29 -- the damage model's parameters                    */
-void nb_material_set_fracture_energy(nb_material_t *mat, double frac_energy);
-void nb_material_set_traction_limit_stress(nb_material_t *mat, double stress);
-void nb_material_set_compression_limit_stress(nb_material_t *mat,
-					      double stress);
-void nb_material_set_energy_release_rate(nb_material_t *mat, double energy_rr);
-void nb_material_set_damage_length_scale(nb_material_t *mat, double length);
 double nb_material_get_fracture_energy(const nb_material_t *const mat);
 double nb_material_get_traction_limit_stress(const nb_material_t *const mat);
-double nb_material_get_compression_limit_stress(const nb_material_t *const mat);
-double nb_material_get_energy_release_rate(const nb_material_t *const mat);
-double nb_material_get_damage_length_scale(const nb_material_t *const mat);
-
-/* headers/nb/pde_bot/boundary_conditions/bcond.h:19-45 (FIFO lists) */
-uint16_t nb_bcond_get_memsize(uint8_t N_dof);
-void nb_bcond_init(void *bcond_ptr, uint8_t N_dof);
-void nb_bcond_finish(void *bcond_ptr);
-void *nb_bcond_create(uint8_t N_dof);
-void nb_bcond_destroy(void *bcond_ptr);
-void nb_bcond_clear(void *bcond_ptr);
+/* headers/nb/pde_bot/boundary_conditions/bcond.h:28 and bcond_iter.h:15-33 */
 uint8_t nb_bcond_get_N_dof(const nb_bcond_t *const bcond);
-void nb_bcond_push(nb_bcond_t *bcond, nb_bcond_id type_id,
-		   nb_bcond_where type_elem, uint32_t elem_id,
-		   const bool dof_mask[], const double value[]);
-void nb_bcond_push_function(nb_bcond_t *bcond, nb_bcond_id type_id,
-			    nb_bcond_where type_elem, uint32_t elem_id,
-			    const bool dof_mask[],
-			    void (*fval)(const double *x, double t, double *out));
+void *nb_bcond_iter_create(void);
+void nb_bcond_iter_destroy(void *iter_ptr);
+void nb_bcond_iter_set_conditions(nb_bcond_iter_t *iter,
+				  const nb_bcond_t *bcond,
+				  nb_bcond_id type_id,
+				  nb_bcond_where type_elem);
+bool nb_bcond_iter_has_more(const nb_bcond_iter_t *iter);
+void nb_bcond_iter_go_next(nb_bcond_iter_t *iter);
+uint32_t nb_bcond_iter_get_id(const nb_bcond_iter_t *iter);
+bool nb_bcond_iter_get_mask(const nb_bcond_iter_t *iter, uint8_t dof_id);
+bool nb_bcond_iter_val_is_function(const nb_bcond_iter_t *iter);
+void nb_bcond_iter_get_val(const nb_bcond_iter_t *iter, uint8_t N_dof,
+			   double *x, double t, double val[]);
+/* headers/nb/pde_bot/finite_element/solid_mechanics/static_damage2D.h:27-32 */
+uint32_t nb_fem_implicit_get_N_steps(nb_fem_implicit_t *isparams);
+uint32_t nb_fem_implicit_get_N_max_iter(nb_fem_implicit_t *isparams);
+uint32_t nb_fem_implicit_get_N_max_iter_without_enhance(nb_fem_implicit_t *isparams);
+double nb_fem_implicit_get_residual_tolerance(nb_fem_implicit_t *isparams);
+/* NBX_IMPORTS_END */
 
 /* headers/nb/pde_bot/finite_element/solid_mechanics/static_elasticity2D.h:
  * 13-24.  Returns 0 ok, 1 assembly failed (distorted element), 2 solver
@@ -233,24 +223,9 @@ void nb_fem_compute_stress_from_strain(uint32_t N_elements,
 				       const bool *elements_enabled,
 				       double *stress);
 
-/* headers/nb/pde_bot/finite_element/solid_mechanics/static_damage2D.h:13-47
- * -- quasi-static damage model, load steps of an implicit scheme.         */
-typedef struct nb_fem_implicit_s nb_fem_implicit_t;
-nb_fem_implicit_t *nb_fem_implicit_create(void);
-void nb_fem_implicit_destroy(nb_fem_implicit_t *isparams);
-void nb_fem_implicit_set_N_steps(nb_fem_implicit_t *isparams,
-				 uint32_t N_time_steps);
-void nb_fem_implicit_set_N_max_iter(nb_fem_implicit_t *isparams,
-				    uint32_t N_max_iter);
-void nb_fem_implicit_set_N_max_iter_without_enhance(nb_fem_implicit_t *isparams,
-						    uint32_t N_max_iter);
-void nb_fem_implicit_set_residual_tolerance(nb_fem_implicit_t *isparams,
-					    double penergy_tol);
-uint32_t nb_fem_implicit_get_N_steps(nb_fem_implicit_t *isparams);
-uint32_t nb_fem_implicit_get_N_max_iter(nb_fem_implicit_t *isparams);
-uint32_t nb_fem_implicit_get_N_max_iter_without_enhance(nb_fem_implicit_t *isparams);
-double nb_fem_implicit_get_residual_tolerance(nb_fem_implicit_t *isparams);
-/* Linear triangles; the log file receives the reference's lines (residual
+/* headers/nb/pde_bot/finite_element/solid_mechanics/static_damage2D.h:34-45
+ * -- quasi-static damage model, load steps of an implicit scheme.
+ * Linear triangles; the log file receives the reference's lines (residual
  * of every iteration of every load step).  Conjugate-gradient path: the
  * direct-solver switch is accepted and runs the same Jacobi-PCG.        */
 void nb_fem_compute_2D_Non_Linear_Solid_Mechanics

@@ -1,6 +1,12 @@
-"""Compiles libnbots_amd.so (HIP for gfx950) in-tree.
+"""Compiles libnbots_amd.so (HIP for gfx950) and libnbx_objects.so (host
+C++, the reference's FEM input objects for callers without libnbots)
+in-tree.
 
     python -m nbots_amd.build
+
+libnbots_amd.so leaves the object accessors of include/nbots_amd.h
+(NBX_IMPORTS_BEGIN .. NBX_IMPORTS_END) undefined: they resolve to libnbots
+in a drop-in link, or to libnbx_objects.so loaded before it.
 
 Flags:
   -ffp-contract=off   the reference's arithmetic has no FMA; assembly, SpMV
@@ -20,6 +26,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnbots_amd.so")
+OBJECTS_SRC = os.path.join(HERE, "objects", "objects.cpp")
+OBJECTS_LIB = os.path.join(HERE, "libnbx_objects.so")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -44,7 +53,21 @@ def _compile(src: str) -> str:
     return obj
 
 
+def build_objects(verbose: bool = True) -> str:
+    tmp = OBJECTS_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+           f"-I{os.path.join(ROOT, 'include')}", OBJECTS_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{CXX} failed on {OBJECTS_SRC}:\n{r.stderr[-6000:]}")
+    os.replace(tmp, OBJECTS_LIB)
+    if verbose:
+        print(f"built {OBJECTS_LIB}")
+    return OBJECTS_LIB
+
+
 def build(verbose: bool = True) -> str:
+    build_objects(verbose)
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "bcplan.hpp"
 #include "internal.hpp"
 
 namespace nbx {
@@ -12,18 +13,6 @@ struct ElemParams {
     const double* elemE = nullptr;     // device, per-element modulus (nullable)
     const uint8_t* enabled = nullptr;  // device, element switch (nullable)
     const double* dmg = nullptr;       // device, damage scaling D by (1 - d) (nullable)
-};
-
-// Boundary conditions reduced to what the device applies, in the
-// reference's order (set_bconditions.c:52-61):
-//  * Neumann: for each touched dof, its additions in order;
-//  * Dirichlet: for each constrained dof, the rank of its first
-//    elimination and its first and last prescribed values.
-struct BcPlan {
-    std::vector<uint32_t> n_dof, n_ptr;
-    std::vector<double> n_add;
-    std::vector<uint32_t> d_dof, d_rank;
-    std::vector<double> d_vfirst, d_vlast;
 };
 
 struct FemSys {

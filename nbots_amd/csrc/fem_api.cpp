@@ -1,6 +1,8 @@
-// C-ABI of the device FEM system (nbx_fem_*) and the drop-in FEM driver
-// nb_fem_compute_2D_Solid_Mechanics with the objects it takes (triangle
-// mesh, material, element type, boundary conditions).
+// C-ABI of the device FEM system (nbx_fem_*) and the drop-in FEM drivers
+// nb_fem_compute_2D_Solid_Mechanics and nb_fem_compute_2D_Non_Linear_Solid_Mechanics.
+// The drivers read the caller's mesh, element type, material, boundary
+// conditions and implicit-scheme parameters only through the reference's
+// public accessors (bcplan.hpp): those objects belong to libnbots.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -8,11 +10,11 @@
 #include <string.h>
 
 #include <algorithm>
-#include <map>
 #include <new>
 #include <vector>
 
 #include "../../include/nbots_amd.h"
+#include "bcplan.hpp"
 #include "fem.hpp"
 #include "opts.hpp"
 #include "simp.hpp"
@@ -21,176 +23,9 @@ struct nbx_fem_s {
     nbx::FemSys sys;
 };
 
-// ---- drop-in objects -------------------------------------------------------
-struct nb_mesh2D_s {
-    std::vector<double> nod;
-    std::vector<uint32_t> adj, vtx, sgm_ptr, sgm_nodes;
-};
-
-struct nb_fem_elem_s {
-    nb_elem_id type;
-};
-
-struct nb_material_s {
-    double E = 0.0, nu = 0.0, density = 0.0;
-    double fracture_energy = 0.0, traction_limit = 0.0, compression_limit = 0.0;
-    double energy_release_rate = 0.0, damage_length_scale = 0.0;
-};
-
-// static_damage2D.c:29-34
-struct nb_fem_implicit_s {
-    uint32_t N_steps = 0;
-    uint32_t N_max_iter = 0;
-    uint32_t N_max_iter_without_enhance = 0;
-    double residual_tolerance = 0.0;
-};
-
 namespace {
 
-
-struct BcAtom {
-    uint32_t id;
-    bool mask[2];
-    double val[2];
-    void (*fval)(const double* x, double t, double* out);
-};
-
-}  // namespace
-
-// FIFO lists, like the reference's NB_QUEUE containers (bcond.c:13).
-struct nb_bcond_s {
-    uint8_t N_dof = 2;
-    std::vector<BcAtom> dv, nv, ds, ns;   // Dirichlet/Neumann x vertex/segment
-};
-
-namespace {
-
-double seg_len(const double* nod, uint32_t a, uint32_t b) {
-    // geometric_bot/mesh/mesh2D/mesh2D.c:650-674
-    const double x1 = nod[2 * a], y1 = nod[2 * a + 1];
-    const double x2 = nod[2 * b], y2 = nod[2 * b + 1];
-    return sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
-}
-
-// Collects the Neumann additions per dof and the Dirichlet eliminations in
-// the order of pde_bot/finite_element/solid_mechanics/set_bconditions.c.
-struct PlanBuilder {
-    std::map<uint32_t, std::vector<double>> nadd;
-    std::vector<std::pair<uint32_t, double>> dseq;
-
-    void neumann(uint32_t dof, double v) { nadd[dof].push_back(v); }
-    void dirichlet(uint32_t dof, double v) { dseq.emplace_back(dof, v); }
-
-    nbx::BcPlan finish() const {
-        nbx::BcPlan p;
-        p.n_ptr.push_back(0);
-        for (const auto& kv : nadd) {
-            p.n_dof.push_back(kv.first);
-            for (double a : kv.second) p.n_add.push_back(a);
-            p.n_ptr.push_back((uint32_t)p.n_add.size());
-        }
-        std::map<uint32_t, size_t> where;
-        for (size_t i = 0; i < dseq.size(); ++i) {
-            const uint32_t dof = dseq[i].first;
-            auto it = where.find(dof);
-            if (it == where.end()) {
-                where[dof] = p.d_dof.size();
-                p.d_dof.push_back(dof);
-                p.d_rank.push_back((uint32_t)i);
-                p.d_vfirst.push_back(dseq[i].second);
-                p.d_vlast.push_back(dseq[i].second);
-            } else {
-                p.d_vlast[it->second] = dseq[i].second;
-            }
-        }
-        return p;
-    }
-};
-
-// set_neumann_sgm (:63-154), set_neumann_vtx (:173-190),
-// set_dirichlet_sgm (:192-238), set_dirichlet_vtx (:240-260).
-int plan_from_lists(const double* nod, uint32_t nn, const uint32_t* vtx, uint32_t n_vtx,
-                    const uint32_t* sgm_ptr, const uint32_t* sgm_nodes, uint32_t n_sgm,
-                    const std::vector<BcAtom>& dv, const std::vector<BcAtom>& nv,
-                    const std::vector<BcAtom>& ds, const std::vector<BcAtom>& ns,
-                    const std::vector<const double*>& ns_samples, double factor,
-                    nbx::BcPlan& out) {
-    PlanBuilder pb;
-    auto bad = [&](const char* what, uint32_t id) {
-        nbx::set_last_error("boundary condition on %s %u does not exist", what, id);
-        return nbx::NBX_ERR_ARG;
-    };
-    for (size_t c = 0; c < ns.size(); ++c) {
-        const BcAtom& a = ns[c];
-        if (a.id >= n_sgm) return bad("segment", a.id);
-        const uint32_t* nodes = sgm_nodes + sgm_ptr[a.id];
-        const uint32_t N = sgm_ptr[a.id + 1] - sgm_ptr[a.id];
-        if (N < 2) continue;
-        const double* fv = ns_samples.empty() ? nullptr : ns_samples[c];
-        if (fv) {   // trapezoid integration of sampled tractions (:87-131)
-            double val1[2] = {fv[0], fv[1]};
-            uint32_t v1 = nodes[0];
-            for (uint32_t i = 0; i + 1 < N; ++i) {
-                const double sub = seg_len(nod, nodes[i], nodes[i + 1]);
-                const uint32_t v2 = nodes[i + 1];
-                const double val2[2] = {fv[2 * (i + 1)], fv[2 * (i + 1) + 1]};
-                for (int j = 0; j < 2; ++j)
-                    if (a.mask[j]) {
-                        const double val = 0.5 * (val1[j] + val2[j]) * sub;
-                        pb.neumann(v1 * 2 + j, factor * val * 0.5);
-                        pb.neumann(v2 * 2 + j, factor * val * 0.5);
-                    }
-                v1 = v2;
-                val1[0] = val2[0];
-                val1[1] = val2[1];
-            }
-        } else {    // constant total force split by length (:133-154)
-            const double L = seg_len(nod, nodes[0], nodes[N - 1]);
-            for (uint32_t i = 0; i + 1 < N; ++i) {
-                const double w = seg_len(nod, nodes[i], nodes[i + 1]) / L;
-                const double f = factor * w * 0.5;
-                for (int j = 0; j < 2; ++j)
-                    if (a.mask[j]) pb.neumann(nodes[i] * 2 + j, f * a.val[j]);
-                for (int j = 0; j < 2; ++j)
-                    if (a.mask[j]) pb.neumann(nodes[i + 1] * 2 + j, f * a.val[j]);
-            }
-        }
-    }
-    for (const BcAtom& a : nv) {
-        if (a.id >= n_vtx) return bad("vertex", a.id);
-        const uint32_t v = vtx[a.id];
-        double val[2] = {a.val[0], a.val[1]};
-        if (a.fval) {   // set_neumann evaluates at x = (0, 0) (:161-163)
-            const double x0[2] = {0.0, 0.0};
-            a.fval(x0, 0.0, val);
-        }
-        for (int j = 0; j < 2; ++j)
-            if (a.mask[j]) pb.neumann(v * 2 + j, factor * val[j]);
-    }
-    auto dirichlet_node = [&](const BcAtom& a, uint32_t v) {
-        double val[2] = {a.val[0], a.val[1]};
-        if (a.fval) {   // set_dirichlet evaluates at the node (:224-229)
-            const double x[2] = {nod[2 * v], nod[2 * v + 1]};
-            a.fval(x, 0.0, val);
-        }
-        for (int j = 0; j < 2; ++j)
-            if (a.mask[j]) pb.dirichlet(v * 2 + j, factor * val[j]);
-    };
-    for (const BcAtom& a : ds) {
-        if (a.id >= n_sgm) return bad("segment", a.id);
-        for (uint32_t i = sgm_ptr[a.id]; i < sgm_ptr[a.id + 1]; ++i) dirichlet_node(a, sgm_nodes[i]);
-    }
-    for (const BcAtom& a : dv) {
-        if (a.id >= n_vtx) return bad("vertex", a.id);
-        dirichlet_node(a, vtx[a.id]);
-    }
-    out = pb.finish();
-    for (uint32_t d : out.d_dof)
-        if (d >= 2 * nn) return bad("node", d / 2);
-    for (uint32_t d : out.n_dof)
-        if (d >= 2 * nn) return bad("node", d / 2);
-    return nbx::NBX_OK;
-}
+using nbx::BcAtom;
 
 std::vector<BcAtom> atoms(uint32_t n, const uint32_t* id, const uint8_t* mask, const double* val) {
     std::vector<BcAtom> v(n);
@@ -200,7 +35,6 @@ std::vector<BcAtom> atoms(uint32_t n, const uint32_t* id, const uint8_t* mask, c
         v[c].mask[1] = mask[2 * c + 1] != 0;
         v[c].val[0] = val[2 * c];
         v[c].val[1] = val[2 * c + 1];
-        v[c].fval = nullptr;
     }
     return v;
 }
@@ -318,7 +152,7 @@ int nbx_fem_set_bconditions(nbx_fem_t* f, const uint32_t* vtx, const uint32_t* s
     // the number of segments implied by sgm_ptr being monotone is not
     // checkable, so only node indices are validated.
     nbx::BcPlan plan;
-    NBX_TRY(plan_from_lists(nod.data(), S.nn, vtx, 0xffffffffu, sgm_ptr, sgm_nodes, 0xffffffffu,
+    NBX_TRY(nbx::plan_from_lists(nod.data(), S.nn, vtx, 0xffffffffu, sgm_ptr, sgm_nodes, 0xffffffffu,
                             atoms(n_dv, dv_id, dv_mask, dv_val), atoms(n_nv, nv_id, nv_mask, nv_val),
                             atoms(n_ds, ds_id, ds_mask, ds_val), atoms(n_ns, ns_id, ns_mask, ns_val),
                             samples, factor, plan));
@@ -552,177 +386,25 @@ int nbx_simp_filter(const nbx_simp_t* h, uint64_t* n_entries, uint32_t* Hptr, ui
     return nbx::NBX_OK;
 }
 
-// ---- drop-in objects ----------------------------------------------------------
-nb_mesh2D_t* nb_mesh2D_create(nb_mesh2D_type type) {
-    if (type != NB_TRIAN) {
-        nbx::set_last_error("only triangle meshes (NB_TRIAN) are supported");
-        return nullptr;
-    }
-    return new nb_mesh2D_s;
-}
-void nb_mesh2D_destroy(nb_mesh2D_t* m) { delete m; }
-uint32_t nb_mesh2D_get_N_invtx(const nb_mesh2D_t* m) { return (uint32_t)m->vtx.size(); }
-uint32_t nb_mesh2D_get_N_insgm(const nb_mesh2D_t* m) {
-    return m->sgm_ptr.empty() ? 0 : (uint32_t)m->sgm_ptr.size() - 1;
-}
-uint32_t nb_mesh2D_get_N_nodes(const nb_mesh2D_t* m) { return (uint32_t)(m->nod.size() / 2); }
-uint32_t nb_mesh2D_get_N_elems(const nb_mesh2D_t* m) { return (uint32_t)(m->adj.size() / 3); }
-double nb_mesh2D_node_get_x(const nb_mesh2D_t* m, uint32_t id) { return m->nod[2 * id]; }
-double nb_mesh2D_node_get_y(const nb_mesh2D_t* m, uint32_t id) { return m->nod[2 * id + 1]; }
-uint32_t nb_mesh2D_elem_get_adj(const nb_mesh2D_t* m, uint32_t e, uint8_t k) { return m->adj[3 * e + k]; }
-uint32_t nb_mesh2D_get_invtx(const nb_mesh2D_t* m, uint32_t id) { return m->vtx[id]; }
-uint32_t nb_mesh2D_insgm_get_N_nodes(const nb_mesh2D_t* m, uint32_t id) {
-    return m->sgm_ptr[id + 1] - m->sgm_ptr[id];
-}
-uint32_t nb_mesh2D_insgm_get_node(const nb_mesh2D_t* m, uint32_t s, uint32_t i) {
-    return m->sgm_nodes[m->sgm_ptr[s] + i];
-}
-int nbx_mesh2D_set_trg(nb_mesh2D_t* m, uint32_t N_nod, const double* nod, uint32_t N_elem,
-                       const uint32_t* adj, uint32_t N_vtx, const uint32_t* vtx, uint32_t N_sgm,
-                       const uint32_t* sgm_ptr, const uint32_t* nod_x_sgm) {
-    m->nod.assign(nod, nod + 2 * (size_t)N_nod);
-    m->adj.assign(adj, adj + 3 * (size_t)N_elem);
-    m->vtx.assign(vtx, vtx + N_vtx);
-    m->sgm_ptr.assign(sgm_ptr, sgm_ptr + N_sgm + 1);
-    m->sgm_nodes.assign(nod_x_sgm, nod_x_sgm + (N_sgm ? sgm_ptr[N_sgm] : 0));
-    for (uint32_t a : m->adj)
-        if (a >= N_nod) {
-            nbx::set_last_error("triangle references node %u >= %u", a, N_nod);
-            return nbx::NBX_ERR_ARG;
-        }
-    return nbx::NBX_OK;
-}
-
-nb_fem_elem_t* nb_fem_elem_create(nb_elem_id type) {
-    // element.c:16-33 (unknown types fall back to linear triangles)
-    nb_fem_elem_t* e = new nb_fem_elem_s;
-    e->type = (type == NB_QUAD_LINEAR) ? NB_QUAD_LINEAR : NB_TRG_LINEAR;
-    return e;
-}
-void nb_fem_elem_destroy(nb_fem_elem_t* e) { delete e; }
-uint8_t nb_fem_elem_get_N_gpoints(const nb_fem_elem_t* const e) { return e->type == NB_TRG_LINEAR ? 1 : 4; }
-uint8_t nb_fem_elem_get_N_nodes(const nb_fem_elem_t* const e) { return e->type == NB_TRG_LINEAR ? 3 : 4; }
-
-nb_material_t* nb_material_create(void) { return new nb_material_s; }
-void nb_material_destroy(nb_material_t* m) { delete m; }
-void nb_material_set_poisson_module(nb_material_t* m, double v) { m->nu = v; }
-void nb_material_set_elasticity_module(nb_material_t* m, double v) { m->E = v; }
-void nb_material_set_density(nb_material_t* m, double v) { m->density = v; }
-double nb_material_get_poisson_module(const nb_material_t* const m) { return m->nu; }
-double nb_material_get_elasticity_module(const nb_material_t* const m) { return m->E; }
-double nb_material_get_density(const nb_material_t* const m) { return m->density; }
-nb_material_t* nb_material_clone(nb_material_t* m) { return new nb_material_s(*m); }
-void nb_material_set_fracture_energy(nb_material_t* m, double v) { m->fracture_energy = v; }
-void nb_material_set_traction_limit_stress(nb_material_t* m, double v) { m->traction_limit = v; }
-void nb_material_set_compression_limit_stress(nb_material_t* m, double v) {
-    m->compression_limit = v;
-}
-void nb_material_set_energy_release_rate(nb_material_t* m, double v) { m->energy_release_rate = v; }
-void nb_material_set_damage_length_scale(nb_material_t* m, double v) { m->damage_length_scale = v; }
-double nb_material_get_fracture_energy(const nb_material_t* const m) { return m->fracture_energy; }
-double nb_material_get_traction_limit_stress(const nb_material_t* const m) {
-    return m->traction_limit;
-}
-double nb_material_get_compression_limit_stress(const nb_material_t* const m) {
-    return m->compression_limit;
-}
-double nb_material_get_energy_release_rate(const nb_material_t* const m) {
-    return m->energy_release_rate;
-}
-double nb_material_get_damage_length_scale(const nb_material_t* const m) {
-    return m->damage_length_scale;
-}
-
-// ---- implicit-integration parameters (static_damage2D.c:79-126) -------------
-nb_fem_implicit_t* nb_fem_implicit_create(void) { return new nb_fem_implicit_s; }
-void nb_fem_implicit_destroy(nb_fem_implicit_t* p) { delete p; }
-void nb_fem_implicit_set_N_steps(nb_fem_implicit_t* p, uint32_t n) { p->N_steps = n; }
-void nb_fem_implicit_set_N_max_iter(nb_fem_implicit_t* p, uint32_t n) { p->N_max_iter = n; }
-void nb_fem_implicit_set_N_max_iter_without_enhance(nb_fem_implicit_t* p, uint32_t n) {
-    p->N_max_iter_without_enhance = n;
-}
-void nb_fem_implicit_set_residual_tolerance(nb_fem_implicit_t* p, double t) {
-    p->residual_tolerance = t;
-}
-uint32_t nb_fem_implicit_get_N_steps(nb_fem_implicit_t* p) { return p->N_steps; }
-uint32_t nb_fem_implicit_get_N_max_iter(nb_fem_implicit_t* p) { return p->N_max_iter; }
-uint32_t nb_fem_implicit_get_N_max_iter_without_enhance(nb_fem_implicit_t* p) {
-    return p->N_max_iter_without_enhance;
-}
-double nb_fem_implicit_get_residual_tolerance(nb_fem_implicit_t* p) {
-    return p->residual_tolerance;
-}
-
-uint16_t nb_bcond_get_memsize(uint8_t) { return (uint16_t)sizeof(nb_bcond_s); }
-void nb_bcond_init(void* p, uint8_t N_dof) {
-    nb_bcond_s* b = new (p) nb_bcond_s;
-    b->N_dof = N_dof;
-}
-void nb_bcond_finish(void* p) { static_cast<nb_bcond_s*>(p)->~nb_bcond_s(); }
-void* nb_bcond_create(uint8_t N_dof) {
-    nb_bcond_s* b = new nb_bcond_s;
-    b->N_dof = N_dof;
-    return b;
-}
-void nb_bcond_destroy(void* p) { delete static_cast<nb_bcond_s*>(p); }
-void nb_bcond_clear(void* p) {
-    nb_bcond_s* b = static_cast<nb_bcond_s*>(p);
-    b->dv.clear();
-    b->nv.clear();
-    b->ds.clear();
-    b->ns.clear();
-}
-uint8_t nb_bcond_get_N_dof(const nb_bcond_t* const b) { return b->N_dof; }
-
-static std::vector<BcAtom>* bc_list(nb_bcond_t* b, nb_bcond_id t, nb_bcond_where w) {
-    if (t == NB_DIRICHLET) return w == NB_BC_ON_POINT ? &b->dv : &b->ds;
-    return w == NB_BC_ON_POINT ? &b->nv : &b->ns;
-}
-
-void nb_bcond_push(nb_bcond_t* b, nb_bcond_id t, nb_bcond_where w, uint32_t id, const bool mask[],
-                   const double value[]) {
-    BcAtom a{id, {mask[0], b->N_dof > 1 ? mask[1] : false}, {0.0, 0.0}, nullptr};
-    for (int j = 0; j < b->N_dof && j < 2; ++j) a.val[j] = value[j];
-    bc_list(b, t, w)->push_back(a);
-}
-
-void nb_bcond_push_function(nb_bcond_t* b, nb_bcond_id t, nb_bcond_where w, uint32_t id,
-                            const bool mask[], void (*fval)(const double*, double, double*)) {
-    BcAtom a{id, {mask[0], b->N_dof > 1 ? mask[1] : false}, {0.0, 0.0}, fval};
-    bc_list(b, t, w)->push_back(a);
-}
-
-// static_elasticity2D.c:31-97 on the GPU: pattern, assembly, BCs, Jacobi-PCG
-// with x = 0, max_iter = N, tol = 1e-8 (absolute), strain.
 }  // extern "C"
 
-// The boundary conditions of a bcond object on a mesh, values scaled by
-// factor (nb_fem_set_bconditions, set_bconditions.c:52-61); function-valued
-// Neumann segments are sampled at their nodes at time 0.
-static int plan_for(const nb_mesh2D_t* part, const nb_bcond_t* bcond, double factor,
-                    nbx::BcPlan& plan) {
-    std::vector<std::vector<double>> buf(bcond->ns.size());
-    std::vector<const double*> samples(bcond->ns.size(), nullptr);
-    for (size_t c = 0; c < bcond->ns.size(); ++c) {
-        const BcAtom& a = bcond->ns[c];
-        if (!a.fval || a.id >= nb_mesh2D_get_N_insgm(part)) continue;
-        const uint32_t N = nb_mesh2D_insgm_get_N_nodes(part, a.id);
-        buf[c].resize(2 * (size_t)N);
-        for (uint32_t i = 0; i < N; ++i) {
-            const uint32_t v = nb_mesh2D_insgm_get_node(part, a.id, i);
-            const double x[2] = {part->nod[2 * v], part->nod[2 * v + 1]};
-            a.fval(x, 0.0, &buf[c][2 * i]);
-        }
-        samples[c] = buf[c].data();
+// The element type the drivers take: linear triangles, one Gauss point
+// (element.c:54-73), read through the element's accessors.
+static int check_t3(const nb_fem_elem_t* elemtype) {
+    if (nb_fem_elem_get_N_nodes(elemtype) != 3 || nb_fem_elem_get_N_gpoints(elemtype) != 1) {
+        nbx::set_last_error("only linear triangles (NB_TRG_LINEAR) are implemented on the GPU "
+                            "path");
+        return nbx::NBX_ERR_ARG;
     }
-    return plan_from_lists(part->nod.data(), nb_mesh2D_get_N_nodes(part), part->vtx.data(),
-                           (uint32_t)part->vtx.size(), part->sgm_ptr.data(),
-                           part->sgm_nodes.data(), nb_mesh2D_get_N_insgm(part), bcond->dv,
-                           bcond->nv, bcond->ds, bcond->ns, samples, factor, plan);
+    return nbx::NBX_OK;
 }
 
 extern "C" {
 
+// static_elasticity2D.c:31-97 on the GPU: pattern, assembly, BCs, Jacobi-PCG
+// with x = 0, max_iter = N, tol = 1e-8 (absolute), strain.  The mesh and the
+// boundary conditions are read on the host first (import_mesh,
+// import_bconditions); an invalid object gives -1 before any device work.
 int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t* const part,
                                       const nb_fem_elem_t* const elemtype,
                                       const nb_material_t* const material,
@@ -730,18 +412,18 @@ int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t* const part,
                                       double gravity[2], nb_analysis2D_t,
                                       nb_analysis2D_params* params2D, const bool* elements_enabled,
                                       double* displacement, double* strain) {
-    if (elemtype->type != NB_TRG_LINEAR) {
-        nbx::set_last_error("only linear triangles are implemented on the GPU path");
+    nbx::MeshArrays mesh;
+    nbx::BcPlan plan;
+    int rc = check_t3(elemtype);
+    if (!rc) rc = nbx::import_mesh(part, mesh);
+    if (!rc) rc = nbx::import_bconditions(part, bcond, 1.0, plan);
+    if (rc) {
         fprintf(stderr, "nb_fem_compute_2D_Solid_Mechanics: %s\n", nbx::last_error());
         return -1;
     }
-    if (bcond->N_dof != 2) {
-        nbx::set_last_error("solid mechanics needs 2 dofs per node");
-        return -1;
-    }
-    const uint32_t nn = nb_mesh2D_get_N_nodes(part), ne = nb_mesh2D_get_N_elems(part);
+    const uint32_t nn = mesh.nn, ne = mesh.ne;
     nbx_fem_t* f = nullptr;
-    int rc = nbx_fem_create(&f, nn, part->nod.data(), ne, part->adj.data());
+    rc = nbx_fem_create(&f, nn, mesh.nod.data(), ne, mesh.conn.data());
     if (rc) {
         fprintf(stderr, "nb_fem_compute_2D_Solid_Mechanics: %s\n", nbx::last_error());
         return -1;
@@ -752,21 +434,17 @@ int nb_fem_compute_2D_Solid_Mechanics(const nb_mesh2D_t* const part,
         for (uint32_t e = 0; e < ne; ++e) en[e] = elements_enabled[e] ? 1 : 0;
     }
     int status = 0;
-    rc = nbx_fem_assemble(f, material->E, material->nu, params2D->thickness, nullptr,
+    rc = nbx_fem_assemble(f, nb_material_get_elasticity_module(material),
+                          nb_material_get_poisson_module(material), params2D->thickness, nullptr,
                           elements_enabled ? en.data() : nullptr, enable_self_weight ? 1 : 0,
-                          material->density, gravity);
+                          nb_material_get_density(material), gravity);
     if (rc == nbx::NBX_ERR_DISTORTED) {
         status = 1;   // static_elasticity2D.c:62-65
         goto DONE;
     }
     if (rc) goto FAIL;
-    {
-        nbx::BcPlan plan;
-        rc = plan_for(part, bcond, 1.0, plan);
-        if (rc) goto FAIL;
-        rc = nbx::fem_apply_bc(f->sys, plan);
-        if (rc) goto FAIL;
-    }
+    rc = nbx::fem_apply_bc(f->sys, plan);
+    if (rc) goto FAIL;
     {
         nbx_solve_info_t info;
         rc = nbx_fem_solve(f, NBX_SOLVE_REFERENCE, 1e-8, 2 * nn, 0.0, nullptr, &info);
@@ -800,11 +478,18 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
                          int enable_self_weight, const double* gravity,
                          const nb_analysis2D_params* params2D, nb_fem_implicit_t* params,
                          const char* logfile, double* displacement, double* damage) {
-    if (elemtype->type != NB_TRG_LINEAR || bcond->N_dof != 2) {
-        nbx::set_last_error("the damage solver needs linear triangles with 2 dofs per node");
-        return nbx::NBX_ERR_ARG;
-    }
-    const uint32_t nn = nb_mesh2D_get_N_nodes(part), ne = nb_mesh2D_get_N_elems(part);
+    nbx::MeshArrays mesh;
+    NBX_TRY(check_t3(elemtype));
+    NBX_TRY(nbx::import_mesh(part, mesh));
+    const uint32_t n_steps = nb_fem_implicit_get_N_steps(params);
+    const uint32_t max_iter = nb_fem_implicit_get_N_max_iter(params);
+    const uint32_t max_wo = nb_fem_implicit_get_N_max_iter_without_enhance(params);
+    const double res_tol = nb_fem_implicit_get_residual_tolerance(params);
+    // the conditions scaled by the load factor of each step (n + 1) / N_steps
+    std::vector<nbx::BcPlan> plans(n_steps);
+    for (uint32_t n = 0; n < n_steps; ++n)
+        NBX_TRY(nbx::import_bconditions(part, bcond, (n + 1.0) / (double)n_steps, plans[n]));
+    const uint32_t nn = mesh.nn, ne = mesh.ne;
     const uint32_t N = 2 * nn;
     FILE* log = logfile ? fopen(logfile, "a") : nullptr;
     auto logf = [&](const char* fmt, auto... a) {
@@ -815,11 +500,14 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
     };
     logf("FEM: Damage Model\n");
     nbx_fem_t* f = nullptr;
-    int rc = nbx_fem_create(&f, nn, part->nod.data(), ne, part->adj.data());
+    int rc = nbx_fem_create(&f, nn, mesh.nod.data(), ne, mesh.conn.data());
     double *dmg = nullptr, *r = nullptr, *r_prev = nullptr, *strain = nullptr, *res = nullptr,
            *du = nullptr;
-    const double E = material->E, nu = material->nu;
-    const double Gf = material->fracture_energy, ft = material->traction_limit;
+    const double E = nb_material_get_elasticity_module(material);
+    const double nu = nb_material_get_poisson_module(material);
+    const double Gf = nb_material_get_fracture_energy(material);
+    const double ft = nb_material_get_traction_limit_stress(material);
+    const double density = nb_material_get_density(material);
     const double g0[2] = {0.0, 0.0};
     const double* grav = gravity ? gravity : g0;
     if (rc) goto END;
@@ -847,7 +535,7 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
         S.P.nu = nu;
         S.P.thickness = params2D->thickness;
         S.P.dmg = dmg;
-        for (uint32_t n = 0; n < params->N_steps; ++n) {
+        for (uint32_t n = 0; n < n_steps; ++n) {
             logf("  [ Load step %i]\n", n + 1);
             if (hipMemcpyAsync(r_prev, r, (size_t)ne * 8, hipMemcpyDeviceToDevice, S.s) !=
                     hipSuccess ||
@@ -857,14 +545,10 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
             }
             uint32_t it = 0, wo = 0;
             double best = 0.0;
-            const double factor = (n + 1.0) / (double)params->N_steps;
             for (;;) {
                 if ((rc = nbx::fem_assemble_device(S))) goto END;
-                if ((rc = nbx::fem_body_force(S, enable_self_weight, material->density, grav)))
-                    goto END;
-                nbx::BcPlan plan;
-                if ((rc = plan_for(part, bcond, factor, plan))) goto END;
-                if ((rc = nbx::fem_apply_bc(S, plan))) goto END;
+                if ((rc = nbx::fem_body_force(S, enable_self_weight, density, grav))) goto END;
+                if ((rc = nbx::fem_apply_bc(S, plans[n]))) goto END;
                 double nrm = 0.0;
                 if ((rc = nbx::fem_residual(S, res, &nrm))) goto END;
                 if (it == 0) {
@@ -875,8 +559,7 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
                 } else {
                     wo += 1;
                 }
-                if ((nrm < params->residual_tolerance && it > 0) || it >= params->N_max_iter ||
-                    wo >= params->N_max_iter_without_enhance) {
+                if ((nrm < res_tol && it > 0) || it >= max_iter || wo >= max_wo) {
                     logf("    [%i: Residual: %e ]\n", it, nrm);
                     break;
                 }
@@ -937,7 +620,8 @@ void nb_fem_compute_stress_from_strain(uint32_t N_elements, const nb_fem_elem_t*
         en.resize(N_elements);
         for (uint32_t e = 0; e < N_elements; ++e) en[e] = elements_enabled[e] ? 1 : 0;
     }
-    if (nbx::stress_from_strain(N_elements, material->E, material->nu,
+    if (nbx::stress_from_strain(N_elements, nb_material_get_elasticity_module(material),
+                                nb_material_get_poisson_module(material),
                                 elements_enabled ? en.data() : nullptr, strain, stress))
         fprintf(stderr, "nb_fem_compute_stress_from_strain: GPU failure: %s\n", nbx::last_error());
 }

@@ -59,7 +59,20 @@ struct DevMatrix {
     unsigned* off_flag = nullptr;
 };
 
-// Per-matrix solver scratch (device).  Sized for n scalar rows.
+// Per-matrix solver scratch (device).  Sized for n scalar rows plus two
+// spare entries.  Slots a run reserves (a caller must not pass them as the
+// run's b or x):
+//   FP64 Jacobi-PCG (pcg_run):  v64[0] g, v64[1] p_j (j % 4 = 0), v64[2] w,
+//                               v64[6..8] p_j (j % 4 = 1..3); plain CG also
+//                               v64[5] (the unit diagonal)
+//   flexible CG (CHEB_FCG):     v64[0] r, v64[1] p, v64[2] w; v32[0..3]
+//                               residual, two directions and z of the
+//                               Chebyshev steps
+//   FP32 CG (CG_RESTART/RELIABLE): v32[0..3]; a fallback to FP64 PCG takes
+//                               the FP64 slots above
+//   free for callers: v64[3], v64[4] (the drop-in's b and x, the system's
+//   residual) and the spare entries past the end of v64[5] (one-value
+//   reductions)
 struct Workspace {
     uint32_t n = 0;
     double* v64[9] = {};       // FP64 vectors

@@ -649,14 +649,17 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
                        w.partials, nu);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
     NBX_HIP(hipGetLastError());
-    uint32_t launched = 0, batch = 4, nsamp = 0;
+    // 64-bit, so that max_iter near UINT32_MAX ("no limit") cannot wrap the
+    // guard below
+    uint64_t launched = 0;
+    uint32_t batch = 4, nsamp = 0;
     for (;;) {
         for (uint32_t i = 0; i < batch; ++i) {
             const bool samp = (launched + i) % Workspace::SAMPLE_EVERY == 0 &&
                               nsamp < (uint32_t)Workspace::NSAMP;
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             // iteration j = launched + i (kernels after the stop return at once)
-            const uint32_t j = launched + i;
+            const uint32_t j = (uint32_t)(launched + i);
             const V* pj = pbuf.p[j & 3u];
             const bool flush = (j & 3u) == 3u;
             hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, pj, wv, w.st, w.partials,
@@ -689,8 +692,9 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
         launched += batch;
         NBX_TRY(poll_state(w, s));
         if (w.h_st->done) break;
-        if (launched > max_iter + 8u) {
-            set_last_error("PCG did not report completion after %u iterations", launched);
+        if (launched > (uint64_t)max_iter + 8u) {
+            set_last_error("PCG did not report completion after %llu iterations",
+                           (unsigned long long)launched);
             return NBX_ERR_BREAKDOWN;
         }
         batch = batch < 64 ? batch * 2 : 64;
@@ -732,7 +736,8 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
     if (rc) return -rc;
     const double* diag = m.diag;
     if (unpreconditioned) {
-        // v64[5] is free during a PCG run (pcg_run uses v64[0..2])
+        // v64[5] is free during a PCG run (pcg_run uses v64[0..2] and the
+        // direction buffers v64[6..8])
         hipLaunchKernelGGL(k_fill_one, dim3(grid_for(m.n)), dim3(BLOCK), 0, s, w.v64[5], m.n);
         if (hipGetLastError() != hipSuccess) return -NBX_ERR_HIP;
         diag = w.v64[5];
@@ -1025,7 +1030,8 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     precondition(1);
     NBX_HIP(hipGetLastError());
     NBX_TRY(poll_state(w, s));
-    uint32_t batch = 4, nsamp = 0, launched = 0;
+    uint32_t batch = 4, nsamp = 0;
+    uint64_t launched = 0;   // 64-bit: max_iter near UINT32_MAX cannot wrap the guard
     constexpr uint32_t every = 8;   // sample one outer iteration in eight
     while (!w.h_st->done) {
         for (uint32_t i = 0; i < batch; ++i) {
@@ -1044,7 +1050,7 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         NBX_HIP(hipGetLastError());
         launched += batch;
         NBX_TRY(poll_state(w, s));
-        if (launched > max_iter + 8u && !w.h_st->done) {
+        if (launched > (uint64_t)max_iter + 8u && !w.h_st->done) {
             set_last_error("flexible CG did not report completion");
             return NBX_ERR_BREAKDOWN;
         }

@@ -2,12 +2,19 @@
 // (headers/nb/solver_bot/sparse/sparse.h:28-78) with SpMV and Jacobi-PCG on
 // the GPU.
 //
-// Host storage is one flat CSR (rowptr, col, val) instead of the reference's
-// arrays of row pointers (solver_bot/sparse/sparse_struct.h:6-11); entry
-// access keeps the reference's semantics (binary search in a sorted row,
-// exit(1) on an entry outside the pattern, sparse.c:134-219).  The device
-// copy is refreshed lazily: any host-side write marks the values stale, and
-// the next GPU operation uploads them.
+// Host storage is the reference's own layout (solver_bot/sparse/
+// sparse_struct.h:6-11): arrays of row pointers to sorted column indices and
+// values, each row allocated with malloc/calloc and released with free, as
+// nb_sparse_allocate/nb_sparse_destroy do (sparse_struct.c:23-31,
+// sparse.c:114-125).  A matrix built here can therefore go to any libnbots
+// function (LU, Cholesky, transposes, permutations), and one built by
+// libnbots can come here: the two libraries' nb_sparse_* functions are
+// interchangeable on either.
+//
+// The device copy lives in a registry beside the handle, not in it.  The
+// host rows are the truth: every GPU operation compares the device copy's
+// pattern and values with the rows (one host pass over them) and uploads
+// what changed, so writes made by any library between calls are seen.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -15,35 +22,80 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nbots_amd.h"
 #include "internal.hpp"
 #include "opts.hpp"
 
+// sparse_struct.h:6-11
 struct nb_sparse_s {
-    uint32_t N = 0;
-    std::vector<uint32_t> rowptr;
-    std::vector<uint32_t> col;
-    std::vector<double> val;
-    // device mirror
-    mutable nbx::DevMatrix dev;
-    mutable nbx::Workspace ws;
-    mutable hipStream_t stream = nullptr;
-    mutable bool dev_pattern = false;
-    mutable bool dev_values = false;
+    double** rows_values;
+    uint32_t** rows_index;
+    uint32_t* rows_size;
+    uint32_t N;
 };
 
 namespace {
 
+// Device copy of one matrix and the host pattern it was built from.
+struct Mirror {
+    nbx::DevMatrix dev;
+    nbx::Workspace ws;
+    hipStream_t stream = nullptr;
+    bool built = false;
+    std::vector<uint32_t> rowptr, col;   // pattern of the device copy
+    std::vector<double> val;             // values of the device copy
+};
 
+std::mutex g_mu;
+std::unordered_map<const nb_sparse_t*, Mirror*> g_mirrors;
+
+void drop_mirror(const nb_sparse_t* A) {
+    Mirror* m = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_mirrors.find(A);
+        if (it == g_mirrors.end()) return;
+        m = it->second;
+        g_mirrors.erase(it);
+    }
+    nbx::mat_free(m->dev);
+    nbx::ws_free(m->ws);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+// sparse_struct.c:23-31
+nb_sparse_t* allocate(uint32_t N) {
+    nb_sparse_t* A = (nb_sparse_t*)malloc(sizeof *A);
+    A->rows_values = (double**)malloc((size_t)N * sizeof(double*));
+    A->rows_index = (uint32_t**)malloc((size_t)N * sizeof(uint32_t*));
+    A->rows_size = (uint32_t*)calloc(N ? N : 1, sizeof(uint32_t));
+    A->N = N;
+    return A;
+}
+
+void free_rows(nb_sparse_t* A) {
+    for (uint32_t i = 0; i < A->N; ++i) {
+        free(A->rows_values[i]);
+        free(A->rows_index[i]);
+    }
+    free(A->rows_values);
+    free(A->rows_index);
+    free(A->rows_size);
+}
+
+// Position of column j in row i, or -1 (rows hold ascending columns).
 int64_t find_slot(const nb_sparse_t* A, uint32_t i, uint32_t j) {
     if (i >= A->N) return -1;
-    const uint32_t* b = A->col.data() + A->rowptr[i];
-    const uint32_t* e = A->col.data() + A->rowptr[i + 1];
+    const uint32_t* b = A->rows_index[i];
+    const uint32_t* e = b + A->rows_size[i];
     const uint32_t* it = std::lower_bound(b, e, j);
     if (it == e || *it != j) return -1;
-    return it - A->col.data();
+    return it - b;
 }
 
 [[noreturn]] void not_allocated(const char* fn, const nb_sparse_t* A, uint32_t i, uint32_t j,
@@ -54,25 +106,69 @@ int64_t find_slot(const nb_sparse_t* A, uint32_t i, uint32_t j) {
     exit(1);
 }
 
-// Bring the device copy up to date; returns an nbx status.
-int sync_device(const nb_sparse_t* A) {
-    if (!A->stream) {
-        hipError_t e = hipStreamCreate(&A->stream);
+// The mirror of A, brought up to date with A's rows (NBX status in *rc).
+Mirror* sync_device(const nb_sparse_t* A, int* rc) {
+    Mirror* m;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        Mirror*& slot = g_mirrors[A];
+        if (!slot) slot = new Mirror;
+        m = slot;
+    }
+    *rc = nbx::NBX_OK;
+    if (!m->stream) {
+        const hipError_t e = hipStreamCreate(&m->stream);
         if (e != hipSuccess) {
             nbx::set_last_error("hipStreamCreate: %s", hipGetErrorString(e));
-            return nbx::NBX_ERR_HIP;
+            *rc = nbx::NBX_ERR_HIP;
+            return nullptr;
         }
     }
-    if (A->dev_pattern && A->dev_values) return nbx::NBX_OK;
-    int rc = nbx::mat_upload_csr(A->dev, A->N, A->rowptr.data(), A->col.data(), A->val.data(),
-                                 A->dev_pattern, A->stream);
-    if (rc) return rc;
-    A->dev_pattern = true;
-    A->dev_values = true;
-    return nbx::ws_alloc(A->ws, A->N);
+    const uint32_t N = A->N;
+    // pattern: the same row sizes and columns as the device copy's?
+    bool same = m->built && m->rowptr.size() == (size_t)N + 1;
+    for (uint32_t i = 0; same && i < N; ++i) {
+        const uint32_t a = m->rowptr[i], n = A->rows_size[i];
+        same = m->rowptr[i + 1] - a == n &&
+               (n == 0 || memcmp(A->rows_index[i], m->col.data() + a, n * sizeof(uint32_t)) == 0);
+    }
+    if (!same) {
+        uint64_t nnz = 0;
+        for (uint32_t i = 0; i < N; ++i) nnz += A->rows_size[i];
+        if (nnz >= 0xffffffffull) {
+            nbx::set_last_error("matrix has %llu entries (32-bit row offsets)",
+                                (unsigned long long)nnz);
+            *rc = nbx::NBX_ERR_ARG;
+            return nullptr;
+        }
+        m->rowptr.assign((size_t)N + 1, 0);
+        m->col.resize(nnz);
+        m->val.resize(nnz);
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint32_t n = A->rows_size[i];
+            m->rowptr[i + 1] = m->rowptr[i] + n;
+            if (n) memcpy(m->col.data() + m->rowptr[i], A->rows_index[i], n * sizeof(uint32_t));
+        }
+    }
+    // values: copy the rows into the staging array, noting any change
+    bool vals_same = same;
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t n = A->rows_size[i];
+        if (!n) continue;
+        double* dst = m->val.data() + m->rowptr[i];
+        if (vals_same && memcmp(dst, A->rows_values[i], n * sizeof(double)) == 0) continue;
+        vals_same = false;
+        memcpy(dst, A->rows_values[i], n * sizeof(double));
+    }
+    if (!same || !vals_same) {
+        *rc = nbx::mat_upload_csr(m->dev, N, m->rowptr.data(), m->col.data(), m->val.data(),
+                                  same, m->stream);
+        if (!*rc) *rc = nbx::ws_alloc(m->ws, N);
+        m->built = *rc == nbx::NBX_OK;
+        if (*rc) return nullptr;
+    }
+    return m;
 }
-
-void stale(nb_sparse_t* A) { A->dev_values = false; }
 
 }  // namespace
 
@@ -83,60 +179,41 @@ constexpr int MODE_PLAIN_CG = 0x100;
 
 extern "C" {
 
-// ---- graph (graph.c:20-40) -------------------------------------------------
-uint32_t nb_graph_get_memsize(void) { return sizeof(nb_graph_t); }
-void nb_graph_init(nb_graph_t* g) { memset(g, 0, sizeof *g); }
-void nb_graph_clear(nb_graph_t* g) {
-    if (g->adj) {
-        for (uint32_t i = 0; i < g->N; ++i) free(g->adj[i]);
-        free(g->adj);
-    }
-    free(g->N_adj);
-    if (g->wij) {
-        for (uint32_t i = 0; i < g->N; ++i) free(g->wij[i]);
-        free(g->wij);
-    }
-    free(g->wi);
-    memset(g, 0, sizeof *g);
-}
-void nb_graph_finish(nb_graph_t* g) { nb_graph_clear(g); }
-
-// ---- creation --------------------------------------------------------------
-// sparse.c:20-60: row perm[i]*vpn+k1 (or i*vpn+k1) holds the columns
-// {adj(i), i} x vpn in ascending order; columns are not permuted.
+// ---- creation (sparse.c:20-95) ------------------------------------------------
+// Row perm[i]*vpn+k1 (or i*vpn+k1) holds the columns {adj(i), i} x vpn in
+// ascending order; columns are not permuted.  Values start at zero.
 nb_sparse_t* nb_sparse_create(const nb_graph_t* const graph, const uint32_t* const perm,
                               uint32_t vpn) {
-    nb_sparse_t* A = new nb_sparse_t;
-    const uint32_t N = graph->N * vpn;
-    A->N = N;
-    std::vector<uint32_t> size(N, 0);
-    for (uint32_t i = 0; i < graph->N; ++i)
+    nb_sparse_t* A = allocate(graph->N * vpn);
+    for (uint32_t i = 0; i < graph->N; ++i) {
+        const uint32_t size = (graph->N_adj[i] + 1) * vpn;
         for (uint32_t k1 = 0; k1 < vpn; ++k1) {
-            const uint32_t r = perm ? perm[i] * vpn + k1 : i * vpn + k1;
-            size[r] = (graph->N_adj[i] + 1) * vpn;
-        }
-    A->rowptr.assign(N + 1, 0);
-    for (uint32_t r = 0; r < N; ++r) A->rowptr[r + 1] = A->rowptr[r] + size[r];
-    A->col.assign(A->rowptr[N], 0);
-    A->val.assign(A->rowptr[N], 0.0);
-    for (uint32_t i = 0; i < graph->N; ++i)
-        for (uint32_t k1 = 0; k1 < vpn; ++k1) {
-            const uint32_t r = perm ? perm[i] * vpn + k1 : i * vpn + k1;
-            uint32_t* row = A->col.data() + A->rowptr[r];
+            const uint32_t r = (perm ? perm[i] : i) * vpn + k1;
+            A->rows_size[r] = size;
+            A->rows_values[r] = (double*)calloc(size ? size : 1, sizeof(double));
+            uint32_t* row = (uint32_t*)malloc((size ? size : 1) * sizeof(uint32_t));
+            A->rows_index[r] = row;
             for (uint32_t j = 0; j < graph->N_adj[i]; ++j)
                 for (uint32_t k2 = 0; k2 < vpn; ++k2) row[j * vpn + k2] = graph->adj[i][j] * vpn + k2;
             for (uint32_t k2 = 0; k2 < vpn; ++k2) row[graph->N_adj[i] * vpn + k2] = i * vpn + k2;
-            std::sort(row, row + size[r]);
+            std::sort(row, row + size);
         }
+    }
     return A;
 }
 
 nb_sparse_t* nb_sparse_clone(nb_sparse_t* A) {
-    nb_sparse_t* B = new nb_sparse_t;
-    B->N = A->N;
-    B->rowptr = A->rowptr;
-    B->col = A->col;
-    B->val = A->val;
+    nb_sparse_t* B = allocate(A->N);
+    for (uint32_t i = 0; i < A->N; ++i) {
+        const uint32_t n = A->rows_size[i];
+        B->rows_size[i] = n;
+        B->rows_index[i] = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+        B->rows_values[i] = (double*)calloc(n ? n : 1, sizeof(double));
+        if (n) {
+            memcpy(B->rows_index[i], A->rows_index[i], n * sizeof(uint32_t));
+            memcpy(B->rows_values[i], A->rows_values[i], n * sizeof(double));
+        }
+    }
     return B;
 }
 
@@ -149,8 +226,8 @@ void nb_sparse_save(const nb_sparse_t* const A, const char* filename) {
     }
     fprintf(fp, "%i %i\n", A->N, A->N);
     for (uint32_t i = 0; i < A->N; ++i)
-        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1]; ++s)
-            fprintf(fp, "%i %i %e \n", i, A->col[s], A->val[s]);
+        for (uint32_t s = 0; s < A->rows_size[i]; ++s)
+            fprintf(fp, "%i %i %e \n", i, A->rows_index[i][s], A->rows_values[i][s]);
     fclose(fp);
 }
 
@@ -201,21 +278,27 @@ static bool mat4_find(FILE* fp, const char* label, int32_t info[5]) {
 void nb_sparse_save_mat4(const nb_sparse_t* const A, const char* url, char* label) {
     FILE* fp = fopen(url, "ab");
     if (!fp) mat4_fail("Impossible to open matlab v4 file.");
-    const uint32_t nnz = (uint32_t)A->col.size();
-    mat4_header(fp, 2, (int32_t)nnz + 1, 3, label);
-    std::vector<uint32_t> order(nnz), row(nnz);
+    std::vector<uint32_t> row, col;
+    std::vector<double> val;
     for (uint32_t i = 0; i < A->N; ++i)
-        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1]; ++s) row[s] = i;
+        for (uint32_t s = 0; s < A->rows_size[i]; ++s) {
+            row.push_back(i);
+            col.push_back(A->rows_index[i][s]);
+            val.push_back(A->rows_values[i][s]);
+        }
+    const uint32_t nnz = (uint32_t)col.size();
+    mat4_header(fp, 2, (int32_t)nnz + 1, 3, label);
+    std::vector<uint32_t> order(nnz);
     for (uint32_t s = 0; s < nnz; ++s) order[s] = s;
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return A->col[a] != A->col[b] ? A->col[a] < A->col[b] : row[a] < row[b];
+        return col[a] != col[b] ? col[a] < col[b] : row[a] < row[b];
     });
     std::vector<double> out(3 * (size_t)nnz + 3);
     const double N = (double)A->N;
     for (uint32_t k = 0; k < nnz; ++k) {
         out[k] = (double)row[order[k]] + 1.0;
-        out[nnz + 1 + k] = (double)A->col[order[k]] + 1.0;
-        out[2 * (size_t)nnz + 2 + k] = A->val[order[k]];
+        out[nnz + 1 + k] = (double)col[order[k]] + 1.0;
+        out[2 * (size_t)nnz + 2 + k] = val[order[k]];
     }
     out[nnz] = N;
     out[2 * (size_t)nnz + 1] = N;
@@ -224,6 +307,8 @@ void nb_sparse_save_mat4(const nb_sparse_t* const A, const char* url, char* labe
     fclose(fp);
 }
 
+// matlab_v4.c:25-184: A takes the record's N and rows (its previous rows are
+// released; the reference overwrites the row arrays without releasing them).
 void nb_sparse_read_mat4(nb_sparse_t* A, const char* url, const char* label) {
     FILE* fp = fopen(url, "rb");
     if (!fp) mat4_fail("Impossible to open matlab v4 file.");
@@ -240,40 +325,42 @@ void nb_sparse_read_mat4(nb_sparse_t* A, const char* url, const char* label) {
         mat4_fail("The matlab v4 file is corrupted.");
     fclose(fp);
     const uint32_t N = (uint32_t)d[nnz];
-    std::vector<uint32_t> rowptr(N + 1, 0), col(nnz);
-    std::vector<double> val(nnz);
+    std::vector<uint32_t> size(N, 0);
     for (uint32_t k = 0; k < nnz; ++k) {
-        const double r = d[k];
-        if (!(r >= 1.0 && r <= (double)N)) mat4_fail("The matlab v4 file is corrupted.");
-        rowptr[(uint32_t)r]++;
+        const double r = d[k], c = d[nnz + 1 + k];
+        if (!(r >= 1.0 && r <= (double)N) || !(c >= 1.0 && c <= (double)N))
+            mat4_fail("The matlab v4 file is corrupted.");
+        size[(uint32_t)r - 1]++;
     }
-    for (uint32_t i = 0; i < N; ++i) rowptr[i + 1] += rowptr[i];
-    std::vector<uint32_t> fill(rowptr.begin(), rowptr.end() - 1);
+    drop_mirror(A);
+    free_rows(A);
+    nb_sparse_t* B = allocate(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        B->rows_size[i] = 0;
+        B->rows_index[i] = (uint32_t*)calloc(size[i] ? size[i] : 1, sizeof(uint32_t));
+        B->rows_values[i] = (double*)calloc(size[i] ? size[i] : 1, sizeof(double));
+    }
     for (uint32_t k = 0; k < nnz; ++k) {
         const uint32_t i = (uint32_t)d[k] - 1;
-        const double c = d[nnz + 1 + k];
-        if (!(c >= 1.0 && c <= (double)N)) mat4_fail("The matlab v4 file is corrupted.");
-        col[fill[i]] = (uint32_t)c - 1;
-        val[fill[i]++] = d[2 * (size_t)nnz + 2 + k];
+        const uint32_t s = B->rows_size[i]++;
+        B->rows_index[i][s] = (uint32_t)d[nnz + 1 + k] - 1;
+        B->rows_values[i][s] = d[2 * (size_t)nnz + 2 + k];
     }
     // columns ascending within a row, values moved with them
     for (uint32_t i = 0; i < N; ++i) {
-        std::vector<std::pair<uint32_t, double>> e;
-        for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; ++s) e.emplace_back(col[s], val[s]);
+        std::vector<std::pair<uint32_t, double>> e(B->rows_size[i]);
+        for (uint32_t s = 0; s < B->rows_size[i]; ++s)
+            e[s] = {B->rows_index[i][s], B->rows_values[i][s]};
         std::stable_sort(e.begin(), e.end(),
                          [](const std::pair<uint32_t, double>& a,
                             const std::pair<uint32_t, double>& b) { return a.first < b.first; });
-        for (uint32_t s = rowptr[i], t = 0; s < rowptr[i + 1]; ++s, ++t) {
-            col[s] = e[t].first;
-            val[s] = e[t].second;
+        for (uint32_t s = 0; s < B->rows_size[i]; ++s) {
+            B->rows_index[i][s] = e[s].first;
+            B->rows_values[i][s] = e[s].second;
         }
     }
-    A->N = N;
-    A->rowptr.swap(rowptr);
-    A->col.swap(col);
-    A->val.swap(val);
-    A->dev_pattern = false;
-    A->dev_values = false;
+    *A = *B;
+    free(B);
 }
 
 void nb_mat4_clear(const char* url) {
@@ -313,47 +400,49 @@ void nb_mat4_read_vec(const char* url, char* label, double* _x) {
     fclose(fp);
 }
 
+// sparse.c:114-125, plus the device copy if there is one.  Works on a matrix
+// that libnbots allocated (nb_sparse_allocate, the LU factors).
 void nb_sparse_destroy(nb_sparse_t* A) {
     if (!A) return;
-    nbx::mat_free(A->dev);
-    nbx::ws_free(A->ws);
-    if (A->stream) (void)hipStreamDestroy(A->stream);
-    delete A;
+    drop_mirror(A);
+    free_rows(A);
+    free(A);
 }
 
 void nb_sparse_reset(nb_sparse_t* A) {
-    std::fill(A->val.begin(), A->val.end(), 0.0);
-    stale(A);
+    for (uint32_t i = 0; i < A->N; ++i)
+        if (A->rows_size[i]) memset(A->rows_values[i], 0, A->rows_size[i] * sizeof(double));
 }
 
 // ---- entry access (sparse.c:134-219) ----------------------------------------
 void nb_sparse_set(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
     const int64_t s = find_slot(A, i, j);
     if (s < 0) not_allocated("nb_sparse_set", A, i, j, value);
-    A->val[s] = value;
-    stale(A);
+    A->rows_values[i][s] = value;
 }
 
 void nb_sparse_set_identity_row(nb_sparse_t* A, uint32_t row) {
-    std::fill(A->val.begin() + A->rowptr[row], A->val.begin() + A->rowptr[row + 1], 0.0);
+    if (A->rows_size[row]) memset(A->rows_values[row], 0, A->rows_size[row] * sizeof(double));
     nb_sparse_set(A, row, row, 1.0);
 }
 
 void nb_sparse_make_diagonal(nb_sparse_t* A, double diag_val) {
-    for (uint32_t i = 0; i < A->N; ++i) nb_sparse_set_identity_row(A, i), nb_sparse_set(A, i, i, diag_val);
+    for (uint32_t i = 0; i < A->N; ++i) {
+        if (A->rows_size[i]) memset(A->rows_values[i], 0, A->rows_size[i] * sizeof(double));
+        nb_sparse_set(A, i, i, diag_val);
+    }
 }
 
 double nb_sparse_get(const nb_sparse_t* const A, uint32_t i, uint32_t j) {
     const int64_t s = find_slot(A, i, j);
-    return s < 0 ? 0.0 : A->val[s];
+    return s < 0 ? 0.0 : A->rows_values[i][s];
 }
 
 double nb_sparse_get_and_set(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
     const int64_t s = find_slot(A, i, j);
     if (s < 0) not_allocated("nb_sparse_get_and_set", A, i, j, value);
-    const double old = A->val[s];
-    A->val[s] = value;
-    stale(A);
+    const double old = A->rows_values[i][s];
+    A->rows_values[i][s] = value;
     return old;
 }
 
@@ -364,13 +453,12 @@ bool nb_sparse_is_non_zero(const nb_sparse_t* const A, uint32_t i, uint32_t j) {
 void nb_sparse_add(nb_sparse_t* A, uint32_t i, uint32_t j, double value) {
     const int64_t s = find_slot(A, i, j);
     if (s < 0) not_allocated("nb_sparse_add", A, i, j, value);
-    A->val[s] += value;
-    stale(A);
+    A->rows_values[i][s] += value;
 }
 
 void nb_sparse_scale(nb_sparse_t* A, double factor) {
-    for (double& v : A->val) v *= factor;
-    stale(A);
+    for (uint32_t i = 0; i < A->N; ++i)
+        for (uint32_t s = 0; s < A->rows_size[i]; ++s) A->rows_values[i][s] *= factor;
 }
 
 void nb_sparse_multiply_scalar(nb_sparse_t* A, double scalar, uint32_t) {
@@ -378,55 +466,65 @@ void nb_sparse_multiply_scalar(nb_sparse_t* A, double scalar, uint32_t) {
 }
 
 uint32_t nb_sparse_get_size(const nb_sparse_t* const A) { return A->N; }
-uint32_t nb_sparse_get_nnz(const nb_sparse_t* const A) { return A->rowptr[A->N]; }
+
+uint32_t nb_sparse_get_nnz(const nb_sparse_t* const A) {
+    uint32_t nnz = 0;
+    for (uint32_t i = 0; i < A->N; ++i) nnz += A->rows_size[i];
+    return nnz;
+}
 
 double nb_sparse_get_asym(const nb_sparse_t* const A) {
-    // sparse.c:362-379
+    // sparse.c:362-379: the entries left of the diagonal against their mirrors
     double sum = 0;
     for (uint32_t i = 0; i < A->N; ++i)
-        for (uint32_t s = A->rowptr[i]; s < A->rowptr[i + 1] && A->col[s] < i; ++s) {
-            const double aji = nb_sparse_get(A, A->col[s], i);
-            sum += (A->val[s] - aji) * (A->val[s] - aji);
+        for (uint32_t s = 0; s < A->rows_size[i] && A->rows_index[i][s] < i; ++s) {
+            const double aji = nb_sparse_get(A, A->rows_index[i][s], i);
+            const double d = A->rows_values[i][s] - aji;
+            sum += d * d;
         }
     return sqrt(sum);
 }
 
 double nb_sparse_get_frobenius_norm(const nb_sparse_t* const A) {
     double sum = 0;
-    for (double v : A->val) sum += v * v;
+    for (uint32_t i = 0; i < A->N; ++i)
+        for (uint32_t s = 0; s < A->rows_size[i]; ++s) sum += A->rows_values[i][s] * A->rows_values[i][s];
     return sqrt(sum);
 }
 
 // sparse.c:416-430: zero row idx and column idx, diagonal 1, RHS[idx] = value
 // and RHS[j] -= A[j][idx] * value for the other rows j of the column.
 void nb_sparse_set_Dirichlet_condition(nb_sparse_t* A, double* RHS, uint32_t idx, double value) {
-    for (uint32_t s = A->rowptr[idx]; s < A->rowptr[idx + 1]; ++s) {
-        const uint32_t jdx = A->col[s];
+    for (uint32_t s = 0; s < A->rows_size[idx]; ++s) {
+        const uint32_t jdx = A->rows_index[idx][s];
         if (idx == jdx) {
-            A->val[s] = 1.0;
+            A->rows_values[idx][s] = 1.0;
             RHS[idx] = value;
         } else {
-            A->val[s] = 0.0;
+            A->rows_values[idx][s] = 0.0;
             const double var = nb_sparse_get_and_set(A, jdx, idx, 0.0);
             RHS[jdx] -= var * value;
         }
     }
-    stale(A);
 }
 
 // ---- GPU operations -----------------------------------------------------------
 // sparse.c:405-414, on the device (bit-identical row sums).
 void nb_sparse_multiply_vector(const nb_sparse_t* A, const double* in, double* out, uint32_t) {
     if (A->N == 0) return;
-    int rc = sync_device(A);
-    double* dx = A->ws.v64[3];
-    double* dy = A->ws.v64[4];
-    if (rc == 0 && hipMemcpyAsync(dx, in, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream) != hipSuccess)
-        rc = nbx::NBX_ERR_HIP;
-    if (rc == 0) rc = nbx::spmv(A->dev, dx, dy, A->stream);
-    if (rc == 0 && (hipMemcpyAsync(out, dy, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream) != hipSuccess ||
-                    hipStreamSynchronize(A->stream) != hipSuccess))
-        rc = nbx::NBX_ERR_HIP;
+    int rc = 0;
+    Mirror* m = sync_device(A, &rc);
+    if (!rc) {
+        double* dx = m->ws.v64[3];
+        double* dy = m->ws.v64[4];
+        if (hipMemcpyAsync(dx, in, (size_t)A->N * 8, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+            rc = nbx::NBX_ERR_HIP;
+        if (!rc) rc = nbx::spmv(m->dev, dx, dy, m->stream);
+        if (!rc && (hipMemcpyAsync(out, dy, (size_t)A->N * 8, hipMemcpyDeviceToHost, m->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(m->stream) != hipSuccess))
+            rc = nbx::NBX_ERR_HIP;
+    }
     if (rc) {
         // the reference has no error path here; report and leave NaNs
         fprintf(stderr, "nb_sparse_multiply_vector: GPU failure: %s\n", nbx::last_error());
@@ -480,30 +578,43 @@ int nbx_device_info(char* name, uint32_t len, uint32_t* n_cu, double* hbm_gb) {
     return 0;
 }
 
-// Bulk copies in the matrix's CSR order (no reference counterpart: the
+// Bulk copies in the matrix's row order (no reference counterpart: the
 // reference exposes entries one at a time, sparse.h:36-43).
 int nbx_sparse_export_csr(const nb_sparse_t* A, uint32_t* rowptr, uint32_t* col, double* val) {
-    if (rowptr) memcpy(rowptr, A->rowptr.data(), (A->N + 1) * sizeof(uint32_t));
-    if (col) memcpy(col, A->col.data(), A->col.size() * sizeof(uint32_t));
-    if (val) memcpy(val, A->val.data(), A->val.size() * sizeof(double));
+    uint32_t off = 0;
+    if (rowptr) rowptr[0] = 0;
+    for (uint32_t i = 0; i < A->N; ++i) {
+        const uint32_t n = A->rows_size[i];
+        if (col && n) memcpy(col + off, A->rows_index[i], n * sizeof(uint32_t));
+        if (val && n) memcpy(val + off, A->rows_values[i], n * sizeof(double));
+        off += n;
+        if (rowptr) rowptr[i + 1] = off;
+    }
     return NBX_OK;
 }
 
 int nbx_sparse_import_values(nb_sparse_t* A, const double* val) {
-    memcpy(A->val.data(), val, A->val.size() * sizeof(double));
-    stale(A);
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < A->N; ++i) {
+        const uint32_t n = A->rows_size[i];
+        if (n) memcpy(A->rows_values[i], val + off, n * sizeof(double));
+        off += n;
+    }
     return NBX_OK;
 }
 
 int nbx_sparse_is_blocked(const nb_sparse_t* A) {
-    if (sync_device(A)) return -1;
-    return A->dev.blocked ? 1 : 0;
+    int rc = 0;
+    Mirror* m = sync_device(A, &rc);
+    if (rc) return -1;
+    return m->dev.blocked ? 1 : 0;
 }
 
 int nbx_sparse_spmv_dev(const nb_sparse_t* A, const double* x_dev, double* y_dev) {
-    int rc = sync_device(A);
+    int rc = 0;
+    Mirror* m = sync_device(A, &rc);
     if (rc) return rc;
-    return nbx::spmv(A->dev, x_dev, y_dev, A->stream);
+    return nbx::spmv(m->dev, x_dev, y_dev, m->stream);
 }
 
 static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mode, double tol,
@@ -512,37 +623,37 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
     if (!info) info = &tmp;
     memset(info, 0, sizeof *info);
     if (A->N == 0) return NBX_OK;
-    int rc = sync_device(A);
+    int rc = 0;
+    Mirror* m = sync_device(A, &rc);
     if (rc) return rc;
-    NBX_TRY(nbx::mat_prepare_sym(A->dev, A->stream));
-    info->layout = A->dev.has_sym ? 1 : 0;
-    double* db = A->ws.v64[3];
-    double* dx = A->ws.v64[4];
-    NBX_HIP(hipMemcpyAsync(db, b, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));
-    NBX_HIP(hipMemcpyAsync(dx, x, (size_t)A->N * 8, hipMemcpyHostToDevice, A->stream));
+    NBX_TRY(nbx::mat_prepare_sym(m->dev, m->stream));
+    info->layout = m->dev.has_sym ? 1 : 0;
+    double* db = m->ws.v64[3];
+    double* dx = m->ws.v64[4];
+    NBX_HIP(hipMemcpyAsync(db, b, (size_t)A->N * 8, hipMemcpyHostToDevice, m->stream));
+    NBX_HIP(hipMemcpyAsync(dx, x, (size_t)A->N * 8, hipMemcpyHostToDevice, m->stream));
     nbx::SolveStats st;
     int status = 0;
     if (mode == NBX_SOLVE_REFERENCE || mode == MODE_PLAIN_CG) {
         uint32_t it = 0;
         double tr = 0;
-        status = nbx::pcg_jacobi(A->dev, db, dx, max_iter, tol, &it, &tr, A->ws, A->stream, &st,
+        status = nbx::pcg_jacobi(m->dev, db, dx, max_iter, tol, &it, &tr, m->ws, m->stream, &st,
                                  mode == MODE_PLAIN_CG);
         if (status < 0) return -status;
         info->iters = it;
         info->fp64_iters = it;
         info->tol_reached = tr;
     } else if (mode == NBX_SOLVE_FP64) {
-        status = nbx::solve_fp64(A->dev, db, dx, tol, mo.true_residual, max_iter, A->ws, A->stream,
+        status = nbx::solve_fp64(m->dev, db, dx, tol, mo.true_residual, max_iter, m->ws, m->stream,
                                  &st);
         if (status > 1) return status;
         info->iters = st.iters;
         info->fp64_iters = st.fp64_iters;
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
-        rc = nbx::mat_prepare_mixed(A->dev, A->stream, mo);
+        rc = nbx::mat_prepare_mixed(m->dev, m->stream, mo);
         if (rc) return rc;
-        status = nbx::solve_mixed(A->dev, db, dx, tol, mo, max_iter,
-                                  A->ws, A->stream, &st);
+        status = nbx::solve_mixed(m->dev, db, dx, tol, mo, max_iter, m->ws, m->stream, &st);
         if (status > 1) return status;
         info->layout |= st.layout & 6;   // the preconditioner's fp16 copy
         info->iters = st.iters;
@@ -557,8 +668,8 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
     info->ms = st.ms;
     info->ms_spmv = st.ms_spmv_avg;
     info->status = status;
-    NBX_HIP(hipMemcpyAsync(x, dx, (size_t)A->N * 8, hipMemcpyDeviceToHost, A->stream));
-    NBX_HIP(hipStreamSynchronize(A->stream));
+    NBX_HIP(hipMemcpyAsync(x, dx, (size_t)A->N * 8, hipMemcpyDeviceToHost, m->stream));
+    NBX_HIP(hipStreamSynchronize(m->stream));
     return status;
 }
 

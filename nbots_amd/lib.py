@@ -1,7 +1,16 @@
-"""ctypes bindings of libnbots_amd.so (declared in include/nbots_amd.h).
+"""ctypes bindings of libnbots_amd.so (include/nbots_amd.h) and of
+libnbx_objects.so (include/nbx_objects.h).
 
-The library is built in-tree by ``python -m nbots_amd.build``.  Loading fails
-loudly when it is missing: there is no CPU fallback for any entry point.
+libnbots_amd.so reads the FEM input objects (mesh, element type, material,
+boundary conditions, implicit-scheme parameters) through accessors it
+imports; in a drop-in link libnbots provides them.  Here, without libnbots,
+libnbx_objects.so provides them: it is loaded first, into the global symbol
+scope, and the product library after it.  ``load()`` returns one namespace
+holding the functions of both.
+
+Both libraries are built in-tree by ``python -m nbots_amd.build``.  Loading
+fails loudly when either is missing: there is no CPU fallback for any entry
+point.
 """
 from __future__ import annotations
 
@@ -12,7 +21,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnbots_amd.so")
+OBJECTS_PATH = os.path.join(HERE, "libnbx_objects.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "nbots_amd.h")
+OBJECTS_HEADER = os.path.join(os.path.dirname(HERE), "include", "nbx_objects.h")
 
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -107,6 +118,9 @@ SIGS = {
     "nb_mesh2D_get_invtx": (U32, [VP, U32]),
     "nb_mesh2D_insgm_get_N_nodes": (U32, [VP, U32]),
     "nb_mesh2D_insgm_get_node": (U32, [VP, U32, U32]),
+    "nb_mesh2D_elem_get_N_adj": (U32, [VP, U32]),
+    "nb_mesh2D_insgm_get_length": (F64, [VP, U32]),
+    "nb_mesh2D_insgm_subsgm_get_length": (F64, [VP, U32, U32]),
     "nbx_mesh2D_set_trg": (I32, [VP, U32, f64p, U32, u32p, U32, u32p, U32, u32p, u32p]),
     "nb_fem_elem_create": (VP, [I32]),
     "nb_fem_elem_destroy": (None, [VP]),
@@ -153,6 +167,18 @@ SIGS = {
     "nb_bcond_get_N_dof": (C.c_uint8, [VP]),
     "nb_bcond_push": (None, [VP, I32, I32, U32, C.POINTER(C.c_bool), C.POINTER(C.c_double)]),
     "nb_bcond_push_function": (None, [VP, I32, I32, U32, C.POINTER(C.c_bool), VP]),
+    "nb_bcond_iter_get_memsize": (C.c_uint16, []),
+    "nb_bcond_iter_init": (None, [VP]),
+    "nb_bcond_iter_finish": (None, [VP]),
+    "nb_bcond_iter_create": (VP, []),
+    "nb_bcond_iter_destroy": (None, [VP]),
+    "nb_bcond_iter_set_conditions": (None, [VP, VP, I32, I32]),
+    "nb_bcond_iter_has_more": (C.c_bool, [VP]),
+    "nb_bcond_iter_go_next": (None, [VP]),
+    "nb_bcond_iter_get_id": (U32, [VP]),
+    "nb_bcond_iter_get_mask": (C.c_bool, [VP, C.c_uint8]),
+    "nb_bcond_iter_val_is_function": (C.c_bool, [VP]),
+    "nb_bcond_iter_get_val": (None, [VP, C.c_uint8, f64p, F64, f64p]),
     "nb_fem_compute_2D_Solid_Mechanics": (I32, [VP, VP, VP, VP, C.c_bool, f64o, I32, VP, VP,
                                                 f64p, f64p]),
     "nb_fem_compute_stress_from_strain": (None, [U32, VP, VP, I32, f64p, VP, f64p]),
@@ -193,23 +219,62 @@ SIGS = {
 }
 
 
-def header_symbols():
-    """Function names declared in include/nbots_amd.h."""
+def _declared(path):
+    import re
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbx?_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def import_symbols():
+    """The accessors libnbots_amd.so imports: include/nbots_amd.h between
+    NBX_IMPORTS_BEGIN and NBX_IMPORTS_END."""
     import re
     txt = open(HEADER).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"\b(nbx?_[A-Za-z0-9_]+)\s*\(", txt)
-    return sorted(set(names))
+    sec = txt[txt.index("NBX_IMPORTS_BEGIN"):txt.index("NBX_IMPORTS_END")]
+    sec = re.sub(r"/\*.*?\*/", "", "/*" + sec + "*/", flags=re.S)
+    return sorted(set(re.findall(r"\b(nb_[A-Za-z0-9_]+)\s*\(", sec)))
+
+
+def header_symbols():
+    """Functions libnbots_amd.so defines: those declared in
+    include/nbots_amd.h outside the imports section."""
+    imports = set(import_symbols())
+    return [n for n in _declared(HEADER) if n not in imports]
+
+
+def objects_symbols():
+    """Functions libnbx_objects.so defines: include/nbx_objects.h and the
+    imports section of include/nbots_amd.h."""
+    own = [n for n in _declared(OBJECTS_HEADER) if n not in _declared(HEADER)]
+    return sorted(set(own) | set(import_symbols()))
+
+
+class _Namespace:
+    """The functions of both libraries under one name (product first)."""
+
+    def __init__(self, product, objects):
+        self._product = product
+        self._objects = objects
+
+    def __getattr__(self, name):
+        try:
+            return getattr(self._product, name)
+        except AttributeError:
+            return getattr(self._objects, name)
 
 
 def load(path: str = LIB_PATH):
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not os.path.exists(path):
-        raise RuntimeError(f"{path} is missing: build it with `python -m nbots_amd.build` "
-                           "(the HIP library is required; there is no CPU fallback)")
-    L = C.CDLL(path)
+    for p in (OBJECTS_PATH, path):
+        if not os.path.exists(p):
+            raise RuntimeError(f"{p} is missing: build it with `python -m nbots_amd.build` "
+                               "(the HIP library is required; there is no CPU fallback)")
+    objects = C.CDLL(OBJECTS_PATH, mode=C.RTLD_GLOBAL)
+    product = C.CDLL(path)
+    L = _Namespace(product, objects)
     for name, (res, args) in SIGS.items():
         f = getattr(L, name)
         f.restype = res

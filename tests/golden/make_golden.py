@@ -240,7 +240,8 @@ def c2_case(nx=1024, ny=512, name="c2_reference_solve", step=101):
     reference's own assembly and boundary conditions, tolerance 1e-8 ||F||
     (8 OpenMP threads; the iteration is the same arithmetic as one thread
     except for the dot-product order).  Kept: every step-th displacement,
-    ||u||, the iteration count, the residual, SHA-256 of K and F."""
+    ||u||, F.u, the values w_k.u for the 8 vectors of synth.projections,
+    the iteration count, the residual, SHA-256 of K and F."""
     import time
     p = synth.cantilever_traction(nx, ny)
     t = time.time()
@@ -250,10 +251,12 @@ def c2_case(nx=1024, ny=512, name="c2_reference_solve", step=101):
     x, it, tr, cst = oracle.ref_pcg(rp, col, val, F, tol=tol, max_iter=p.N, threads=8)
     r = F - oracle.ref_spmv(rp, col, val, x, threads=8)
     idx = np.arange(0, p.N, step)
+    W = synth.projections(p.N)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), idx=idx, u=x[idx],
                         u_norm=np.linalg.norm(x), niter=it, status=cst, tol=tol,
                         tol_reached=tr, rel_res=np.linalg.norm(r) / np.linalg.norm(F),
-                        F_norm=np.linalg.norm(F), K_sha=sha(val), F_sha=sha(F))
+                        F_norm=np.linalg.norm(F), K_sha=sha(val), F_sha=sha(F),
+                        F_dot_u=float(F @ x), proj=W @ x, x_sha=sha(x))
     print(f"{name}: {p.N} dof, niter {it}, status {cst}, "
           f"rel res {np.linalg.norm(r) / np.linalg.norm(F):.3e}, {time.time() - t:.0f}s")
 
