@@ -8,18 +8,27 @@ Jacobi-PCG solve (the reference's algorithm) from u = 0 to relative residual
 ||F - K u|| / ||F|| <= 1e-8.
 
 The JSON line reports DOF * CG-iteration / s over the timed steps (whole
-job), the time to 1e-8 of FP64 PCG and of the mixed-precision solver
-(FP32 inner work, FP64 residual and update) on the same GPU, the HBM
-roofline of the dominant kernel (the FP64 SpMV with its p.w partials, timed
-with HIP events on the solver's stream), and the reference's own CPU
-Jacobi-PCG timed on this host over a bounded number of iterations.  A SIMP
-topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF, 50
-iterations) gives the ms per SIMP iteration with mixed-precision and with
-FP64 solves, and the largest relative difference of their compliance
-histories.
+job), the HBM roofline of the dominant kernel (the FP64 SpMV with its p.w
+partials, timed with HIP events on the solver's stream), the reference's
+own CPU Jacobi-PCG timed on this host over a bounded number of iterations,
+and, timed separately on the same GPU, time to 1e-8 of:
+  * FP64 Jacobi-PCG (the steps above);
+  * FP32 CG sweeps (inner relative tolerance 1e-4) inside FP64 iterative
+    refinement, the scheme the workload names;
+  * FP64 flexible CG with a degree-40 Chebyshev-Jacobi preconditioner whose
+    matrix values, vectors and arithmetic are all FP64;
+  * the same with the preconditioner's matrix in fp16 and its vectors in
+    FP32 (FP64 row sums): the library's default mixed-precision solver.
+The preconditioner factor (Jacobi-PCG / FP64-preconditioner FCG) and the
+precision factor (FP64-preconditioner FCG / fp16-preconditioner FCG) are
+reported apart; their product is the mixed solver's speedup.  A SIMP
+topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF,
+50 iterations) gives ms per SIMP iteration with each solver.
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu] [--mixed cheb]
-                    [--simp-iters 50] [--no-simp]
+    python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu]
+                    [--simp-iters 50] [--no-simp] [--no-decompose]
+    python bench.py --cpu-full        # the CPU baseline plan at full size
+    python bench.py --pmc-probe [--probe-mixed]   # a short run for rocprofv3 --pmc
 """
 from __future__ import annotations
 
@@ -33,8 +42,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-
-from nbots_amd.probe import spmv_bytes, sym_spmv_bytes  # noqa: E402  (algorithmic bytes of one SpMV)
 
 METRIC = ("DOF·CG-iter/s + time-to-1e-8 residual (FP32+IR vs FP64); "
           "ms per SIMP iteration")
@@ -56,114 +63,404 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# ---- algorithmic bytes ---------------------------------------------------------
+
+def csr_spmv_bytes(n: int, nnz: int, sv: int = 8) -> int:
+    """One SpMV on scalar CSR with 4-byte columns and row pointers: every
+    value and column once, the row pointers, x read and y written once."""
+    return nnz * (sv + 4) + 4 * (n + 1) + 2 * n * sv
+
+
+def spmv_bytes(n_nodes: int, n_blocks: int, prec: int) -> int:
+    """One 2x2-block sliced-ELL SpMV: per valid block its column (4 B) and 4
+    values; per node its row length (1 B) and the x read and y write of 2
+    values; per 64-node slice its offset.  prec 64, 32 or 16 (fp16 values
+    with FP32 vectors)."""
+    sv = {64: 8, 32: 4, 16: 2}[prec]          # matrix value bytes
+    sx = 8 if prec == 64 else 4               # vector value bytes (fp16 values: FP32 vectors)
+    return n_blocks * (4 * sv + 4) + n_nodes + 4 * ((n_nodes + 63) // 64) + 2 * (2 * n_nodes) * sx
+
+
+def sym_spmv_bytes(n_nodes: int, n_valid: int) -> int:
+    """One FP64 SpMV in the symmetric layout: per node its diagonal block's 3
+    values (24 B), two row lengths, the x read and the y write; per upper
+    block (v, w > v) its column and 4 values (36 B); per lower block its
+    (column, upper position) pair (8 B; the transposed values are those of
+    upper blocks streamed by nearby workgroups, served by L2); per 64-node
+    slice two offsets.  With every diagonal block stored, upper = lower =
+    (valid - nodes) / 2 blocks."""
+    up = lo = (n_valid - n_nodes) // 2
+    return (n_nodes * 24 + up * 36 + lo * 8 + 2 * n_nodes + 8 * ((n_nodes + 63) // 64)
+            + 2 * (2 * n_nodes) * 8)
+
+
+def cheb_step_bytes(n_nodes: int, n_valid: int, adds_z: bool) -> int:
+    """One Chebyshev step on the fp16 copy without diagonal blocks: per
+    off-diagonal block 8 B of values and a 2-B packed column; per node the
+    diagonal pair (4 B), the row length (1 B), the d gather (8 B), the
+    residual read and write (16 B), the new d (8 B) and, on the steps that
+    add to z, its read and write (16 B); per 64-node slice two offsets."""
+    off = n_valid - n_nodes
+    per_node = 4 + 1 + 8 + 16 + 8 + (16 if adds_z else 0)
+    return off * 10 + n_nodes * per_node + 8 * ((n_nodes + 63) // 64)
+
+
 def pcg_iter_bytes(n_nodes, n_blocks, prec, sym=False):
-    """Algorithmic bytes of one CG iteration as implemented: SpMV kernel
-    (above) + update kernel (FP64: read g,w,diag, write g = 4 vectors;
-    FP32 scaled: read y,p,r,w, write y,r = 6) + direction kernel (FP64: read
-    p,g,diag and write the next p = 4 vectors, and on every fourth iteration
-    also read the three previous p and read and write x = 9; FP32: read r,p,
-    write p = 3)."""
+    """One CG iteration as implemented: SpMV kernel (above) + update kernel
+    (FP64: read g,w,diag, write g = 4 vectors; FP32 scaled: read y,p,r,w,
+    write y,r = 6) + direction kernel (FP64: read p,g,diag and write the
+    next p = 4 vectors, and on every fourth iteration also read the three
+    previous p and read and write x = 9; FP32: read r,p, write p = 3)."""
     sv = 8 if prec == 64 else 4
     nv = 9.25 if prec == 64 else 9
     spmv = sym_spmv_bytes(n_nodes, n_blocks) if sym else spmv_bytes(n_nodes, n_blocks, prec)
     return spmv + nv * (2 * n_nodes) * sv
 
 
-def cpu_baseline(f, p, seconds, threads):
-    """The reference's nb_sparse_solve_CG_precond_Jacobi (compiled from its
-    own sources into oracle/_ref) on the same K and F, cut after about
-    `seconds` of work (10 calibration iterations decide the count), on
-    `threads` OpenMP threads and again on 1 (the reference FEM driver's own
-    setting, static_elasticity2D.c:88-90) for a quarter of the time; falls
-    back to this repo's C restatement of it (1 thread)."""
+# ---- host description ---------------------------------------------------------
+
+def host_info():
+    """nproc, the CPUs this process may run on, OMP_NUM_THREADS and the CPU
+    model.  A shared GPU host allots each GPU's job a share of its cores
+    (the affinity set and OMP_NUM_THREADS say how many)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return dict(nproc=os.cpu_count(), affinity=aff,
+                omp_num_threads=int(omp) if omp and omp.isdigit() else None, cpu_model=model)
+
+
+def default_cpu_threads():
+    h = host_info()
+    n = h["affinity"]
+    if h["omp_num_threads"]:
+        n = min(n, h["omp_num_threads"])
+    return max(1, n)
+
+
+# ---- CPU baseline -----------------------------------------------------------------
+
+def _ref_pcg_runner(p, rp, col, val, F):
+    """(run(n_iter, threads) -> seconds, kind, release): the reference's
+    nb_sparse_solve_CG_precond_Jacobi compiled from its own sources into
+    oracle/_ref, or this repository's C restatement of it (1 thread)."""
     import ctypes as C
     import oracle
-    rp, col, val, F = f.system()
-    kind = "reference"
+    it, tr = C.c_uint32(0), C.c_double(0)
     try:
         R = oracle.ref()
     except Exception:
         R = None
-        kind = "port"
-    it, tr = C.c_uint32(0), C.c_double(0)
     if R is not None:
         h = R.ref_sparse_import(p.N, rp, col, val)
 
-        def run(n, nt):
+        def run(n, nt, tol=0.0):
             x = np.zeros(p.N)
             t0 = time.perf_counter()
-            R.ref_pcg_handle(h, F, x, n, 0.0, nt, C.byref(it), C.byref(tr))
-            return time.perf_counter() - t0
-    else:
+            R.ref_pcg_handle(h, F, x, n, tol, nt, C.byref(it), C.byref(tr))
+            return time.perf_counter() - t0, it.value, x
+        return run, "reference", lambda: R.ref_sparse_free(h)
+
+    def run(n, nt, tol=0.0):
+        x = np.zeros(p.N)
+        t0 = time.perf_counter()
+        oracle.lib().orc_pcg_jacobi(p.N, rp, col, val, F, x, n, tol, C.byref(it), C.byref(tr))
+        return time.perf_counter() - t0, it.value, x
+    return run, "port", lambda: None
+
+
+def cpu_baseline(f, p, seconds, threads):
+    """The reference's Jacobi-PCG on the same K and F, cut after about
+    `seconds` of work (10 calibration iterations decide the count), on
+    `threads` OpenMP threads and again on 1 (the reference FEM driver's own
+    setting, static_elasticity2D.c:88-90) for a quarter of the time."""
+    rp, col, val, F = f.system()
+    run, kind, release = _ref_pcg_runner(p, rp, col, val, F)
+    if kind == "port":
         threads = 1
 
-        def run(n, nt):
-            x = np.zeros(p.N)
-            t0 = time.perf_counter()
-            oracle.lib().orc_pcg_jacobi(p.N, rp, col, val, F, x, n, 0.0, C.byref(it), C.byref(tr))
-            return time.perf_counter() - t0
-
     def sample(secs, nt):
-        t_cal = run(10, nt)
+        t_cal, _, _ = run(10, nt)
         n = int(max(10, min(20000, secs / max(t_cal / 10, 1e-6))))
-        dt = run(n, nt)
-        return it.value, dt
+        dt, n_it, _ = run(n, nt)
+        return n_it, dt
 
     n_it, dt = sample(seconds, threads)
     one = None
-    if R is not None and threads > 1:
+    if kind == "reference" and threads > 1:
         n1, dt1 = sample(seconds / 4, 1)
         one = dict(value=p.N * n1 / dt1, iters=n1, seconds=dt1)
-    if R is not None:
-        R.ref_sparse_free(h)
+    release()
     return dict(value=p.N * n_it / dt, unit="DOF*CG-iter/s", cores=threads, kind=kind,
                 sample=f"{n_it} Jacobi-PCG iterations on the same {p.N}-DOF system "
                        f"(reference build gcc -O2, {threads} OpenMP threads), {dt:.2f} s; "
                        f"time to 1e-8 extrapolated from the FP64 GPU iteration count",
-                seconds=dt, iters=n_it, one_thread=one)
+                seconds=dt, iters=n_it, one_thread=one, host=host_info())
 
 
-def simp_bench(args, opts):
+def cpu_full(args):
+    """The CPU baseline plan at full size, on this host's cores (no GPU):
+    C1 64x32 jittered cantilever, the reference driver's solve (absolute
+    tolerance 1e-8, 1 thread); C2 1024x512: the reference's pattern,
+    assembly and BCs, then its Jacobi-PCG to relative 1e-8 on `threads` and
+    on 1 thread; C5 1024x512 MBB from rho = 0.5: one SIMP iteration (the
+    reference's assembly with E(xPhys) and its Jacobi-PCG to relative 1e-8,
+    then this repository's C restatement of the element energies,
+    sensitivities, filter and OC update)."""
+    import oracle
+    from nbots_amd import synth
+    threads = args.cpu_threads or default_cpu_threads()
+    out = dict(host=host_info(), threads=threads, build="gcc -O2 -std=c99 -fopenmp "
+               "-ffp-contract=off (reference sources, oracle/Makefile)")
+
+    def timed(fn, *a, **k):
+        t0 = time.perf_counter()
+        r = fn(*a, **k)
+        return r, time.perf_counter() - t0
+
+    # C1
+    p = synth.cantilever_point(64, 32, jitter=0.1)
+    res, t = timed(oracle.ref_fem_compute, p)
+    out["c1"] = dict(workload="64x32 jittered cantilever, 4,290 DOF, the reference driver "
+                              "(pattern, assembly, BCs, PCG to absolute 1e-8, strain), 1 thread",
+                     seconds=round(t, 4), status=int(res[0]))
+    log(f"[cpu-full] C1 driver {t:.3f}s")
+    # C2
+    p = synth.cantilever_traction(1024, 512, seed=args.seed)
+    rp, col, val, F, st, t_pat, t_asm = oracle.ref_assemble_timed(p)
+    assert st == 0
+    tol = 1e-8 * float(np.linalg.norm(F))
+    run, kind, release = _ref_pcg_runner(p, rp, col, val, F)
+    c2 = dict(workload=f"1024x512 lognormal cantilever, {p.N} DOF", kind=kind,
+              pattern_s=round(t_pat, 3), pattern_assembly_bc_s=round(t_asm, 3),
+              assembly_bc_s=round(t_asm - t_pat, 3),
+              note="pattern = mesh graph + nb_sparse_create + CSR export; assembly with "
+                   "per-element moduli through the reference's element kernels")
+    for nt in sorted({threads, 1}, reverse=True):
+        dt, it, x = run(p.N, nt, tol)
+        rel = float(np.linalg.norm(F - oracle.ref_spmv(rp, col, val, x, threads=nt)) /
+                    np.linalg.norm(F))
+        c2[f"solve_{nt}t"] = dict(seconds=round(dt, 3), niter=it, rel_res=rel,
+                                  dof_iter_per_s=p.N * it / dt)
+        log(f"[cpu-full] C2 PCG {nt} threads: {it} it in {dt:.1f}s (rel res {rel:.2e})")
+    release()
+    out["c2"] = c2
+    # C5, one SIMP iteration
+    p = synth.mbb(1024, 512, rho=0.5)
+    cx, cy = synth.elem_centroids(p)
+    H, t_filter = timed(oracle.filter_build, cx, cy, 1.5)
+    x = np.full(p.N_elem, 0.5)
+    xp = oracle.filter_apply(H, x)
+    q = synth.mbb(1024, 512, rho=xp)
+    rp, col, val, F, st, _, t_asm = oracle.ref_assemble_timed(q)
+    tol = 1e-8 * float(np.linalg.norm(F))
+    run, kind, release = _ref_pcg_runner(q, rp, col, val, F)
+    t_solve, it, u = run(q.N * 20, threads, tol)
+    release()
+    t0 = time.perf_counter()
+    ce = oracle.elem_energy(q, u)
+    C_, dc = oracle.simp_sens(xp, ce)
+    dcf = oracle.filter_sens(H, dc)
+    dvf = oracle.filter_sens(H, synth.elem_areas(q))
+    xnew, _, ch = oracle.oc_update(H, synth.elem_areas(q), x, dcf, dvf, 0.5, 0.2)
+    t_sens = time.perf_counter() - t0
+    out["c5"] = dict(workload=f"1024x512 MBB half-beam from rho = 0.5, {q.N} DOF, first SIMP "
+                              "iteration", kind=kind,
+                     assembly_bc_s=round(t_asm, 3), solve_s=round(t_solve, 3), solve_niter=it,
+                     solve_threads=threads,
+                     sens_filter_oc_s=round(t_sens, 3), sens_kind="port (1 thread)",
+                     filter_build_s=round(t_filter, 3),
+                     ms_per_simp_iter=round(1e3 * (t_asm + t_solve + t_sens), 1),
+                     compliance=float(C_))
+    log(f"[cpu-full] C5 SIMP iteration: assembly {t_asm:.2f}s, solve {t_solve:.1f}s "
+        f"({it} it), sens/filter/OC {t_sens:.2f}s")
+    print(json.dumps(out), flush=True)
+
+
+# ---- counter probe ------------------------------------------------------------------
+
+def pmc_probe(args):
+    """A short FP64 Jacobi-PCG run (or, with --probe-mixed, outer iterations
+    of the mixed solver) on the cantilever, then a plain elementwise kernel
+    of known size, for hardware-counter runs:
+
+      rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'k_pcg_A|elementwise' \\
+          -d <dir> -- python bench.py --pmc-probe
+
+    The elementwise b = 2a reads and writes exactly --copy-mib MiB, so its
+    counter reading calibrates the counter for the solver's kernels."""
+    import torch
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, SOLVE_FP64, SOLVE_MIXED
+    cfg = CONFIGS[args.config]
+    p = synth.cantilever_traction(cfg["nx"], cfg["ny"], seed=args.seed)
+    f = Fem.from_problem(p)
+    N, nnz, nblk = f.sizes()
+    n_nodes, n_valid = N // 2, nnz // 4
+    print(f"[probe] {cfg['nx']}x{cfg['ny']} cantilever: {N} DOF, {n_valid} blocks, "
+          f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64, full layout), "
+          f"{sym_spmv_bytes(n_nodes, n_valid)} (symmetric layout)", flush=True)
+    if args.probe_mixed:
+        print(f"[probe] Chebyshev step algorithmic bytes {cheb_step_bytes(n_nodes, n_valid, True)} "
+              f"(adding to z), {cheb_step_bytes(n_nodes, n_valid, False)} (not)", flush=True)
+        info = f.solve(SOLVE_MIXED, tol=0.0, max_iter=args.probe_iters)
+        print(f"[probe] mixed: {info['iters']} outer iterations, layout {info['layout']}", flush=True)
+    else:
+        info = f.solve(SOLVE_FP64, tol=0.0, max_iter=args.probe_iters)
+        print(f"[probe] FP64 PCG: {info['iters']} iterations, layout {info['layout']}", flush=True)
+    f.destroy()
+    # b = 2 a: an elementwise kernel that reads and writes every byte once
+    # (a same-type copy_ would go to the runtime's blit path instead)
+    n = args.copy_mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(f"[probe] elementwise b = 2a: 3 launches of {n * 4} bytes read + {n * 4} bytes written",
+          flush=True)
+
+
+# ---- GPU solvers --------------------------------------------------------------------
+
+def time_solver(f, mode, opts, reps, label, max_iter=200000):
+    """One untimed solve, then `reps` timed ones; returns (median seconds,
+    last info)."""
+    f.solve(mode, tol=1e-8, max_iter=max_iter, opts=opts)
+    ts, info = [], None
+    for s in range(reps):
+        t0 = time.perf_counter()
+        info = f.solve(mode, tol=1e-8, max_iter=max_iter, opts=opts)
+        ts.append(time.perf_counter() - t0)
+        log(f"[bench] {label} {s}: {info['iters']} it, {info['outer']} inner, {ts[-1]:.3f}s, "
+            f"rel_res {info['rel_res']:.2e}, fallback {info['fallback']}, status {info['status']}")
+    return float(np.median(ts)), info
+
+
+def decompose(f, args, t_fp64, iters64, u64, Fl):
+    """The mixed-precision speedup split into the preconditioner's gain and
+    the precision's, plus the FP32-CG + FP64-refinement scheme."""
+    from nbots_amd.fem import SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART, solve_opts
+    out = {}
+    o16 = solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+                     precond_bits=args.precond_bits, true_residual=True)
+    t16, i16 = time_solver(f, SOLVE_MIXED, o16, 2, "mixed (fp16 preconditioner)")
+    u16 = f.displacement()
+    true16 = (f.true_residual(True), f.true_residual(False))
+    out.update(
+        time_to_1e8_fp64_s=round(t_fp64, 4),
+        time_to_1e8_mixed_s=round(t16, 4),
+        mixed_speedup_vs_fp64=round(t_fp64 / t16, 3),
+        mixed_solver=("FP64 flexible CG, Chebyshev-Jacobi preconditioner (degree "
+                      f"{o16.cheb_degree} on [lmax/{o16.cheb_ratio:g}, lmax]) on "
+                      f"{'fp16' if o16.precond_bits == 16 else 'FP32'} matrix values, FP32 vectors, "
+                      "FP64 row sums"),
+        mixed_precond_matrix=(("fp16 copy without diagonal blocks"
+                               + (", 16-bit column deltas" if i16["layout"] & 4 else ""))
+                              if i16["layout"] & 2 else "full scaled copy"),
+        mixed_iters=i16["iters"], mixed_inner=i16["outer"], mixed_rel_res=i16["rel_res"],
+        mixed_true_rel_res=true16[0], mixed_true_rel_res_fp64_eval=true16[1],
+        mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u16 - u64) / np.linalg.norm(u64)),
+        mixed_vs_fp64_compliance_rel=abs(float(Fl @ u16) - float(Fl @ u64)) / abs(float(Fl @ u64)))
+    if args.no_decompose:
+        return out
+    o64 = solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+                     precond_bits=64, true_residual=True)
+    t64, i64 = time_solver(f, SOLVE_MIXED, o64, 2, "FCG, FP64 preconditioner")
+    true64p = f.true_residual(True)
+    oir = solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, true_residual=True)
+    tir, iir = time_solver(f, SOLVE_MIXED, oir, 1, "FP32 CG + FP64 refinement",
+                           max_iter=args.mixed_max_iter)
+    trueir = f.true_residual(True)
+    out.update(
+        decomposition=dict(
+            jacobi_pcg_fp64=dict(seconds=round(t_fp64, 4), iters=iters64),
+            cheb_fcg_fp64_precond=dict(
+                seconds=round(t64, 4), iters=i64["iters"], precond_spmvs=i64["outer"],
+                true_rel_res=true64p,
+                solver="FP64 flexible CG, Chebyshev-Jacobi preconditioner on the FP64 matrix, "
+                       "FP64 vectors and arithmetic"),
+            cheb_fcg_fp16_precond=dict(seconds=round(t16, 4), iters=i16["iters"],
+                                       precond_spmvs=i16["outer"], true_rel_res=true16[0]),
+            preconditioner_factor=round(t_fp64 / t64, 3),
+            precision_factor=round(t64 / t16, 3),
+            note="mixed_speedup_vs_fp64 = preconditioner_factor x precision_factor"),
+        time_to_1e8_fp32ir_s=round(tir, 4),
+        fp32ir=dict(seconds=round(tir, 4), speedup_vs_fp64=round(t_fp64 / tir, 3),
+                    inner_fp32_iters=iir["iters"], refinement_sweeps=iir["outer"],
+                    fp64_pcg_iters=iir["fp64_iters"], fallback_to_fp64=bool(iir["fallback"]),
+                    status=iir["status"], rel_res=iir["rel_res"], true_rel_res=trueir,
+                    solver="FP32 CG sweeps to inner relative residual 1e-4 on the FP32 "
+                           "Jacobi-scaled matrix, FP64 residual and update; a sweep whose "
+                           "true FP64 residual does not fall hands over to FP64 PCG"))
+    return out
+
+
+def simp_bench(args, opts16):
     """1024x512 MBB half-beam (1,051,650 DOF), volume fraction 0.5, penalty 3,
     density filter radius 1.5, OC move 0.2, from rho = 0.5: the SIMP loop
-    run with mixed-precision solves and again with FP64 solves (both to
-    relative residual 1e-8, warm-started from the previous displacement)."""
+    with each solver (all to relative residual 1e-8, warm-started from the
+    previous displacement): the fp16-preconditioner flexible CG, the same
+    with an FP64 preconditioner, FP64 Jacobi-PCG, and, for the first
+    --simp-fp32ir-iters iterations, FP32 CG in FP64 refinement."""
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED, solve_opts
+    from nbots_amd.fem import (Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG,
+                               MIXED_CG_RESTART, solve_opts)
     nx, ny = 1024, 512
     p = synth.mbb(nx, ny, rho=0.5)
     out = dict(simp_workload=f"{nx}x{ny} MBB half-beam, {p.N} DOF, {p.N_elem} T3, volfrac 0.5, "
                              "penal 3, filter radius 1.5, OC move 0.2, rho0 = 0.5",
                simp_iterations=args.simp_iters)
+    variants = [("mixed", SOLVE_MIXED, opts16, args.simp_iters),
+                ("fp64", SOLVE_FP64, solve_opts(true_residual=True, simp=True), args.simp_iters)]
+    if not args.no_decompose:
+        variants.insert(1, ("fcg_fp64_precond", SOLVE_MIXED,
+                            solve_opts(MIXED_CHEB_FCG, precond_bits=64, simp=True,
+                                       true_residual=True), args.simp_iters))
+        if args.simp_fp32ir_iters > 0:
+            variants.append(("fp32ir", SOLVE_MIXED,
+                             solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, simp=True,
+                                        true_residual=True), args.simp_fp32ir_iters))
     runs = {}
-    o64 = solve_opts(true_residual=True, simp=True)
-    for name, mode, o in (("mixed", SOLVE_MIXED, opts), ("fp64", SOLVE_FP64, o64)):
+    for name, mode, o, n_it in variants:
         f = Fem.from_problem(p)
         t0 = time.perf_counter()
         s = Simp(f, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2)
         t_setup = time.perf_counter() - t0
-        hist, ch, its, solve_ms = [], [], [], []
-        t_loop0 = time.perf_counter()
-        for it in range(args.simp_iters):
+        hist, ch, its, solve_ms, it_s, fb = [], [], [], [], [], 0
+        for it in range(n_it):
+            t0 = time.perf_counter()
             C, change, info = s.iterate(mode, tol=1e-8, opts=o)
+            it_s.append(time.perf_counter() - t0)
             hist.append(C)
             ch.append(change)
             its.append(info["iters"])
             solve_ms.append(info["ms"])
-            if it % 10 == 0 or it == args.simp_iters - 1:
+            fb += int(info["fallback"] != 0 and name == "fp32ir")
+            if it % 10 == 0 or it == n_it - 1:
                 log(f"[bench] SIMP {name} it {it}: C {C:.6e} change {change:.4f} "
                     f"solve {info['iters']} it {info['ms']:.1f} ms rel_res {info['rel_res']:.1e}")
-        t_loop = time.perf_counter() - t_loop0
-        runs[name] = dict(hist=np.array(hist), ms=1e3 * t_loop / args.simp_iters,
-                          solve_ms=float(np.mean(solve_ms)), iters=its, setup_s=t_setup,
-                          change=ch)
+        runs[name] = dict(hist=np.array(hist), it_s=np.array(it_s), solve_ms=float(np.mean(solve_ms)),
+                          iters=its, setup_s=t_setup, change=ch, fallbacks=fb)
         s.destroy()
         f.destroy()
     m, d = runs["mixed"], runs["fp64"]
     rel = np.abs(m["hist"] - d["hist"]) / np.abs(d["hist"])
+    ms = {k: 1e3 * float(v["it_s"].mean()) for k, v in runs.items()}
     out.update(
-        ms_per_simp_iter=round(m["ms"], 2),
-        ms_per_simp_iter_fp64=round(d["ms"], 2),
+        ms_per_simp_iter=round(ms["mixed"], 2),
+        ms_per_simp_iter_fp64=round(ms["fp64"], 2),
         simp_solve_ms_mean=round(m["solve_ms"], 2),
         simp_solve_ms_mean_fp64=round(d["solve_ms"], 2),
         simp_filter_setup_s=round(m["setup_s"], 3),
@@ -172,6 +469,27 @@ def simp_bench(args, opts):
         simp_compliance_first_last=[float(d["hist"][0]), float(d["hist"][-1])],
         simp_compliance_hist_max_rel_diff=float(rel.max()),
         simp_change_last=float(m["change"][-1]))
+    if "fcg_fp64_precond" in runs:
+        q = runs["fcg_fp64_precond"]
+        out["simp_decomposition"] = dict(
+            ms_per_iter=dict(jacobi_pcg_fp64=round(ms["fp64"], 2),
+                             cheb_fcg_fp64_precond=round(ms["fcg_fp64_precond"], 2),
+                             cheb_fcg_fp16_precond=round(ms["mixed"], 2)),
+            preconditioner_factor=round(ms["fp64"] / ms["fcg_fp64_precond"], 3),
+            precision_factor=round(ms["fcg_fp64_precond"] / ms["mixed"], 3),
+            fcg_fp64_precond_hist_max_rel_diff=float(
+                (np.abs(q["hist"] - d["hist"]) / np.abs(d["hist"])).max()))
+    if "fp32ir" in runs:
+        r = runs["fp32ir"]
+        n = len(r["it_s"])
+        first = {k: round(1e3 * float(v["it_s"][:n].mean()), 2) for k, v in runs.items()}
+        out["simp_fp32ir"] = dict(
+            iterations=n, ms_per_iter_first_n=first,
+            solver_iters_mean=round(float(np.mean(r["iters"])), 1),
+            solves_handed_to_fp64=r["fallbacks"],
+            hist_max_rel_diff_vs_fp64=float(
+                (np.abs(r["hist"] - d["hist"][:n]) / np.abs(d["hist"][:n])).max()),
+            note=f"ms per iteration over the first {n} iterations for every solver")
     return out
 
 
@@ -183,27 +501,40 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--mixed", default="cheb", choices=["cheb", "restart", "reliable"])
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="default: the cores this job may use (affinity, OMP_NUM_THREADS)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="the CPU baseline plan at full size instead (no GPU)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="a short solve and a calibration kernel for rocprofv3 --pmc")
+    ap.add_argument("--probe-mixed", action="store_true")
+    ap.add_argument("--probe-iters", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-decompose", action="store_true",
+                    help="skip the FP64-preconditioner and FP32-CG + FP64-refinement runs")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
-    ap.add_argument("--cheb-degree", type=int, default=None,
-                    help="library default 40")
-    ap.add_argument("--cheb-ratio", type=float, default=None,
-                    help="library default 3e4")
+    ap.add_argument("--cheb-degree", type=int, default=None, help="library default 40")
+    ap.add_argument("--cheb-ratio", type=float, default=None, help="library default 3e4")
     ap.add_argument("--precond-bits", type=int, default=16, choices=[16, 32])
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--simp-iters", type=int, default=50)
+    ap.add_argument("--simp-fp32ir-iters", type=int, default=10)
     ap.add_argument("--no-simp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_full:
+        cpu_full(args)
+        return
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
 
     from nbots_amd import synth
-    from nbots_amd.fem import (Fem, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART,
-                               MIXED_CG_RELIABLE, solve_opts)
+    from nbots_amd.fem import Fem, SOLVE_FP64, MIXED_CHEB_FCG, solve_opts
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
@@ -224,9 +555,9 @@ def main():
 
     # Steps: FP64 Jacobi-PCG solves to 1e-8 (the reference's algorithm,
     # device resident); value = DOF * CG iterations / s over the timed steps.
-    # both solvers stop on the true residual ||F - K u|| / ||F|| <= 1e-8
+    # The solvers stop on the true residual ||F - K u|| / ||F|| <= 1e-8
     # (restarting from the iterate when their recursively updated residual
-    # has drifted from it)
+    # has drifted from it).
     o64 = solve_opts(true_residual=True)
     for w in range(args.warmup):
         info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000, opts=o64)
@@ -250,6 +581,7 @@ def main():
     true64 = (f.true_residual(True), f.true_residual(False))
     value = N * sum(iters) / t_wall
     ms_step = 1e3 * t_wall / args.steps
+    t_fp64 = float(np.median(times))
 
     # dominant kernel: the FP64 SpMV (+ p.w partials), sampled with HIP
     # events on the solver's stream during the timed steps
@@ -264,54 +596,25 @@ def main():
                     kernel=("k_pcg_A<OpSym> (FP64 SpMV, symmetric 2x2-block sliced ELL, "
                             "+ p.w partials)" if sym else
                             "k_pcg_A<OpS2> (FP64 2x2-block sliced-ELL SpMV + p.w partials)"),
-                    bytes_per_launch=bk, us_per_launch=round(spmv_ms * 1e3, 2))
+                    bytes_per_launch=bk, us_per_launch=round(spmv_ms * 1e3, 2),
+                    byte_model=("symmetric 2x2-block sliced ELL: 24 B per node for the "
+                                "diagonal block, 36 B per upper block (column + 4 FP64 values), "
+                                "8 B per lower block (column + position of its upper partner, "
+                                "values read transposed from L2), x and y 16 B per node each"
+                                if sym else
+                                "2x2-block sliced ELL: 36 B per block, x and y 16 B per node"),
+                    csr_bytes_per_launch=csr_spmv_bytes(N, nnz),
+                    csr_note="the same product on scalar CSR with 4-byte indices (the survey's "
+                             "byte model); this layout moves fewer bytes for the same result")
 
-    # mixed precision, timed separately: time to 1e-8 against the FP64 steps
     extra = {}
     if not args.no_mixed:
-        variant = dict(cheb=MIXED_CHEB_FCG, restart=MIXED_CG_RESTART,
-                       reliable=MIXED_CG_RELIABLE)[args.mixed]
-        opts = solve_opts(variant, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
-                          precond_bits=args.precond_bits, true_residual=True)
-        f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
-        mtimes, minfos = [], []
-        for s in range(2):
-            t0 = time.perf_counter()
-            info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=args.mixed_max_iter, opts=opts)
-            mtimes.append(time.perf_counter() - t0)
-            minfos.append(info)
-            log(f"[bench] mixed ({args.mixed}) {s}: {info['iters']} it, {info['outer']} inner, "
-                f"{mtimes[-1]:.3f}s, rel_res {info['rel_res']:.2e}, fallback {info['fallback']}")
-        u32 = f.displacement()
-        true32 = (f.true_residual(True), f.true_residual(False))
-        Fl = f.load_vector()
-        c64, c32 = float(Fl @ u64), float(Fl @ u32)
-        t_mixed = float(np.median(mtimes))
-        t_fp64 = float(np.median(times))
+        extra = decompose(f, args, t_fp64, iters[-1], u64, f.load_vector())
         ms32 = f.spmv_ms(32, 20)
         ms16 = f.spmv_ms(16, 20)
         ms64 = f.spmv_ms(64, 20)
         ms65 = f.spmv_ms(65, 20) if sym else None
-        extra = dict(
-            time_to_1e8_fp64_s=round(t_fp64, 4),
-            time_to_1e8_mixed_s=round(t_mixed, 4),
-            mixed_speedup_vs_fp64=round(t_fp64 / t_mixed, 3),
-            mixed_variant=args.mixed,
-            mixed_solver=("FP64 flexible CG, Chebyshev-Jacobi preconditioner in FP32 arithmetic on "
-                          f"{'fp16' if opts.precond_bits == 16 else 'FP32'} matrix values "
-                          f"(degree {opts.cheb_degree}, [lmax/{opts.cheb_ratio:g}, lmax])"
-                          if variant == MIXED_CHEB_FCG else
-                          "FP32 CG sweeps (inner rtol 1e-4) in FP64 iterative refinement"
-                          if variant == MIXED_CG_RESTART else
-                          "FP32 CG with FP64 reliable updates"),
-            mixed_precond_matrix=(("fp16 copy without diagonal blocks"
-                                   + (", 16-bit column deltas" if minfos[-1]["layout"] & 4 else ""))
-                                  if minfos[-1]["layout"] & 2 else "full scaled copy"),
-            mixed_iters=minfos[-1]["iters"], mixed_inner=minfos[-1]["outer"],
-            mixed_rel_res=minfos[-1]["rel_res"], mixed_fallback=minfos[-1]["fallback"],
-            mixed_true_rel_res=true32[0], mixed_true_rel_res_fp64_eval=true32[1],
-            mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
-            mixed_vs_fp64_compliance_rel=abs(c32 - c64) / abs(c64),
+        extra.update(
             spmv64_us=round(ms64 * 1e3, 2),
             spmv64_gbs=round(b64 / (ms64 * 1e-3) / 1e9, 1),
             spmv64_sym_us=None if ms65 is None else round(ms65 * 1e3, 2),
@@ -327,8 +630,9 @@ def main():
             fp64_layout="symmetric" if sym else "full")
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU baseline: ~{args.cpu_seconds}s of iterations, {args.cpu_threads} threads")
-        cpu = cpu_baseline(f, p, args.cpu_seconds, args.cpu_threads)
+        threads = args.cpu_threads or default_cpu_threads()
+        log(f"[bench] CPU baseline: ~{args.cpu_seconds}s of iterations, {threads} threads")
+        cpu = cpu_baseline(f, p, args.cpu_seconds, threads)
         log(f"[bench] CPU: {cpu['value']:.3e} DOF*it/s ({cpu['kind']}, {cpu['seconds']:.1f}s)")
         cpu["time_to_1e8_fp64_extrapolated_s"] = round(iters[-1] * cpu["seconds"] / cpu["iters"], 1)
         if cpu["one_thread"]:

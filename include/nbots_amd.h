@@ -289,7 +289,11 @@ typedef struct {
 	double cheb_ratio;       /* interval [lambda_max/ratio, lambda_max] (3e4) */
 	double inner_rtol;       /* CG_RESTART inner tolerance (1e-4)         */
 	double reliable_delta;   /* CG_RELIABLE update threshold (0.1)        */
-	int32_t precond_bits;    /* CHEB_FCG matrix values: 16 (default) or 32 */
+	int32_t precond_bits;    /* CHEB_FCG preconditioner: 16 (default) or 32
+				    scaled matrix values with FP64 row sums
+				    and FP32 vectors; 64: the FP64 matrix,
+				    FP64 vectors (the same algorithm with no
+				    reduced precision, for comparison)      */
 	int32_t true_residual;   /* FP64 and MIXED (Chebyshev): 1 stops on the
 				    true residual (restarts, above); 0 on the
 				    recursively updated one, as the reference

@@ -67,7 +67,8 @@ struct DevMatrix {
 //                               v64[5] (the unit diagonal)
 //   flexible CG (CHEB_FCG):     v64[0] r, v64[1] p, v64[2] w; v32[0..3]
 //                               residual, two directions and z of the
-//                               Chebyshev steps
+//                               Chebyshev steps (FP64 preconditioner:
+//                               v64[5..8] instead)
 //   FP32 CG (CG_RESTART/RELIABLE): v32[0..3]; a fallback to FP64 PCG takes
 //                               the FP64 slots above
 //   free for callers: v64[3], v64[4] (the drop-in's b and x, the system's
@@ -112,7 +113,8 @@ struct MixedOpts {
     double delta = 0.1;        // reliable update when r.r < delta^2 max r.r
     uint32_t cheb_degree = 40;  // SpMVs per preconditioner application
     double cheb_ratio = 3e4;    // Chebyshev interval [lambda_max/ratio, lambda_max]
-    int precond_bits = 16;     // Chebyshev preconditioner matrix: fp16 or FP32 values
+    int precond_bits = 16;     // Chebyshev preconditioner: fp16 or FP32 scaled values (FP64
+                               // sums, FP32 vectors), or 64: the FP64 matrix, FP64 vectors
     bool true_residual = false; // stop on the true residual (restarts), FP64 and CHEB_FCG
 };
 

@@ -728,10 +728,12 @@ int mat_make_scaled(DevMatrix& m, hipStream_t s, int bits) {
         set_last_error("mixed precision needs a 2-dof node matrix");
         return NBX_ERR_ARG;
     }
-    if (bits != 16 && bits != 32) {
-        set_last_error("scaled copy of %d-bit values (16 or 32)", bits);
+    if (bits != 16 && bits != 32 && bits != 64) {
+        set_last_error("scaled copy of %d-bit values (16, 32 or 64)", bits);
         return NBX_ERR_ARG;
     }
+    // 64: the FP64 preconditioner reads the FP64 matrix itself and needs
+    // only the scaling and its bound
     NBX_TRY(ensure_scale(m, s));
     if (bits == 32 && !m.has_val32) {
         if (!m.s.val32) NBX_HIP(hipMalloc(&m.s.val32, (4 * m.s.n_blocks + 4) * sizeof(float)));

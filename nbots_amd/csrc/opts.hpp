@@ -21,7 +21,7 @@ inline MixedOpts mixed_opts(double inner_rtol) {
 }
 
 // Fields at 0 keep the defaults in `base`.  NBX_ERR_ARG for an unknown
-// variant or a precond_bits other than 0, 16 or 32.
+// variant or a precond_bits other than 0, 16, 32 or 64.
 inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& base = MixedOpts()) {
     *o = base;
     if (!p) return NBX_OK;
@@ -29,8 +29,9 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& 
         set_last_error("unknown mixed-precision variant %d", (int)p->mixed_variant);
         return NBX_ERR_ARG;
     }
-    if (p->precond_bits != 0 && p->precond_bits != 16 && p->precond_bits != 32) {
-        set_last_error("precond_bits %d (0, 16 or 32)", (int)p->precond_bits);
+    if (p->precond_bits != 0 && p->precond_bits != 16 && p->precond_bits != 32 &&
+        p->precond_bits != 64) {
+        set_last_error("precond_bits %d (0, 16, 32 or 64)", (int)p->precond_bits);
         return NBX_ERR_ARG;
     }
     o->variant = (MixedVariant)p->mixed_variant;

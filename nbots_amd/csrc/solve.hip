@@ -22,6 +22,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 
 #include "rows.hpp"
@@ -404,18 +405,21 @@ __global__ __launch_bounds__(BLOCK) void k_mix_update(double2* __restrict__ u,
 // weakens the preconditioner and never the attainable residual.
 
 // first: r = b - K x; otherwise x += alpha p, r -= alpha w.  Then the
-// preconditioner's start: s = fl32(dis r), res = s, d = z = s / theta.
+// preconditioner's start, P = float2 (scaled, FP32): s = fl32(dis r),
+// res = s, d = z = s / theta; P = double2 (unscaled, FP64): res = r,
+// d = z = D^-1 r / theta.
+template <class P>
 __global__ __launch_bounds__(BLOCK) void k_fcg_update(Sell2 K, const double2* __restrict__ b,
                                                       double2* __restrict__ x,
                                                       double2* __restrict__ r,
                                                       const double2* __restrict__ p,
                                                       const double2* __restrict__ w,
                                                       const double2* __restrict__ dis,
-                                                      float2* __restrict__ res,
-                                                      float2* __restrict__ d,
-                                                      float2* __restrict__ z, PcgState* st,
+                                                      P* __restrict__ res,
+                                                      P* __restrict__ d,
+                                                      P* __restrict__ z, PcgState* st,
                                                       double* partials, uint32_t n, int first,
-                                                      float inv_theta) {
+                                                      double inv_theta) {
     if (st->done) return;
     const double alpha = st->alpha;
     double acc[1] = {0.0};
@@ -433,11 +437,20 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_update(Sell2 K, const double2* __
         }
         r[i] = rv;
         const double2 dd = dis[i];
-        const float2 sv = make_float2((float)(dd.x * rv.x), (float)(dd.y * rv.y));
-        res[i] = sv;
-        const float2 dv = make_float2(sv.x * inv_theta, sv.y * inv_theta);
-        d[i] = dv;
-        z[i] = dv;
+        if constexpr (std::is_same<P, float2>::value) {
+            const float2 sv = make_float2((float)(dd.x * rv.x), (float)(dd.y * rv.y));
+            const float it = (float)inv_theta;
+            res[i] = sv;
+            const float2 dv = make_float2(sv.x * it, sv.y * it);
+            d[i] = dv;
+            z[i] = dv;
+        } else {
+            res[i] = rv;
+            const double2 dv = make_double2(dd.x * dd.x * rv.x * inv_theta,
+                                            dd.y * dd.y * rv.y * inv_theta);
+            d[i] = dv;
+            z[i] = dv;
+        }
         acc[0] += rv.x * rv.x + rv.y * rv.y;
     }
     __shared__ double red[1][BLOCK / WAVE];
@@ -549,8 +562,70 @@ static void launch_cheb_step64(uint32_t k, uint32_t last, unsigned G, hipStream_
                            res, z, st, n, c1, c2);
 }
 
-// z64 = dis z; partials z64.w (Polak-Ribiere numerator) and r.z64
-__global__ __launch_bounds__(BLOCK) void k_fcg_dots(const float2* __restrict__ z,
+// One step of the all-FP64 preconditioner (precond_bits = 64): the same
+// Chebyshev recurrence on the FP64 matrix itself (the symmetric copy the
+// FP64 products read, or the full one), vectors and arithmetic in FP64, in
+// unscaled variables: res -= K d; d' = c1 d + c2 D^-1 res; z += d'.  With
+// d = D^-1/2 d~ and res = D^1/2 res~ this is the scaled step on
+// D^-1/2 K D^-1/2 exactly; z then holds D^-1/2 z~, the preconditioned
+// residual.  ZM as in k_cheb_step64.
+template <class Op, int ZM>
+__global__ __launch_bounds__(BLOCK) void k_cheb_step_f64(Op op, const double2* __restrict__ din,
+                                                         double2* __restrict__ dout,
+                                                         double2* __restrict__ res,
+                                                         double2* __restrict__ z,
+                                                         const double2* __restrict__ dis,
+                                                         PcgState* st, uint32_t n, double c1,
+                                                         double c2) {
+    if (st->done) return;
+    const Chunk c = my_chunk(n);
+    for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
+        const double2 t = op.row((uint32_t)i, din);
+        const double2 r0 = res[i], di = din[i], dd = dis[i];
+        const double rx = r0.x - t.x, ry = r0.y - t.y;
+        const double dx = c1 * di.x + c2 * (dd.x * dd.x * rx);
+        const double dy = c1 * di.y + c2 * (dd.y * dd.y * ry);
+        res[i] = make_double2(rx, ry);
+        dout[i] = make_double2(dx, dy);
+        if constexpr (ZM != Z_SKIP) {
+            const double2 zv = z[i];
+            const double ax = ZM == Z_PAIR ? di.x + dx : dx;
+            const double ay = ZM == Z_PAIR ? di.y + dy : dy;
+            z[i] = make_double2(zv.x + ax, zv.y + ay);
+        }
+    }
+}
+
+template <class Op>
+static void launch_cheb_step_f64(uint32_t k, uint32_t last, unsigned G, hipStream_t s, const Op& op,
+                                 const double2* din, double2* dout, double2* res, double2* z,
+                                 const double2* dis, PcgState* st, uint32_t n, double c1,
+                                 double c2) {
+    if (k % 2 == 0)
+        hipLaunchKernelGGL((k_cheb_step_f64<Op, Z_PAIR>), dim3(G), dim3(BLOCK), 0, s, op, din, dout,
+                           res, z, dis, st, n, c1, c2);
+    else if (k == last)
+        hipLaunchKernelGGL((k_cheb_step_f64<Op, Z_ONE>), dim3(G), dim3(BLOCK), 0, s, op, din, dout,
+                           res, z, dis, st, n, c1, c2);
+    else
+        hipLaunchKernelGGL((k_cheb_step_f64<Op, Z_SKIP>), dim3(G), dim3(BLOCK), 0, s, op, din, dout,
+                           res, z, dis, st, n, c1, c2);
+}
+
+// the preconditioned residual z64: dis z (scaled FP32 z) or z itself
+// (unscaled FP64 z)
+__device__ __forceinline__ double2 z_unscaled(const float2* z, const double2* dis, size_t i) {
+    const double2 dd = dis[i];
+    const float2 zv = z[i];
+    return make_double2(dd.x * (double)zv.x, dd.y * (double)zv.y);
+}
+__device__ __forceinline__ double2 z_unscaled(const double2* z, const double2*, size_t i) {
+    return z[i];
+}
+
+// partials z64.w (Polak-Ribiere numerator) and r.z64
+template <class P>
+__global__ __launch_bounds__(BLOCK) void k_fcg_dots(const P* __restrict__ z,
                                                     const double2* __restrict__ dis,
                                                     const double2* __restrict__ w,
                                                     const double2* __restrict__ r, PcgState* st,
@@ -559,9 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_dots(const float2* __restrict__ z
     double acc[2] = {0.0, 0.0};
     const Chunk c = my_chunk(n);
     for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
-        const double2 dd = dis[i];
-        const float2 zv = z[i];
-        const double2 zz = make_double2(dd.x * (double)zv.x, dd.y * (double)zv.y);
+        const double2 zz = z_unscaled(z, dis, i);
         if (!first) acc[0] += dotu(zz, w[i]);
         acc[1] += dotu(r[i], zz);
     }
@@ -592,7 +665,8 @@ __global__ __launch_bounds__(FIN) void k_fin_fcg_dots(PcgState* st, const double
 }
 
 // p = z64 + beta p  (p = z64 on the first step)
-__global__ __launch_bounds__(BLOCK) void k_fcg_p(const float2* __restrict__ z,
+template <class P>
+__global__ __launch_bounds__(BLOCK) void k_fcg_p(const P* __restrict__ z,
                                                  const double2* __restrict__ dis,
                                                  double2* __restrict__ p, PcgState* st,
                                                  uint32_t n, int first) {
@@ -600,9 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_fcg_p(const float2* __restrict__ z,
     const double beta = st->beta;
     const Chunk c = my_chunk(n);
     for (size_t i = c.beg + threadIdx.x; i < c.end; i += BLOCK) {
-        const double2 dd = dis[i];
-        const float2 zv = z[i];
-        const double2 zz = make_double2(dd.x * (double)zv.x, dd.y * (double)zv.y);
+        const double2 zz = z_unscaled(z, dis, i);
         p[i] = first ? zz : axpyu(zz, beta, p[i]);
     }
 }
@@ -968,7 +1040,7 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         c2[k] = (float)c2d[k];
         rho = rn;
     }
-    const float inv_theta = (float)(1.0 / theta);
+    const double inv_theta = 1.0 / theta;
     double2* r = (double2*)w.v64[0];
     double2* p = (double2*)w.v64[1];
     double2* wv = (double2*)w.v64[2];
@@ -976,6 +1048,12 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     float2* da = (float2*)w.v32[1];
     float2* db = (float2*)w.v32[2];
     float2* z = (float2*)w.v32[3];
+    // the all-FP64 preconditioner's vectors (free during a flexible CG)
+    double2* res64 = (double2*)w.v64[5];
+    double2* da64 = (double2*)w.v64[6];
+    double2* db64 = (double2*)w.v64[7];
+    double2* z64 = (double2*)w.v64[8];
+    const bool p64 = opt.precond_bits == 64;
     const double2* dis = (const double2*)m.dis;
     const double2* bv = (const double2*)b;
     double2* xv = (double2*)x;
@@ -985,20 +1063,33 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     // The fp16 preconditioner's working set (matrix + four FP32 vectors) is
     // read by each of the degree - 1 steps in a row: when it fits in the
     // Infinity Cache, cached loads keep it there between steps (at 4.2M DOF,
-    // 243 MB: 1.48 -> 1.32 s per solve); otherwise the matrix streams.
+    // 243 MB: 1.48 -> 1.32 s per solve); otherwise the matrix streams.  The
+    // FP64 preconditioner reads the FP64 matrix the same way, cached when
+    // its working set (matrix + five FP64 vectors) fits.
     const bool off16 = opt.precond_bits == 16 && m.has_off16;
     const bool packed = off16 && m.off_packed;
-    const double cheb_mat = packed  ? 8.0 * (double)m.o.n_blocks + 4.0 * (double)m.o.n_pairs +
-                                          4.0 * (double)nu
+    const double sym_mat = 24.0 * (double)m.y.n_nodes + 36.0 * (double)m.y.n_upper +
+                           8.0 * (double)m.y.n_lower;
+    const double cheb_mat = p64     ? (m.has_sym ? sym_mat : 36.0 * (double)m.s.n_blocks)
+                            : packed ? 8.0 * (double)m.o.n_blocks + 4.0 * (double)m.o.n_pairs +
+                                           4.0 * (double)nu
                             : off16 ? 12.0 * (double)m.o.n_blocks + 4.0 * (double)nu
                                     : 12.0 * (double)m.s.n_blocks;
-    const bool cheb_cached = cheb_mat + 33.0 * (double)nu <= (double)INFINITY_CACHE_BYTES;
+    const bool cheb_cached =
+        cheb_mat + (p64 ? 80.0 : 33.0) * (double)nu <= (double)INFINITY_CACHE_BYTES;
     auto precondition = [&](int first) {
-        hipLaunchKernelGGL(k_fcg_update, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv, dis, res,
-                           da, z, w.st, w.partials, nu, first, inv_theta);
+        if (p64) {
+            hipLaunchKernelGGL(k_fcg_update<double2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p,
+                               wv, dis, res64, da64, z64, w.st, w.partials, nu, first, inv_theta);
+        } else {
+            hipLaunchKernelGGL(k_fcg_update<float2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p,
+                               wv, dis, res, da, z, w.st, w.partials, nu, first, inv_theta);
+        }
         hipLaunchKernelGGL(k_fin_fcg_update, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
         float2* din = da;
         float2* dout = db;
+        double2* din64 = da64;
+        double2* dout64 = db64;
         for (uint32_t k = 1; k < deg; ++k) {
             auto step = [&](const auto& K) {
                 if (cheb_cached)
@@ -1008,7 +1099,16 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
                     launch_cheb_step64<true>(k, deg - 1, G, s, K, din, dout, res, z, w.st, nu,
                                              c1d[k], c2d[k]);
             };
-            if (packed)
+            if (p64 && m.has_sym && cheb_cached)
+                launch_cheb_step_f64(k, deg - 1, G, s, OpSymC{m.y}, din64, dout64, res64, z64, dis,
+                                     w.st, nu, c1d[k], c2d[k]);
+            else if (p64 && m.has_sym)
+                launch_cheb_step_f64(k, deg - 1, G, s, ops, din64, dout64, res64, z64, dis, w.st,
+                                     nu, c1d[k], c2d[k]);
+            else if (p64)
+                launch_cheb_step_f64(k, deg - 1, G, s, op, din64, dout64, res64, z64, dis, w.st,
+                                     nu, c1d[k], c2d[k]);
+            else if (packed)
                 step(Off16P{m.o});
             else if (off16)
                 step(m.o);
@@ -1017,14 +1117,23 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
             else
                 hipLaunchKernelGGL(k_cheb_step<32>, dim3(G), dim3(BLOCK), 0, s, m.s, din, dout, res, z,
                                    w.st, nu, c1[k], c2[k]);
-            float2* t = din;
-            din = dout;
-            dout = t;
+            std::swap(din, dout);
+            std::swap(din64, dout64);
         }
-        hipLaunchKernelGGL(k_fcg_dots, dim3(G), dim3(BLOCK), 0, s, z, dis, wv, r, w.st, w.partials, nu,
-                           first);
+        if (p64) {
+            hipLaunchKernelGGL(k_fcg_dots<double2>, dim3(G), dim3(BLOCK), 0, s, z64, dis, wv, r,
+                               w.st, w.partials, nu, first);
+        } else {
+            hipLaunchKernelGGL(k_fcg_dots<float2>, dim3(G), dim3(BLOCK), 0, s, z, dis, wv, r, w.st,
+                               w.partials, nu, first);
+        }
         hipLaunchKernelGGL(k_fin_fcg_dots, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
-        hipLaunchKernelGGL(k_fcg_p, dim3(G), dim3(BLOCK), 0, s, z, dis, p, w.st, nu, first);
+        if (p64)
+            hipLaunchKernelGGL(k_fcg_p<double2>, dim3(G), dim3(BLOCK), 0, s, z64, dis, p, w.st, nu,
+                               first);
+        else
+            hipLaunchKernelGGL(k_fcg_p<float2>, dim3(G), dim3(BLOCK), 0, s, z, dis, p, w.st, nu,
+                               first);
     };
     NBX_HIP(hipEventRecord(w.ev[0], s));
     precondition(1);
@@ -1063,7 +1172,7 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         NBX_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
         stats->ms = ms;
         stats->iters = w.h_st->k;
-        stats->outer = w.h_st->k * (deg - 1);    // FP32 SpMVs in the preconditioner
+        stats->outer = w.h_st->k * (deg - 1);    // SpMVs in the preconditioner
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
@@ -1110,8 +1219,11 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
 
 int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, const MixedOpts& opt,
                 uint32_t max_iter, Workspace& w, hipStream_t s, SolveStats* stats) {
-    const bool fp16 = opt.variant == CHEB_FCG && opt.precond_bits == 16;
-    if (!m.blocked || !m.has_scale || !(fp16 ? m.has_off16 || m.has_val16 : m.has_val32)) {
+    const bool cheb = opt.variant == CHEB_FCG;
+    const bool fp16 = cheb && opt.precond_bits == 16;
+    const bool fp64 = cheb && opt.precond_bits == 64;
+    if (!m.blocked || !m.has_scale ||
+        !(fp64 || (fp16 ? m.has_off16 || m.has_val16 : m.has_val32))) {
         set_last_error("solve_mixed: matrix has no scaled %d-bit copy", fp16 ? 16 : 32);
         return NBX_ERR_ARG;
     }

@@ -443,6 +443,28 @@ def ref_assemble(p, apply_bc=True, enabled=None, self_weight=False, density=0.0,
     return rowptr, col, val, F, (1 if st == 0xFFFFFFFF else 0)
 
 
+def ref_assemble_timed(p):
+    """ref_assemble with BCs, timed: (rowptr, col, val, F, status,
+    seconds of the pattern call, seconds of the pattern + assembly + BCs
+    call)."""
+    import time
+    R = ref()
+    rowptr = np.zeros(p.N + 1, np.uint32)
+    args = _mesh_args(p) + _bc_args(p)
+    tail = [p.E, p.nu, None if p.elem_E is None else _c(p.elem_E, np.float64), p.thickness,
+            None, 0, 0.0, np.zeros(2), 1]
+    t0 = time.perf_counter()
+    nnz = R.ref_assemble(*args, *tail, rowptr, None, None, None)
+    t_pat = time.perf_counter() - t0
+    col = np.zeros(nnz, np.uint32)
+    val = np.zeros(nnz)
+    F = np.zeros(p.N)
+    t0 = time.perf_counter()
+    st = R.ref_assemble(*args, *tail, rowptr, col, val, F)
+    t_all = time.perf_counter() - t0
+    return rowptr, col, val, F, (1 if st == 0xFFFFFFFF else 0), t_pat, t_all
+
+
 def ref_fem_compute(p, enabled=None):
     disp = np.zeros(p.N)
     eps = np.zeros(3 * p.N_elem)

@@ -25,7 +25,8 @@ def test_bench_json_line(monkeypatch):
     sys.path.insert(0, ROOT)
     import bench
     monkeypatch.setattr(sys, "argv", ["bench.py", "--config", "c4", "--steps", "1", "--warmup",
-                                      "0", "--cpu-seconds", "1", "--simp-iters", "2"])
+                                      "0", "--cpu-seconds", "1", "--simp-iters", "2",
+                                      "--simp-fp32ir-iters", "1"])
     out = io.StringIO()
     with redirect_stdout(out):
         bench.main()
@@ -46,3 +47,15 @@ def test_bench_json_line(monkeypatch):
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("reference", "port")
     assert d["mixed_true_rel_res"] <= 1e-8
+    assert rl["byte_model"] and rl["csr_bytes_per_launch"] > rl["bytes_per_launch"]
+    assert cb["host"]["nproc"] >= 1 and cb["host"]["affinity"] >= 1
+    # the mixed-precision speedup split into the preconditioner's gain and
+    # the precision's, and the FP32-CG + FP64-refinement scheme beside them
+    dec = d["decomposition"]
+    assert abs(dec["preconditioner_factor"] * dec["precision_factor"]
+               - d["mixed_speedup_vs_fp64"]) <= 0.01 * d["mixed_speedup_vs_fp64"]
+    assert dec["cheb_fcg_fp64_precond"]["true_rel_res"] <= 1e-8
+    assert d["fp32ir"]["seconds"] == d["time_to_1e8_fp32ir_s"] > 0
+    assert set(d["simp_decomposition"]["ms_per_iter"]) == {
+        "jacobi_pcg_fp64", "cheb_fcg_fp64_precond", "cheb_fcg_fp16_precond"}
+    assert d["simp_fp32ir"]["iterations"] == 1
