@@ -29,6 +29,7 @@ topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF,
                     [--simp-iters 50] [--no-simp] [--no-decompose]
     python bench.py --cpu-full        # the CPU baseline plan at full size
     python bench.py --pmc-probe [--probe-mixed]   # a short run for rocprofv3 --pmc
+    python bench.py --simp-only [--simp-iters N]  # the SIMP loop alone
 """
 from __future__ import annotations
 
@@ -521,6 +522,8 @@ def main():
     ap.add_argument("--simp-iters", type=int, default=50)
     ap.add_argument("--simp-fp32ir-iters", type=int, default=10)
     ap.add_argument("--no-simp", action="store_true")
+    ap.add_argument("--simp-only", action="store_true",
+                    help="only the SIMP loop, its results as one JSON line (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only (asked for --gpus {args.gpus}); "
@@ -531,6 +534,12 @@ def main():
         return
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.simp_only:
+        from nbots_amd.fem import MIXED_CHEB_FCG, solve_opts
+        print(json.dumps(simp_bench(args, solve_opts(
+            MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+            precond_bits=args.precond_bits, simp=True, true_residual=True))), flush=True)
         return
 
     from nbots_amd import synth

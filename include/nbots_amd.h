@@ -113,6 +113,19 @@ int nb_sparse_solve_conjugate_gradient(const nb_sparse_t *const A,
 				       double *tolerance_reached,
 				       uint32_t omp_parallel_threads);
 
+/* headers/nb/solver_bot/sparse/solvers/gauss_seidel.h:8-15
+ * Gauss-Seidel on the GPU: the reference's sweeps (the error ||b - A x||
+ * and the stop test each sweep, then (D + L) x = b - U x), its forward
+ * substitution run by dependency levels, each row in the reference's
+ * order.  Same outputs and return codes (-1: GPU failure).              */
+int nb_sparse_solve_Gauss_Seidel(const nb_sparse_t *const A,
+				 const double *const b,
+				 double *_x,
+				 uint32_t max_iter, double tolerance,
+				 uint32_t *niter_performed,
+				 double *tolerance_reached,
+				 uint32_t omp_parallel_threads);
+
 /* ======================================================================== */
 /* FEM objects used by the driver                                          */
 /* ======================================================================== */
@@ -354,6 +367,7 @@ int nbx_sparse_export_csr(const nb_sparse_t *A, uint32_t *rowptr,
 int nbx_sparse_import_values(nb_sparse_t *A, const double *val);
 /* 1 when the device copy uses the 2x2-block format. */
 int nbx_sparse_is_blocked(const nb_sparse_t *A);
+
 
 /* ---- device-resident T3 plane-stress system ----------------------------- */
 typedef struct nbx_fem_s nbx_fem_t;

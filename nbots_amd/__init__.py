@@ -138,6 +138,17 @@ class Sparse:
             raise RuntimeError(f"GPU solve failed: {last_error()}")
         return x, st, it.value, tr.value
 
+    def solve_gauss_seidel(self, b, x0=None, max_iter=None, tol=1e-8):
+        """nb_sparse_solve_Gauss_Seidel: returns x, status, sweeps, error."""
+        x = np.zeros(self.N) if x0 is None else np.array(x0, np.float64)
+        it, tr = C.c_uint32(0), C.c_double(0)
+        st = self.L.nb_sparse_solve_Gauss_Seidel(
+            self.h, np.ascontiguousarray(b, np.float64), x,
+            60 * self.N if max_iter is None else max_iter, tol, C.byref(it), C.byref(tr), 1)
+        if st < 0:
+            raise RuntimeError(f"GPU solve failed: {last_error()}")
+        return x, st, it.value, tr.value
+
     def solve(self, b, mode=SOLVE_FP64, tol=1e-8, max_iter=None, x0=None, inner_rtol=0.0,
               opts=None):
         """nbx_sparse_solve, or nbx_sparse_solve_ex when opts (a SolveOpts,

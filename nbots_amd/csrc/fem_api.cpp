@@ -16,6 +16,7 @@
 #include "../../include/nbots_amd.h"
 #include "bcplan.hpp"
 #include "fem.hpp"
+#include "gauss_seidel.hpp"
 #include "opts.hpp"
 #include "simp.hpp"
 
@@ -470,9 +471,10 @@ FAIL:
 // or N_max_iter or N_max_iter_without_enhance iterations without a new best
 // residual, else solves K du = F - K u with Jacobi-PCG (absolute tol 1e-8,
 // N iterations, from the previous du; du = 0 at the start of a step),
-// u += du, and updates strain and damage.  The log has the reference's lines.
-// The direct-solver option runs the same PCG (its factorisations are not on
-// this path).
+// u += du, and updates strain and damage; when the PCG does not converge,
+// Gauss-Seidel continues from its du (gauss_seidel.hip).  The log has the
+// reference's lines.  The direct-solver option runs the same PCG (its
+// factorisations are not on this path).
 int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
                          const nb_material_t* material, const nb_bcond_t* bcond,
                          int enable_self_weight, const double* gravity,
@@ -503,6 +505,9 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
     int rc = nbx_fem_create(&f, nn, mesh.nod.data(), ne, mesh.conn.data());
     double *dmg = nullptr, *r = nullptr, *r_prev = nullptr, *strain = nullptr, *res = nullptr,
            *du = nullptr;
+    nbx::GsSystem gs;                           // the Gauss-Seidel fallback's copy of K
+    std::vector<uint32_t> csr_rp, csr_col;
+    std::vector<double> csr_val;
     const double E = nb_material_get_elasticity_module(material);
     const double nu = nb_material_get_poisson_module(material);
     const double Gf = nb_material_get_fracture_energy(material);
@@ -572,9 +577,33 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
                     rc = -st;
                     goto END;
                 }
-                if (st != 0)   // the reference then tries Gauss-Seidel (:417-427)
-                    logf("    [Conjugate Gradient (p. Jacobi) did not converge: %u iterations, "
-                         "error %e ]\n", iters, err);
+                if (st != 0) {
+                    // :414-427: Gauss-Seidel from the CG's du, 60 N sweeps,
+                    // tolerance 1e-8; its failure is logged
+                    if (csr_rp.empty()) {
+                        csr_rp.resize(N + 1);
+                        csr_col.resize(S.nnz);
+                        csr_val.resize(S.nnz);
+                    }
+                    if ((rc = nbx::fem_download_csr(S, csr_rp.data(), csr_col.data(),
+                                                    csr_val.data())))
+                        goto END;
+                    if ((rc = nbx::gs_prepare(gs, N, csr_rp.data(), csr_col.data(),
+                                              csr_val.data(), S.s)))
+                        goto END;
+                    uint32_t gi = 0;
+                    double ge = 0.0;
+                    const int gst = nbx::gauss_seidel(gs, res, du, 60 * N, 1e-8, &gi, &ge, S.s);
+                    if (gst > 1) {
+                        rc = gst;
+                        goto END;
+                    }
+                    if (gst == 1) {
+                        logf("    [Conjugate Gradient (p. Jacobi) and Gauss-Seidel fails.]\n");
+                        logf("      [Gauss-Seidel iterations: %u]\n", gi);
+                        logf("      [Gauss-Seidel error: %e ]\n", ge);
+                    }
+                }
                 logf("    [%i: Residual: %e]\n", it, nrm);
                 if ((rc = nbx::fem_add(S, S.u, du))) goto END;
                 if ((rc = nbx::fem_damage_update(S, E, nu, Gf, ft, r_prev, r, dmg, strain)))
@@ -589,6 +618,7 @@ int nbx_fem_damage_solve(const nb_mesh2D_t* part, const nb_fem_elem_t* elemtype,
             rc = nbx::NBX_ERR_HIP;
     }
 END:
+    nbx::gs_free(gs);
     if (log) fclose(log);
     void* ps[] = {dmg, r, r_prev, strain, res, du};
     for (void* q : ps)

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/nbots_amd.h"
+#include "gauss_seidel.hpp"
 #include "internal.hpp"
 #include "opts.hpp"
 
@@ -48,6 +49,7 @@ struct Mirror {
     bool built = false;
     std::vector<uint32_t> rowptr, col;   // pattern of the device copy
     std::vector<double> val;             // values of the device copy
+    nbx::GsSystem gs;                    // scalar CSR copy for Gauss-Seidel
 };
 
 std::mutex g_mu;
@@ -64,6 +66,7 @@ void drop_mirror(const nb_sparse_t* A) {
     }
     nbx::mat_free(m->dev);
     nbx::ws_free(m->ws);
+    nbx::gs_free(m->gs);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -562,6 +565,47 @@ int nb_sparse_solve_conjugate_gradient(const nb_sparse_t* const A, const double*
     if (niter_performed) *niter_performed = info.iters;
     if (tolerance_reached) *tolerance_reached = info.tol_reached;
     return rc;
+}
+
+// solvers/gauss_seidel.c:13-63 (declared in solvers/gauss_seidel.h:8-15) on
+// the device: the same sweeps, the forward substitution by dependency
+// levels with each row summed in the reference's order (gauss_seidel.hip).
+int nb_sparse_solve_Gauss_Seidel(const nb_sparse_t* const A, const double* const b, double* _x,
+                                 uint32_t max_iter, double tolerance, uint32_t* niter_performed,
+                                 double* tolerance_reached, uint32_t) {
+    const uint32_t N = A->N;
+    int rc = 0;
+    Mirror* m = N ? sync_device(A, &rc) : nullptr;
+    uint32_t it = 0;
+    double err = 0.0;
+    int status = 0;
+    if (!rc && N) rc = nbx::gs_prepare(m->gs, N, m->rowptr.data(), m->col.data(), m->val.data(),
+                                       m->stream);
+    if (!rc && N) {
+        double* db = m->ws.v64[3];
+        double* dx = m->ws.v64[4];
+        if (hipMemcpyAsync(db, b, (size_t)N * 8, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+            hipMemcpyAsync(dx, _x, (size_t)N * 8, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+            rc = nbx::NBX_ERR_HIP;
+        if (!rc) {
+            status = nbx::gauss_seidel(m->gs, db, dx, max_iter, tolerance, &it, &err, m->stream);
+            if (status > 1) rc = status;
+        }
+        if (!rc && nbx::copy_on(_x, dx, (size_t)N * 8, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
+            rc = nbx::NBX_ERR_HIP;
+    } else if (!rc) {
+        // no rows: the first sweep's error is the empty sum 0, which meets
+        // any tolerance; without a sweep it stays 1e10 (gauss_seidel.c:27-41)
+        err = max_iter ? 0.0 : 1e5;
+        status = max_iter ? 0 : (1e10 > tolerance * tolerance ? 1 : 0);
+    }
+    if (rc) {
+        fprintf(stderr, "nb_sparse_solve_Gauss_Seidel: GPU failure: %s\n", nbx::last_error());
+        return -1;
+    }
+    if (niter_performed) *niter_performed = it;
+    if (tolerance_reached) *tolerance_reached = err;
+    return status;
 }
 
 const char* nbx_last_error(void) { return nbx::last_error(); }

@@ -106,6 +106,8 @@ SIGS = {
                                                 C.POINTER(F64), U32]),
     "nb_sparse_solve_conjugate_gradient": (I32, [VP, f64p, f64p, U32, F64, C.POINTER(U32),
                                                  C.POINTER(F64), U32]),
+    "nb_sparse_solve_Gauss_Seidel": (I32, [VP, f64p, f64p, U32, F64, C.POINTER(U32),
+                                           C.POINTER(F64), U32]),
     "nb_mesh2D_create": (VP, [I32]),
     "nb_mesh2D_destroy": (None, [VP]),
     "nb_mesh2D_get_N_invtx": (U32, [VP]),

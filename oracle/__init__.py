@@ -80,6 +80,9 @@ def lib():
         L.orc_pcg_jacobi.restype = I32
         L.orc_pcg_jacobi.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
                                      C.POINTER(U32), C.POINTER(F64)]
+        L.orc_gauss_seidel.restype = I32
+        L.orc_gauss_seidel.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64,
+                                       C.POINTER(U32), C.POINTER(F64)]
         L.orc_assemble_t3_damage.restype = I32
         L.orc_assemble_t3_damage.argtypes = [U32, f64p, U32, u32p, F64, F64, f64p, F64, I32, F64,
                                              f64o, u32p, u32p, f64o, f64o]
@@ -140,6 +143,9 @@ def ref():
         R.ref_sparse_save.argtypes = [U32, u32p, u32p, f64p, C.c_char_p]
         R.ref_sparse_save_mat4.argtypes = [U32, u32p, u32p, f64p, C.c_char_p, C.c_char_p]
         R.ref_mat4_save_vec.argtypes = [C.c_char_p, C.c_char_p, f64p, U32]
+        R.ref_gauss_seidel.restype = I32
+        R.ref_gauss_seidel.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64, U32,
+                                       C.POINTER(U32), C.POINTER(F64)]
         R.ref_cg.restype = I32
         R.ref_cg.argtypes = [U32, u32p, u32p, f64p, f64p, f64p, U32, F64, U32,
                              C.POINTER(U32), C.POINTER(F64)]
@@ -246,13 +252,26 @@ def cg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8):
     return x, it.value, tr.value, st
 
 
+def gauss_seidel(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8):
+    """Gauss-Seidel of gauss_seidel.c:13-63: returns x, sweeps, sqrt(error),
+    status."""
+    N = rowptr.size - 1
+    x = np.zeros(N) if x0 is None else np.array(x0, np.float64)
+    it, tr = U32(0), F64(0)
+    st = lib().orc_gauss_seidel(N, rowptr, col, val, _c(b, np.float64), x,
+                                60 * N if max_iter is None else max_iter, tol,
+                                C.byref(it), C.byref(tr))
+    return x, it.value, tr.value, st
+
+
 def damage_solve(p, Gf, ft, n_steps, max_iter, max_iter_wo_enhance, res_tol, logfile=None):
     """nb_fem_compute_2D_Non_Linear_Solid_Mechanics (static_damage2D.c:239-472)
     on the restatement, with the conjugate-gradient path
     (enable_Cholesky_solver false), plane stress, no self weight: load
     factor (n+1)/n_steps on the boundary conditions, residual F - K u of the
-    damaged K, Jacobi-PCG (tol 1e-8, max_iter N) for du from the previous du,
-    u += du, strain and damage update.  Writes the reference's log lines
+    damaged K, Jacobi-PCG (tol 1e-8, max_iter N) for du from the previous du
+    (Gauss-Seidel from there when the PCG does not converge), u += du,
+    strain and damage update.  Writes the reference's log lines
     (to logfile if given) and returns u, damage, r and the log text."""
     L = lib()
     N = p.N
@@ -293,8 +312,14 @@ def damage_solve(p, Gf, ft, n_steps, max_iter, max_iter_wo_enhance, res_tol, log
                 break
             du, _, _, st = pcg(rp, col, val, res, x0=du, max_iter=N, tol=1e-8)
             if st != 0:
-                # the reference then runs Gauss-Seidel (static_damage2D.c:417-427)
-                raise NotImplementedError("Jacobi-PCG did not converge in a damage iteration")
+                # static_damage2D.c:414-427: Gauss-Seidel from the CG's du,
+                # 60 N sweeps, the same tolerance; logged only if it fails too
+                du, gi, ge, gst = gauss_seidel(rp, col, val, res, x0=du, max_iter=60 * N,
+                                               tol=1e-8)
+                if gst != 0:
+                    lines.append("    [Conjugate Gradient (p. Jacobi) and Gauss-Seidel fails.]")
+                    lines.append("      [Gauss-Seidel iterations: %u]" % gi)
+                    lines.append("      [Gauss-Seidel error: %s ]" % _fmt_e(ge))
             lines.append("    [%i: Residual: %s]" % (it, _fmt_e(nrm)))
             u += du
             L.orc_damage_update(p.N_elem, nod, conn, u, p.E, p.nu, Gf, ft, r_prev, r, dmg,
@@ -489,6 +514,17 @@ def ref_pcg(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8, threads=1):
     st = ref().ref_pcg(N, rowptr, col, val, _c(b, np.float64), x,
                        N if max_iter is None else max_iter, tol, threads,
                        C.byref(it), C.byref(tr))
+    return x, it.value, tr.value, st
+
+
+def ref_gauss_seidel(rowptr, col, val, b, x0=None, max_iter=None, tol=1e-8, threads=1):
+    """The compiled reference's nb_sparse_solve_Gauss_Seidel."""
+    N = rowptr.size - 1
+    x = np.zeros(N) if x0 is None else np.array(x0, np.float64)
+    it, tr = U32(0), F64(0)
+    st = ref().ref_gauss_seidel(N, rowptr, col, val, _c(b, np.float64), x,
+                                60 * N if max_iter is None else max_iter, tol, threads,
+                                C.byref(it), C.byref(tr))
     return x, it.value, tr.value, st
 
 

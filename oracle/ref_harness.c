@@ -419,6 +419,20 @@ int ref_cg(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 	return st;
 }
 
+/* The reference's Gauss-Seidel (solvers/gauss_seidel.c:13-63) on an
+ * imported matrix; threads = 1 makes its error sum deterministic. */
+int ref_gauss_seidel(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		     const double *val, const double *b, double *x,
+		     uint32_t max_iter, double tol, uint32_t threads,
+		     uint32_t *niter, double *tol_reached)
+{
+	nb_sparse_t *A = import_csr(N, rowptr, col, val);
+	int st = nb_sparse_solve_Gauss_Seidel(A, b, x, max_iter, tol, niter,
+					      tol_reached, threads);
+	nb_sparse_destroy(A);
+	return st;
+}
+
 /* The reference's file writers on an imported matrix (sparse.c:97-112,
  * matlab_v4.c:186-245, :493-513). */
 void ref_sparse_save(uint32_t N, const uint32_t *rowptr, const uint32_t *col,

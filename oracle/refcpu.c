@@ -470,6 +470,62 @@ void orc_spmv(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 	}
 }
 
+/* Gauss-Seidel, single thread: solver_bot/sparse/solvers/gauss_seidel.c:13-63
+ * with the forward substitution of triangular.c:14-35.  Each sweep forms
+ * the residual b - A x (its squared norm is the error) and c = b - U x
+ * (U: the entries right of the diagonal, old x) in one pass over the row,
+ * stops when error <= tol^2, else solves (D + L) x = c row by row: x_i = c_i
+ * minus the entries left of the diagonal times the new x, divided by the
+ * diagonal (the row's last entry when that is the diagonal, else the entry
+ * found by search; 0 when absent).  x is the initial guess.  Returns 0 when
+ * the error met tol, 1 after max_iter sweeps. */
+int orc_gauss_seidel(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		     const double *val, const double *b, double *x,
+		     uint32_t max_iter, double tol, uint32_t *niter,
+		     double *tol_reached)
+{
+	double *c = malloc((N ? N : 1) * sizeof(double));
+	uint32_t k = 0;
+	double error = 1e10;
+	while (k < max_iter) {
+		error = 0.0;
+		for (uint32_t i = 0; i < N; i++) {
+			double ci = b[i], ei = b[i];
+			for (uint32_t s = rowptr[i]; s < rowptr[i + 1]; s++) {
+				const double p = val[s] * x[col[s]];
+				ei -= p;
+				if (col[s] > i)
+					ci -= p;
+			}
+			c[i] = ci;
+			error += ei * ei;
+		}
+		if (error <= tol * tol)
+			break;
+		for (uint32_t i = 0; i < N; i++) {
+			double xi = c[i];
+			uint32_t s = rowptr[i];
+			while (s < rowptr[i + 1] && col[s] < i) {
+				xi -= val[s] * x[col[s]];
+				s++;
+			}
+			double d = 0.0;
+			if (rowptr[i + 1] > rowptr[i] && col[rowptr[i + 1] - 1] == i)
+				d = val[rowptr[i + 1] - 1];
+			else if (s < rowptr[i + 1] && col[s] == i)
+				d = val[s];
+			x[i] = xi / d;
+		}
+		k++;
+	}
+	free(c);
+	if (niter)
+		*niter = k;
+	if (tol_reached)
+		*tol_reached = sqrt(error);
+	return error > tol * tol ? 1 : 0;
+}
+
 /* Jacobi-preconditioned CG, single thread: solver_bot/sparse/solvers/
  * cg_precond_jacobi.c:13-90.  Absolute tolerance on ||Ax-b||_2; the stop
  * test uses g.g computed at the start of the previous iteration (:45,:56),

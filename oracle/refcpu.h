@@ -19,6 +19,8 @@
  *   orc_assemble_t3        golden K/F (bit-exact), closed-form CST element,
  *                          rigid-body null space, Ke(E) linear in E
  *   orc_set_bconditions    golden K/F after BCs, dense elimination identity
+ *   orc_gauss_seidel       bit-exact against the compiled reference's
+ *                          nb_sparse_solve_Gauss_Seidel (1 thread)
  *   orc_pcg_jacobi         golden x/niter/tol_reached, hand-traced 3x3 case,
  *                          dense numpy solve
  *   orc_cg                 golden x/niter/tol_reached of the compiled
@@ -31,8 +33,10 @@
  *                          reproduced character for character
  *   orc_elem_energy, orc_simp_*, orc_filter_*, orc_oc_update
  *       no reference code exists (nbots has no topology optimisation);
- *       pinned by energy identity sum_e E_e u_e'K0u_e = F'u, central finite
- *       differences and the published 88-line algorithm's invariants.
+ *       pinned by the energy identity sum_e E_e u_e'K0u_e = F'u, central
+ *       finite differences of the full solve and of C(xPhys(x)), the
+ *       transpose identity <filter_sens(g), v> = <g, filter_apply(v)>, a
+ *       hand-computed OC step and the OC optimality condition.
  */
 #ifndef NB_ORACLE_REFCPU_H
 #define NB_ORACLE_REFCPU_H
@@ -89,6 +93,10 @@ void orc_set_dirichlet(uint32_t N, const uint32_t *rowptr,
 		       uint32_t idx, double value);
 void orc_spmv(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 	      const double *val, const double *x, double *y);
+int orc_gauss_seidel(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
+		     const double *val, const double *b, double *x,
+		     uint32_t max_iter, double tol, uint32_t *niter,
+		     double *tol_reached);
 int orc_pcg_jacobi(uint32_t N, const uint32_t *rowptr, const uint32_t *col,
 		   const double *val, const double *b, double *x,
 		   uint32_t max_iter, double tol,

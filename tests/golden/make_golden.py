@@ -234,6 +234,39 @@ def damage_cases():
     print("damage log:", sum(1 for _ in open(log)), "lines")
 
 
+DAMAGE_CUT_CASE = dict(Gf=1e-3, ft=1e6, n_steps=2, max_iter=4, max_iter_wo_enhance=3,
+                       res_tol=1e-10)
+
+
+def damage_cut_case():
+    """The reference's damage solver on a slender 32x1 beam (132 DOF, no
+    damage: tensile limit 1e6): its Jacobi-PCG stops at N iterations short
+    of the absolute tolerance 1e-8, and the Gauss-Seidel fallback
+    (static_damage2D.c:414-427) runs its 60 N sweeps and fails, logged."""
+    p = synth.damage_beam(32, 1)
+    log = os.path.join(OUT, "damage_cut_32x1_log.txt")
+    if os.path.exists(log):
+        os.remove(log)
+    oracle.ref_damage_compute(p, logfile=log, **DAMAGE_CUT_CASE)
+    print("damage cut log:", sum(1 for _ in open(log)), "lines")
+
+
+def gauss_seidel_cases():
+    """The reference's nb_sparse_solve_Gauss_Seidel (1 thread) on K and F
+    of the 16x8 traction cantilever: 200 sweeps with an unreachable
+    tolerance, and 20000 sweeps towards 1e-6 ||F||, both from x = 0."""
+    p = synth.cantilever_traction(16, 8)
+    rp, col, val, F, st = oracle.ref_assemble(p, apply_bc=True)
+    assert st == 0
+    out = {}
+    for key, mi, tol in (("cut", 200, 1e-30), ("tol", 20000, 1e-6 * float(np.linalg.norm(F)))):
+        x, it, tr, gst = oracle.ref_gauss_seidel(rp, col, val, F, max_iter=mi, tol=tol)
+        out.update({key + "_x": x, key + "_niter": it, key + "_error": tr, key + "_status": gst,
+                    key + "_max_iter": mi, key + "_tol": tol})
+        print(f"gauss-seidel {key}: {it} sweeps, error {tr:.3e}, status {gst}")
+    np.savez_compressed(os.path.join(OUT, "gauss_seidel_16x8.npz"), **out)
+
+
 def c2_case(nx=1024, ny=512, name="c2_reference_solve", step=101):
     """The reference's FP64 Jacobi-PCG on the nx x ny lognormal cantilever
     (1024x512: 1,051,650 DOF; 2048x1024: 4,200,450 DOF), K and F from the
@@ -274,6 +307,8 @@ def main(argv):
         return
     if argv[1:] == ["damage"]:
         damage_cases()
+        damage_cut_case()
+        gauss_seidel_cases()
         return
     if argv[1:] == ["plate"]:
         beam_case("plate_with_hole.txt", "plate_with_hole", kirsch=True)
@@ -296,6 +331,8 @@ def main(argv):
     cg_cases()
     file_cases()
     damage_cases()
+    damage_cut_case()
+    gauss_seidel_cases()
     c2_case()
     c2_case(2048, 1024, "c3_reference_solve", step=401)
 

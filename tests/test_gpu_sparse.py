@@ -358,3 +358,39 @@ def test_mixed_solve_wide_node_numbering():
     assert info0["status"] == 0 and info0["layout"] == 7, info0
     assert np.linalg.norm(x[dof] - x0) <= 1e-6 * np.linalg.norm(x0)
     A0.destroy()
+
+
+def test_gauss_seidel_drop_in_matches_reference():
+    """nb_sparse_solve_Gauss_Seidel on the GPU against the compiled
+    reference's (gauss_seidel.c:13-63, fixture gauss_seidel_16x8.npz) on K and
+    F of the 16x8 cantilever: after 200 sweeps and after 20000, x bit for bit
+    (the level-scheduled substitution sums each row in the reference's
+    order), the same sweep count and status, the error within 1e-12 (its
+    sum runs in another order).  A system that converges stops at the same
+    sweep as the oracle's."""
+    import nbots_amd as nb
+    import oracle
+    from nbots_amd import synth
+    g = load_golden("gauss_seidel_16x8")
+    p = synth.cantilever_traction(16, 8)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    for key in ("cut", "tol"):
+        x, st, it, err = A.solve_gauss_seidel(F, max_iter=int(g[key + "_max_iter"]),
+                                              tol=float(g[key + "_tol"]))
+        assert np.array_equal(x, g[key + "_x"]), key
+        assert (st, it) == (int(g[key + "_status"]), int(g[key + "_niter"]))
+        assert abs(err - float(g[key + "_error"])) <= 1e-12 * float(g[key + "_error"])
+    A.destroy()
+    # diagonally dominant: converges; x, sweeps and status as the oracle's
+    rp2 = np.array([0, 2, 5, 7], np.uint32)
+    col2 = np.array([0, 1, 0, 1, 2, 1, 2], np.uint32)
+    val2 = np.array([4.0, 1.0, 1.0, 5.0, 2.0, 2.0, 6.0])
+    b2 = np.array([1.0, 2.0, 3.0])
+    B = nb.Sparse.from_csr(rp2, col2, val2, vars_per_node=1)
+    x, st, it, err = B.solve_gauss_seidel(b2, tol=1e-12)
+    xo, ito, erro, sto = oracle.gauss_seidel(rp2, col2, val2, b2, tol=1e-12)
+    assert st == sto == 0 and it == ito and np.array_equal(x, xo)
+    B.destroy()

@@ -191,3 +191,38 @@ def test_damage_solver_log_matches_reference():
     with open(os.path.join(GOLDEN, "damage_24x12_log.txt")) as f:
         assert r["log"] == f.read()
     assert 0.5 < r["damage"].max() < 1.0 and (r["damage"] > 0).sum() >= 5
+
+
+def test_gauss_seidel_matches_reference():
+    """orc_gauss_seidel against the compiled reference's
+    nb_sparse_solve_Gauss_Seidel (gauss_seidel.c:13-63, 1 thread) on K and F
+    of the 16x8 cantilever: x bit for bit, the sweep count, the error and the
+    status, cut at 200 sweeps and after 20000."""
+    from nbots_amd import synth
+    g = load_golden("gauss_seidel_16x8")
+    p = synth.cantilever_traction(16, 8)
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    for key in ("cut", "tol"):
+        x, it, err, st = oracle.gauss_seidel(rp, col, val, F, max_iter=int(g[key + "_max_iter"]),
+                                             tol=float(g[key + "_tol"]))
+        assert np.array_equal(x, g[key + "_x"])
+        assert (it, err, st) == (int(g[key + "_niter"]), float(g[key + "_error"]),
+                                 int(g[key + "_status"]))
+
+
+def test_damage_solver_gauss_seidel_fallback_log():
+    """A slender 32x1 beam whose Jacobi-PCG stops at N iterations short of
+    its absolute tolerance 1e-8: the reference then runs Gauss-Seidel and
+    logs its failures (static_damage2D.c:414-427).  The restated loop
+    writes the compiled reference's log character for character."""
+    import os
+    from conftest import GOLDEN
+    from nbots_amd import synth
+    r = oracle.damage_solve(synth.damage_beam(32, 1), Gf=1e-3, ft=1e6, n_steps=2, max_iter=4,
+                            max_iter_wo_enhance=3, res_tol=1e-10)
+    with open(os.path.join(GOLDEN, "damage_cut_32x1_log.txt")) as f:
+        want = f.read()
+    assert r["log"] == want
+    assert want.count("Gauss-Seidel fails") == 4 and r["damage"].max() == 0.0
