@@ -368,6 +368,30 @@ int nbx_sparse_import_values(nb_sparse_t *A, const double *val);
 /* 1 when the device copy uses the 2x2-block format. */
 int nbx_sparse_is_blocked(const nb_sparse_t *A);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * (code between graph captures, tight serving loops, harnesses that forbid
+ * hipMalloc): nb_sparse_solve_CG_precond_Jacobi (cg_precond_jacobi.h:8-15,
+ * same semantics: absolute tolerance on ||Ax - b||_2, lagged stop test, x
+ * the initial guess, 0 converged / 1 not) on a scalar CSR matrix
+ * (rowptr[n+1], col[nnz] ascending per row, val[nnz]) and vectors b[n],
+ * x[n] already in device memory.  It allocates nothing: its work vectors,
+ * reduction partials and solver state live in a fixed array of this
+ * library (16.8 MiB on each GPU it runs on, held until the process ends).
+ * At most NBX_NOALLOC_MAX_N rows (else NBX_ERR_ARG).  Calls that overlap on
+ * one GPU share that array and race.  stream: a hipStream_t, NULL for the
+ * default stream; the call returns when the solve is done.              */
+#define NBX_NOALLOC_MAX_N 262144u
+int nbx_sparse_solve_CG_precond_Jacobi_noalloc(uint32_t n,
+					       const uint32_t *rowptr_dev,
+					       const uint32_t *col_dev,
+					       const double *val_dev,
+					       const double *b_dev,
+					       double *x_dev,
+					       uint32_t max_iter,
+					       double tolerance,
+					       uint32_t *niter_performed,
+					       double *tolerance_reached,
+					       void *stream);
 
 /* ---- device-resident T3 plane-stress system ----------------------------- */
 typedef struct nbx_fem_s nbx_fem_t;

@@ -150,6 +150,13 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
                double tol, uint32_t* niter, double* tol_reached, Workspace& w,
                hipStream_t s, SolveStats* stats, bool unpreconditioned = false);
 
+// The same on a scalar CSR already in device memory with a caller-made
+// workspace (vectors v64[0..2], v64[6..8], partials, st, a host mirror
+// h_st; no events): nothing is allocated.  diag holds A(i,i).
+int pcg_jacobi_csr_view(const DevCsr& c, const double* diag, const double* b, double* x,
+                        uint32_t max_iter, double tol, uint32_t* niter, double* tol_reached,
+                        Workspace& w, hipStream_t s);
+
 // FP32 inner CG on the Jacobi-scaled FP32 copy inside FP64 iterative
 // refinement; stops when ||b - A x||_2 <= rtol * ||b||_2.
 int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol,

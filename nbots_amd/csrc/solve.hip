@@ -716,7 +716,9 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
     h.max_iter = max_iter;
     NBX_HIP(hipMemcpyAsync(w.st, &h, sizeof h, hipMemcpyHostToDevice, s));
     const unsigned G = red_grid(nu);
-    NBX_HIP(hipEventRecord(w.ev[0], s));
+    // a workspace without events (the no-allocation entry point) is not timed
+    const bool timed = w.ev[0] != nullptr;
+    if (timed) NBX_HIP(hipEventRecord(w.ev[0], s));
     hipLaunchKernelGGL(k_pcg_init<Op>, dim3(G), dim3(BLOCK), 0, s, op, b, x, g, p, diag, w.st,
                        w.partials, nu);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
@@ -727,7 +729,7 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
     uint32_t batch = 4, nsamp = 0;
     for (;;) {
         for (uint32_t i = 0; i < batch; ++i) {
-            const bool samp = (launched + i) % Workspace::SAMPLE_EVERY == 0 &&
+            const bool samp = timed && (launched + i) % Workspace::SAMPLE_EVERY == 0 &&
                               nsamp < (uint32_t)Workspace::NSAMP;
             if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
             // iteration j = launched + i (kernels after the stop return at once)
@@ -773,6 +775,14 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
     }
     hipLaunchKernelGGL(k_pcg_tail<V>, dim3(G), dim3(BLOCK), 0, s, x, pbuf, w.st, nu);
     NBX_HIP(hipGetLastError());
+    if (!timed) {
+        NBX_HIP(hipStreamSynchronize(s));
+        if (stats) {
+            stats->iters = stats->fp64_iters = w.h_st->k;
+            stats->res0 = sqrt(w.h_st->gg0);
+        }
+        return NBX_OK;
+    }
     NBX_HIP(hipEventRecord(w.ev[1], s));
     NBX_HIP(hipEventSynchronize(w.ev[1]));
     if (stats) {
@@ -835,6 +845,21 @@ int pcg_jacobi(const DevMatrix& m, const double* b, double* x, uint32_t max_iter
         rc = pcg_run(OpCsr{m.c}, m.c.n, diag, b, x, max_iter, tol, stream, w, s, stats);
     if (rc) return -rc;
     if (stats) stats->layout = m.blocked && m.has_sym ? 1 : 0;
+    if (niter) *niter = w.h_st->k;
+    if (tol_reached) *tol_reached = w.h_st->tol_reached;
+    return (int)w.h_st->status;
+}
+
+int pcg_jacobi_csr_view(const DevCsr& c, const double* diag, const double* b, double* x,
+                        uint32_t max_iter, double tol, uint32_t* niter, double* tol_reached,
+                        Workspace& w, hipStream_t s) {
+    if (c.n == 0) {
+        if (niter) *niter = 0;
+        if (tol_reached) *tol_reached = 0.0;
+        return 0;
+    }
+    const int rc = pcg_run(OpCsr{c}, c.n, diag, b, x, max_iter, tol, false, w, s, nullptr);
+    if (rc) return -rc;
     if (niter) *niter = w.h_st->k;
     if (tol_reached) *tol_reached = w.h_st->tol_reached;
     return (int)w.h_st->status;

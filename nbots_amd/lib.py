@@ -193,6 +193,8 @@ SIGS = {
     "nbx_simp_opts_default": (None, [C.POINTER(SolveOpts)]),
     "nbx_sparse_spmv_dev": (I32, [VP, VP, VP]),
     "nbx_sparse_is_blocked": (I32, [VP]),
+    "nbx_sparse_solve_CG_precond_Jacobi_noalloc": (I32, [U32, VP, VP, VP, VP, VP, U32, F64,
+                                                         C.POINTER(U32), C.POINTER(F64), VP]),
     "nbx_sparse_export_csr": (I32, [VP, u32o, u32o, f64o]),
     "nbx_sparse_import_values": (I32, [VP, f64p]),
     "nbx_fem_create": (I32, [C.POINTER(VP), U32, f64p, U32, u32p]),
