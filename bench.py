@@ -335,6 +335,15 @@ def pmc_probe(args):
 
 # ---- GPU solvers --------------------------------------------------------------------
 
+EAGER_LAUNCH = False   # --eager-launch: kernel by kernel, no captured-graph replay
+
+
+def _opts(*a, **kw):
+    """nbx_solve_opts_t for every solve of this run (see nbots_amd.fem.solve_opts)."""
+    from nbots_amd.fem import solve_opts
+    return solve_opts(*a, eager_launch=EAGER_LAUNCH, **kw)
+
+
 def time_solver(f, mode, opts, reps, label, max_iter=200000):
     """One untimed solve, then `reps` timed ones; returns (median seconds,
     last info)."""
@@ -352,9 +361,9 @@ def time_solver(f, mode, opts, reps, label, max_iter=200000):
 def decompose(f, args, t_fp64, iters64, u64, Fl):
     """The mixed-precision speedup split into the preconditioner's gain and
     the precision's, plus the FP32-CG + FP64-refinement scheme."""
-    from nbots_amd.fem import SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART, solve_opts
+    from nbots_amd.fem import SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART
     out = {}
-    o16 = solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+    o16 = _opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
                      precond_bits=args.precond_bits, true_residual=True)
     t16, i16 = time_solver(f, SOLVE_MIXED, o16, 2, "mixed (fp16 preconditioner)")
     u16 = f.displacement()
@@ -376,11 +385,11 @@ def decompose(f, args, t_fp64, iters64, u64, Fl):
         mixed_vs_fp64_compliance_rel=abs(float(Fl @ u16) - float(Fl @ u64)) / abs(float(Fl @ u64)))
     if args.no_decompose:
         return out
-    o64 = solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
+    o64 = _opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
                      precond_bits=64, true_residual=True)
     t64, i64 = time_solver(f, SOLVE_MIXED, o64, 2, "FCG, FP64 preconditioner")
     true64p = f.true_residual(True)
-    oir = solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, true_residual=True)
+    oir = _opts(MIXED_CG_RESTART, inner_rtol=1e-4, true_residual=True)
     tir, iir = time_solver(f, SOLVE_MIXED, oir, 1, "FP32 CG + FP64 refinement",
                            max_iter=args.mixed_max_iter)
     trueir = f.true_residual(True)
@@ -417,21 +426,21 @@ def simp_bench(args, opts16):
     --simp-fp32ir-iters iterations, FP32 CG in FP64 refinement."""
     from nbots_amd import synth
     from nbots_amd.fem import (Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG,
-                               MIXED_CG_RESTART, solve_opts)
+                               MIXED_CG_RESTART)
     nx, ny = 1024, 512
     p = synth.mbb(nx, ny, rho=0.5)
     out = dict(simp_workload=f"{nx}x{ny} MBB half-beam, {p.N} DOF, {p.N_elem} T3, volfrac 0.5, "
                              "penal 3, filter radius 1.5, OC move 0.2, rho0 = 0.5",
                simp_iterations=args.simp_iters)
     variants = [("mixed", SOLVE_MIXED, opts16, args.simp_iters),
-                ("fp64", SOLVE_FP64, solve_opts(true_residual=True, simp=True), args.simp_iters)]
+                ("fp64", SOLVE_FP64, _opts(true_residual=True, simp=True), args.simp_iters)]
     if not args.no_decompose:
         variants.insert(1, ("fcg_fp64_precond", SOLVE_MIXED,
-                            solve_opts(MIXED_CHEB_FCG, precond_bits=64, simp=True,
+                            _opts(MIXED_CHEB_FCG, precond_bits=64, simp=True,
                                        true_residual=True), args.simp_iters))
         if args.simp_fp32ir_iters > 0:
             variants.append(("fp32ir", SOLVE_MIXED,
-                             solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, simp=True,
+                             _opts(MIXED_CG_RESTART, inner_rtol=1e-4, simp=True,
                                         true_residual=True), args.simp_fp32ir_iters))
     runs = {}
     for name, mode, o, n_it in variants:
@@ -522,9 +531,14 @@ def main():
     ap.add_argument("--simp-iters", type=int, default=50)
     ap.add_argument("--simp-fp32ir-iters", type=int, default=10)
     ap.add_argument("--no-simp", action="store_true")
+    ap.add_argument("--eager-launch", action="store_true",
+                    help="launch the solvers' kernels one by one instead of replaying captured "
+                         "HIP graphs of their repeated steps (same results)")
     ap.add_argument("--simp-only", action="store_true",
                     help="only the SIMP loop, its results as one JSON line (for profiling)")
     args = ap.parse_args()
+    global EAGER_LAUNCH
+    EAGER_LAUNCH = args.eager_launch
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)
@@ -536,14 +550,14 @@ def main():
         pmc_probe(args)
         return
     if args.simp_only:
-        from nbots_amd.fem import MIXED_CHEB_FCG, solve_opts
-        print(json.dumps(simp_bench(args, solve_opts(
+        from nbots_amd.fem import MIXED_CHEB_FCG
+        print(json.dumps(simp_bench(args, _opts(
             MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
             precond_bits=args.precond_bits, simp=True, true_residual=True))), flush=True)
         return
 
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, SOLVE_FP64, MIXED_CHEB_FCG, solve_opts
+    from nbots_amd.fem import Fem, SOLVE_FP64, MIXED_CHEB_FCG
 
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
@@ -567,7 +581,7 @@ def main():
     # The solvers stop on the true residual ||F - K u|| / ||F|| <= 1e-8
     # (restarting from the iterate when their recursively updated residual
     # has drifted from it).
-    o64 = solve_opts(true_residual=True)
+    o64 = _opts(true_residual=True)
     for w in range(args.warmup):
         info = f.solve(SOLVE_FP64, tol=1e-8, max_iter=200000, opts=o64)
         log(f"[bench] warmup {w}: FP64 PCG {info['iters']} it, {info['ms']/1e3:.3f}s")
@@ -651,7 +665,7 @@ def main():
     f.destroy()
     simp = {}
     if not args.no_simp:
-        simp = simp_bench(args, solve_opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree,
+        simp = simp_bench(args, _opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree,
                                            cheb_ratio=args.cheb_ratio,
                                            precond_bits=args.precond_bits, simp=True,
                                            true_residual=True))

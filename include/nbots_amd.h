@@ -311,6 +311,15 @@ typedef struct {
 				    true residual (restarts, above); 0 on the
 				    recursively updated one, as the reference
 				    does (default)                            */
+	int32_t eager_launch;    /* 0 (default): the FP64 PCG and the
+				    Chebyshev flexible CG capture their
+				    repeated steps (four PCG iterations, one
+				    outer iteration) once per solve as HIP
+				    graphs and replay them, one host call per
+				    step; 1: launch kernel by kernel.  The
+				    same kernels in the same order: the same
+				    results either way.                       */
+	int32_t reserved;        /* 0                                         */
 } nbx_solve_opts_t;
 /* Fills *opts with the defaults above. */
 void nbx_solve_opts_default(nbx_solve_opts_t *opts);
@@ -341,7 +350,9 @@ typedef struct {
 				 (each scaled diagonal entry is 1 in fp16),
 				 else the full fp16 copy; bit 2: ... with
 				 its block columns as packed 16-bit deltas
-				 (every |w - v| < 32768)                    */
+				 (every |w - v| < 32768); bit 3: ... with
+				 every slice as wide as the widest row
+				 (rows placed by their index alone)         */
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 

@@ -122,6 +122,10 @@ struct Off16 {
     size_t n_pairs = 0;              // stored words, padding included
     uint32_t* pslice_ptr = nullptr;  // [n_slices + 1]
     uint32_t* pcol = nullptr;        // [n_pairs]
+    // > 0: every slice is `width` blocks (and (width + 1) / 2 words) wide,
+    // so row v's entries start at (v / 64) 64 width + v % 64 and
+    // (v / 64) 64 ((width + 1) / 2) + v % 64; padding has value 0, column v
+    uint32_t width = 0;
 };
 
 // Symmetric 2x2-block sliced ELL, for a node matrix whose blocks satisfy

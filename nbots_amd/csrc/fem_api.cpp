@@ -45,6 +45,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
     nbx::SolveStats st;
     int status = 0;
     memset(info, 0, sizeof *info);
+    S.ws.replay = !mo.eager_launch;
     NBX_TRY(nbx::mat_prepare_sym(S.K, S.s));
     info->layout = S.K.has_sym ? 1 : 0;
     if (mode == NBX_SOLVE_REFERENCE) {
@@ -66,7 +67,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
         status = nbx::solve_mixed(S.K, S.F, S.u, tol, mo, max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
-        info->layout |= st.layout & 6;   // the preconditioner's fp16 copy
+        info->layout |= st.layout & 14;   // the preconditioner's fp16 copy
         info->iters = st.iters;
         info->outer = st.outer;
         info->fp64_iters = st.fp64_iters;
@@ -180,6 +181,8 @@ static void opts_from(const nbx::MixedOpts& d, nbx_solve_opts_t* o) {
     o->reliable_delta = d.delta;
     o->precond_bits = d.precond_bits;
     o->true_residual = d.true_residual ? 1 : 0;
+    o->eager_launch = d.eager_launch ? 1 : 0;
+    o->reserved = 0;
 }
 
 void nbx_solve_opts_default(nbx_solve_opts_t* o) { opts_from(nbx::MixedOpts(), o); }

@@ -80,13 +80,16 @@ struct Workspace {
     float* v32[4] = {};        // FP32 vectors
     double* partials = nullptr;
     float* partials32 = nullptr;
-    unsigned* ticket = nullptr;
     PcgState* st = nullptr;    // device state (2 slots: outer, inner)
     PcgState* h_st = nullptr;  // pinned host mirror
     hipEvent_t ev[4] = {};
     static constexpr int NSAMP = 32;    // sampled SpMV launches per solve, one every
     static constexpr int SAMPLE_EVERY = 64;   // SAMPLE_EVERY iterations (PCG)
     hipEvent_t sev[2 * NSAMP] = {};
+    // Replay a solve's repeated steps as captured HIP graphs (one host call
+    // per step instead of one per kernel; same kernels, same results).  Off
+    // for launches that must not allocate.
+    bool replay = true;
 };
 
 struct SolveStats {
@@ -116,6 +119,7 @@ struct MixedOpts {
     int precond_bits = 16;     // Chebyshev preconditioner: fp16 or FP32 scaled values (FP64
                                // sums, FP32 vectors), or 64: the FP64 matrix, FP64 vectors
     bool true_residual = false; // stop on the true residual (restarts), FP64 and CHEB_FCG
+    bool eager_launch = false;  // launch kernel by kernel instead of replaying captured steps
 };
 
 int ws_alloc(Workspace& w, uint32_t n);

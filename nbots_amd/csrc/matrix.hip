@@ -393,8 +393,6 @@ int ws_alloc(Workspace& w, uint32_t n) {
     for (auto& p : w.v32) NBX_HIP(hipMalloc(&p, nn * sizeof(float)));
     NBX_HIP(hipMalloc(&w.partials, 4 * MAX_GRID * sizeof(double)));
     NBX_HIP(hipMalloc(&w.partials32, 4 * MAX_GRID * sizeof(float)));
-    NBX_HIP(hipMalloc(&w.ticket, 8 * sizeof(unsigned)));
-    NBX_HIP(hipMemset(w.ticket, 0, 8 * sizeof(unsigned)));
     NBX_HIP(hipMalloc(&w.st, 2 * sizeof(PcgState)));
     NBX_HIP(hipHostMalloc(&w.h_st, 2 * sizeof(PcgState), hipHostMallocDefault));
     for (auto& e : w.ev) NBX_HIP(hipEventCreate(&e));
@@ -414,7 +412,6 @@ void ws_free(Workspace& w) {
     }
     if (w.partials) (void)hipFree(w.partials);
     if (w.partials32) (void)hipFree(w.partials32);
-    if (w.ticket) (void)hipFree(w.ticket);
     if (w.st) (void)hipFree(w.st);
     if (w.h_st) (void)hipHostFree(w.h_st);
     for (auto& e : w.ev) {
@@ -427,7 +424,6 @@ void ws_free(Workspace& w) {
     }
     w.partials = nullptr;
     w.partials32 = nullptr;
-    w.ticket = nullptr;
     w.st = nullptr;
     w.h_st = nullptr;
     w.n = 0;
@@ -774,6 +770,18 @@ static int off_build_pattern(DevMatrix& m, hipStream_t s) {
     NBX_TRY(read_flag(m.off_flag, s, &bad));
     std::vector<uint32_t> osp;
     if (bad || !slice_offsets(ol, ns, osp)) return give_up();
+    // Every slice as wide as the widest row when that adds at most 1/16 of
+    // padding: a row's entries then sit where its index alone says, and a
+    // product reads them without first loading the slice offset and the
+    // row length.
+    uint32_t wmax = 0;
+    for (uint8_t l : ol) wmax = wmax > l ? wmax : l;
+    const uint64_t uni = (uint64_t)ns * SLICE * wmax;
+    O.width = 0;
+    if (wmax > 0 && uni <= (uint64_t)osp[ns] + osp[ns] / 16 && uni <= 0xffffffffull) {
+        for (uint32_t sl = 0; sl <= ns; ++sl) osp[sl] = sl * SLICE * wmax;
+        O.width = wmax;
+    }
     O.n_nodes = nn;
     O.n_slices = ns;
     O.n_blocks = osp[ns];
@@ -784,6 +792,8 @@ static int off_build_pattern(DevMatrix& m, hipStream_t s) {
         return give_up();
     NBX_HIP(hipMemcpyAsync(O.slice_ptr, osp.data(), (ns + 1) * sizeof(uint32_t),
                            hipMemcpyHostToDevice, s));
+    // padding values stay 0 (the value kernels write the rows' own blocks)
+    NBX_HIP(hipMemsetAsync(O.val, 0, (4 * O.n_blocks + 4) * sizeof(_Float16), s));
     hipLaunchKernelGGL(k_off_pattern, dim3(grid_for(nn)), dim3(BLOCK), 0, s, A, O);
     NBX_HIP(hipGetLastError());
     m.off_pattern = true;

@@ -95,8 +95,9 @@ extern "C" int nbx_sparse_solve_CG_precond_Jacobi_noalloc(
     NBX_TRY(nbx::mat_extract_diag(m, s));
 
     nbx::PcgState host_state{};
-    nbx::Workspace w;   // a view: no events, no pinned mirror
+    nbx::Workspace w;   // a view: no events, no pinned mirror, no captured graphs
     w.n = n;
+    w.replay = false;
     w.v64[0] = slot(1);   // g
     w.v64[1] = slot(2);   // p_j, j % 4 = 0
     w.v64[2] = slot(3);   // w

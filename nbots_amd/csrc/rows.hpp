@@ -349,11 +349,14 @@ __device__ __forceinline__ double2 row16d(const Sell2& A, uint32_t v, const floa
 // loaded), then the off-diagonal blocks.
 // P: read the columns as packed 16-bit deltas (A.pcol), one 32-bit load per
 // two blocks, instead of A.bcol.
-template <bool NT, bool P>
+// U: the uniform-width layout (A.width > 0), whose rows run over the full
+// width, padding included (value 0, column v: adds 0).
+template <bool NT, bool P, bool U = false>
 __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const float2* __restrict__ x,
                                            float2 xo) {
-    const size_t base = (size_t)A.slice_ptr[v >> 6] + (v & 63);
-    const int len = A.rowlen[v];
+    const size_t base = U ? (size_t)(v >> 6) * SLICE * A.width + (v & 63)
+                          : (size_t)A.slice_ptr[v >> 6] + (v & 63);
+    const int len = U ? (int)A.width : (int)A.rowlen[v];
     const hv4* __restrict__ val = reinterpret_cast<const hv4*>(A.val);
     hv2 od;
     if constexpr (NT)
@@ -367,7 +370,9 @@ __device__ __forceinline__ double2 row16od(const Off16& A, uint32_t v, const flo
         hv4 a[UNROLL_OD];
         float2 xv[UNROLL_OD];
         if constexpr (P) {
-            const size_t pb = (size_t)A.pslice_ptr[v >> 6] + (v & 63) + (size_t)(k0 / 2) * SLICE;
+            const size_t pb = (U ? (size_t)(v >> 6) * SLICE * ((A.width + 1) / 2)
+                                 : (size_t)A.pslice_ptr[v >> 6]) +
+                              (v & 63) + (size_t)(k0 / 2) * SLICE;
             uint32_t pw[UNROLL_OD / 2];
 #pragma unroll
             for (int j = 0; j < UNROLL_OD / 2; ++j)
@@ -426,6 +431,15 @@ template <bool NT>
 __device__ __forceinline__ double2 cheb_row(const Off16P& A, uint32_t v, const float2* __restrict__ x,
                                             float2 xo) {
     return row16od<NT, true>(A.O, v, x, xo);
+}
+// ... in the uniform-width layout
+struct Off16U {
+    Off16 O;
+};
+template <bool NT>
+__device__ __forceinline__ double2 cheb_row(const Off16U& A, uint32_t v, const float2* __restrict__ x,
+                                            float2 xo) {
+    return row16od<NT, true, true>(A.O, v, x, xo);
 }
 
 // y = A x with the scaled copy of BITS-bit values

@@ -699,6 +699,39 @@ static int sampled_ms(Workspace& w, uint32_t n, double* avg) {
     return NBX_OK;
 }
 
+// A fixed sequence of kernel launches captured once and replayed with
+// hipGraphLaunch.  The solvers' repeated steps launch the same kernels with
+// the same arguments every time (their state lives on the device), so one
+// replay issues a step's kernels with one host call instead of one per
+// kernel: the same kernels in the same order as launched one by one, so the
+// same results.  The sequences hold kernel launches only.
+struct Replay {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Replay() = default;
+    Replay(const Replay&) = delete;
+    Replay& operator=(const Replay&) = delete;
+    ~Replay() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+template <class F>
+static int capture(hipStream_t s, F&& launches, Replay& r) {
+    NBX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    launches();
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(s, &r.graph);
+    if (le != hipSuccess || ee != hipSuccess) {
+        set_last_error("stream capture of the solver's steps: %s",
+                       hipGetErrorString(le != hipSuccess ? le : ee));
+        return NBX_ERR_HIP;
+    }
+    NBX_HIP(hipGraphInstantiate(&r.exec, r.graph, nullptr, nullptr, 0));
+    return NBX_OK;
+}
+
 // diag: the Jacobi diagonal, or all ones for unpreconditioned CG (q = g/1
 // is g exactly, so the same kernels give conjugate_gradient.c:13-77).
 template <class Op>
@@ -723,43 +756,62 @@ static int pcg_run(const Op& op, uint32_t nu, const double* diag_in, const typen
                        w.partials, nu);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
     NBX_HIP(hipGetLastError());
+    uint32_t nsamp = 0;   // sampled SpMV launches so far
+    // Iteration j's kernels (after the stop they return at once).  They
+    // depend on j only through j % 4: the direction buffers rotate and every
+    // fourth iteration flushes the x updates.
+    auto iteration = [&](uint32_t j, bool samp) {
+        j &= 3u;
+        const V* pj = pbuf.p[j];
+        const bool flush = j == 3u;
+        if (samp) (void)hipEventRecord(w.sev[2 * nsamp], s);
+        hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, pj, wv, w.st, w.partials, nu);
+        if (samp) (void)hipEventRecord(w.sev[2 * nsamp++ + 1], s);
+        hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+        if (stream) {
+            hipLaunchKernelGGL((k_pcg_B<V, true>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
+                               w.partials, nu);
+            hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+            if (flush)
+                hipLaunchKernelGGL((k_pcg_C<V, true, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf, j,
+                                   g, diag, w.st, nu);
+            else
+                hipLaunchKernelGGL((k_pcg_C<V, true, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf, j,
+                                   g, diag, w.st, nu);
+        } else {
+            hipLaunchKernelGGL((k_pcg_B<V, false>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
+                               w.partials, nu);
+            hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+            if (flush)
+                hipLaunchKernelGGL((k_pcg_C<V, false, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf, j,
+                                   g, diag, w.st, nu);
+            else
+                hipLaunchKernelGGL((k_pcg_C<V, false, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
+                                   j, g, diag, w.st, nu);
+        }
+    };
+    // Four iterations (j % 4 = 0..3) captured once and replayed: batches
+    // hold multiples of four iterations and start at j % 4 = 0.
+    Replay four;
+    if (w.replay && s != nullptr)
+        NBX_TRY(capture(s, [&] {
+            for (uint32_t q = 0; q < 4; ++q) iteration(q, false);
+        }, four));
     // 64-bit, so that max_iter near UINT32_MAX ("no limit") cannot wrap the
     // guard below
     uint64_t launched = 0;
-    uint32_t batch = 4, nsamp = 0;
+    uint32_t batch = 4;
     for (;;) {
-        for (uint32_t i = 0; i < batch; ++i) {
-            const bool samp = timed && (launched + i) % Workspace::SAMPLE_EVERY == 0 &&
+        for (uint32_t i = 0; i < batch; i += 4) {
+            // iterations j0 .. j0 + 3; a sampled one (j0 % SAMPLE_EVERY = 0)
+            // is launched kernel by kernel between its events
+            const uint64_t j0 = launched + i;
+            const bool samp = timed && j0 % Workspace::SAMPLE_EVERY == 0 &&
                               nsamp < (uint32_t)Workspace::NSAMP;
-            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
-            // iteration j = launched + i (kernels after the stop return at once)
-            const uint32_t j = (uint32_t)(launched + i);
-            const V* pj = pbuf.p[j & 3u];
-            const bool flush = (j & 3u) == 3u;
-            hipLaunchKernelGGL(k_pcg_A<Op>, dim3(G), dim3(BLOCK), 0, s, op, pj, wv, w.st, w.partials,
-                               nu);
-            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
-            hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            if (stream) {
-                hipLaunchKernelGGL((k_pcg_B<V, true>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag, w.st,
-                                   w.partials, nu);
-                hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-                if (flush)
-                    hipLaunchKernelGGL((k_pcg_C<V, true, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf, j,
-                                       g, diag, w.st, nu);
-                else
-                    hipLaunchKernelGGL((k_pcg_C<V, true, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
-                                       j, g, diag, w.st, nu);
+            if (four.exec && !samp) {
+                NBX_HIP(hipGraphLaunch(four.exec, s));
             } else {
-                hipLaunchKernelGGL((k_pcg_B<V, false>), dim3(G), dim3(BLOCK), 0, s, g, wv, diag,
-                                   w.st, w.partials, nu);
-                hipLaunchKernelGGL(k_fin_B, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-                if (flush)
-                    hipLaunchKernelGGL((k_pcg_C<V, false, true>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
-                                       j, g, diag, w.st, nu);
-                else
-                    hipLaunchKernelGGL((k_pcg_C<V, false, false>), dim3(G), dim3(BLOCK), 0, s, x, pbuf,
-                                       j, g, diag, w.st, nu);
+                for (uint32_t q = 0; q < 4; ++q) iteration(q, samp && q == 0);
             }
         }
         NBX_HIP(hipGetLastError());
@@ -1133,6 +1185,8 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
             else if (p64)
                 launch_cheb_step_f64(k, deg - 1, G, s, op, din64, dout64, res64, z64, dis, w.st,
                                      nu, c1d[k], c2d[k]);
+            else if (packed && m.o.width)
+                step(Off16U{m.o});
             else if (packed)
                 step(Off16P{m.o});
             else if (off16)
@@ -1167,19 +1221,30 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     uint32_t batch = 4, nsamp = 0;
     uint64_t launched = 0;   // 64-bit: max_iter near UINT32_MAX cannot wrap the guard
     constexpr uint32_t every = 8;   // sample one outer iteration in eight
+    // An outer iteration: the FP64 product, then the preconditioner and the
+    // flexible-CG updates; the same kernels and arguments every time.
+    auto outer = [&](bool samp) {
+        if (samp) (void)hipEventRecord(w.sev[2 * nsamp], s);
+        if (m.has_sym)
+            hipLaunchKernelGGL(k_pcg_A<OpSym>, dim3(G), dim3(BLOCK), 0, s, ops,
+                               (const double2*)p, wv, w.st, w.partials, nu);
+        else
+            hipLaunchKernelGGL(k_pcg_A<OpS2>, dim3(G), dim3(BLOCK), 0, s, op,
+                               (const double2*)p, wv, w.st, w.partials, nu);
+        if (samp) (void)hipEventRecord(w.sev[2 * nsamp++ + 1], s);
+        hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
+        precondition(0);
+    };
+    Replay one;
+    if (w.replay && s != nullptr && !w.h_st->done)
+        NBX_TRY(capture(s, [&] { outer(false); }, one));
     while (!w.h_st->done) {
         for (uint32_t i = 0; i < batch; ++i) {
             const bool samp = (launched + i) % every == 0 && nsamp < (uint32_t)Workspace::NSAMP;
-            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp], s));
-            if (m.has_sym)
-                hipLaunchKernelGGL(k_pcg_A<OpSym>, dim3(G), dim3(BLOCK), 0, s, ops,
-                                   (const double2*)p, wv, w.st, w.partials, nu);
+            if (one.exec && !samp)
+                NBX_HIP(hipGraphLaunch(one.exec, s));
             else
-                hipLaunchKernelGGL(k_pcg_A<OpS2>, dim3(G), dim3(BLOCK), 0, s, op,
-                                   (const double2*)p, wv, w.st, w.partials, nu);
-            if (samp) NBX_HIP(hipEventRecord(w.sev[2 * nsamp++ + 1], s));
-            hipLaunchKernelGGL(k_fin_A, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G);
-            precondition(0);
+                outer(samp);
         }
         NBX_HIP(hipGetLastError());
         launched += batch;
@@ -1201,7 +1266,8 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
-        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0) | (packed ? 4 : 0);
+        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0) | (packed ? 4 : 0) |
+                        (packed && m.o.width ? 8 : 0);
         stats->res0 = sqrt(w.h_st->gg0);
         const uint32_t ran = (w.h_st->k + every - 1) / every;
         NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));

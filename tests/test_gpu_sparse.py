@@ -116,7 +116,7 @@ def test_sparse_true_residual_stop(mode):
     oracle.set_bconditions(p, rp, col, val, F)
     A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
     x, info = A.solve(F, mode=mode, tol=1e-10, opts=solve_opts(true_residual=True))
-    assert info["status"] == 0 and info["layout"] == (7 if mode == 2 else 1)
+    assert info["status"] == 0 and info["layout"] == (15 if mode == 2 else 1)
     r = F - oracle.spmv(rp, col, val, x)
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(F)
     A.destroy()
@@ -322,7 +322,7 @@ def test_plain_cg_edge_cases():
 def test_mixed_solve_wide_node_numbering():
     """Nodes numbered at random (512x128 cantilever, 66,177 nodes): block
     columns lie up to 66k rows from the diagonal, beyond the 16-bit deltas,
-    so the fp16 preconditioner reads its 32-bit columns (layout 3, not 7).
+    so the fp16 preconditioner reads its 32-bit columns (layout 3, not 15).
     The mixed solve still meets the tolerance on the oracle's residual and
     agrees with the solve in the original numbering."""
     import scipy.sparse as sp
@@ -355,7 +355,7 @@ def test_mixed_solve_wide_node_numbering():
     A.destroy()
     A0 = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
     x0, info0 = A0.solve(F, mode=nb.SOLVE_MIXED, tol=1e-8, opts=opts)
-    assert info0["status"] == 0 and info0["layout"] == 7, info0
+    assert info0["status"] == 0 and info0["layout"] == 15, info0
     assert np.linalg.norm(x[dof] - x0) <= 1e-6 * np.linalg.norm(x0)
     A0.destroy()
 
@@ -394,3 +394,30 @@ def test_gauss_seidel_drop_in_matches_reference():
     xo, ito, erro, sto = oracle.gauss_seidel(rp2, col2, val2, b2, tol=1e-12)
     assert st == sto == 0 and it == ito and np.array_equal(x, xo)
     B.destroy()
+
+
+@pytest.mark.parametrize("case,uniform", [("traction_128x64", True), ("beam_cantilever", True),
+                                          ("plate_with_hole", False)])
+def test_fp16_off_diagonal_placements(case, uniform):
+    """The fp16 preconditioner's two placements of the off-diagonal blocks:
+    every slice as wide as the widest row (layout bit 3; the structured
+    cantilever, whose interior nodes all have six neighbours, and the beam,
+    whose widest rows pad 1.6 %) or one width per slice (the plate with a
+    hole, whose nine-neighbour rows would pad 13 %).  Both solve
+    K x = K x_true to the true-residual tolerance."""
+    import oracle
+    import nbots_amd as nb
+    from nbots_amd.fem import solve_opts
+    p = problem_from_golden(load_golden(case))
+    rp, col = oracle.pattern(p.N_nod, p.conn)
+    val, F, _ = oracle.assemble(p, rp, col)
+    oracle.set_bconditions(p, rp, col, val, F)
+    x_true = np.random.default_rng(5).standard_normal(len(F))
+    b = oracle.spmv(rp, col, val, x_true)
+    A = nb.Sparse.from_csr(rp, col, val, vars_per_node=2)
+    x, info = A.solve(b, mode=2, tol=1e-10, opts=solve_opts(true_residual=True))
+    assert info["status"] == 0, info
+    assert info["layout"] & 7 == 7 and bool(info["layout"] & 8) == uniform, info["layout"]
+    r = b - oracle.spmv(rp, col, val, x)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b)
+    A.destroy()
