@@ -45,7 +45,7 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
     nbx::SolveStats st;
     int status = 0;
     memset(info, 0, sizeof *info);
-    S.ws.replay = !mo.eager_launch;
+    S.ws.replay = mo.replay;
     NBX_TRY(nbx::mat_prepare_sym(S.K, S.s));
     info->layout = S.K.has_sym ? 1 : 0;
     if (mode == NBX_SOLVE_REFERENCE) {
@@ -181,7 +181,7 @@ static void opts_from(const nbx::MixedOpts& d, nbx_solve_opts_t* o) {
     o->reliable_delta = d.delta;
     o->precond_bits = d.precond_bits;
     o->true_residual = d.true_residual ? 1 : 0;
-    o->eager_launch = d.eager_launch ? 1 : 0;
+    o->replay = d.replay ? 1 : 0;
     o->reserved = 0;
 }
 

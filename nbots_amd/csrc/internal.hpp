@@ -87,9 +87,8 @@ struct Workspace {
     static constexpr int SAMPLE_EVERY = 64;   // SAMPLE_EVERY iterations (PCG)
     hipEvent_t sev[2 * NSAMP] = {};
     // Replay a solve's repeated steps as captured HIP graphs (one host call
-    // per step instead of one per kernel; same kernels, same results).  Off
-    // for launches that must not allocate.
-    bool replay = true;
+    // per step instead of one per kernel; same kernels, same results).
+    bool replay = false;
 };
 
 struct SolveStats {
@@ -119,7 +118,7 @@ struct MixedOpts {
     int precond_bits = 16;     // Chebyshev preconditioner: fp16 or FP32 scaled values (FP64
                                // sums, FP32 vectors), or 64: the FP64 matrix, FP64 vectors
     bool true_residual = false; // stop on the true residual (restarts), FP64 and CHEB_FCG
-    bool eager_launch = false;  // launch kernel by kernel instead of replaying captured steps
+    bool replay = false;        // replay captured steps as HIP graphs (else kernel by kernel)
 };
 
 int ws_alloc(Workspace& w, uint32_t n);

@@ -41,7 +41,7 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& 
     if (p->reliable_delta > 0.0) o->delta = p->reliable_delta;
     if (p->precond_bits) o->precond_bits = p->precond_bits;
     o->true_residual = p->true_residual != 0;
-    o->eager_launch = p->eager_launch != 0;
+    o->replay = p->replay != 0;
     return NBX_OK;
 }
 

@@ -932,6 +932,22 @@ static bool stagnated(int pass, double res0, double prev_res0) {
     return pass > 0 && res0 >= 0.9 * prev_res0;
 }
 
+// The relative-tolerance solves refuse a right-hand side with a non-finite
+// entry, and report a non-finite final residual (a non-finite matrix value
+// or initial guess, or an iteration that broke down) as a breakdown rather
+// than as convergence: their stop tests compare with `>`, which a NaN fails.
+static int finite_rhs(double bb) {
+    if (std::isfinite(bb)) return NBX_OK;
+    set_last_error("the right-hand side has a non-finite entry (||b||^2 = %g)", bb);
+    return NBX_ERR_BREAKDOWN;
+}
+static int finite_result(double rel_res) {
+    if (std::isfinite(rel_res)) return NBX_OK;
+    set_last_error("non-finite residual (%g): a matrix value or the initial guess is not "
+                   "finite, or the iteration broke down", rel_res);
+    return NBX_ERR_BREAKDOWN;
+}
+
 static void add_pass(SolveStats& tot, const SolveStats& p, int pass) {
     tot.ms += p.ms;
     tot.iters += p.iters;
@@ -949,6 +965,7 @@ int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol, bool
     if (rc) return rc;
     const double bb = device_norm2(m.n, b, w, s, &rc);
     if (rc) return rc;
+    NBX_TRY(finite_rhs(bb));
     SolveStats tot{};
     int status = NBX_NOT_CONVERGED;
     uint32_t used = 0;
@@ -975,6 +992,7 @@ int solve_fp64(const DevMatrix& m, const double* b, double* x, double rtol, bool
         prev = ps.res0;
     }
     if (stats) *stats = tot;
+    NBX_TRY(finite_result(tot.rel_res));
     return status;
 }
 
@@ -985,6 +1003,7 @@ static int solve_mixed_cg(const DevMatrix& m, const double* b, double* x, double
     const uint32_t nu = m.s.n_nodes;
     const double bb = device_norm2(m.n, b, w, s, &rc);
     if (rc) return rc;
+    NBX_TRY(finite_rhs(bb));
     PcgState h{};
     h.tol2 = rtol * rtol * bb;
     h.max_iter = max_iter;
@@ -1086,6 +1105,7 @@ static int solve_mixed_cg(const DevMatrix& m, const double* b, double* x, double
         stats->ms = ms;
         stats->fallback = stalled;
     }
+    NBX_TRY(finite_result(rel));
     return status;
 }
 
@@ -1282,6 +1302,7 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
     int rc = NBX_OK;
     const double bn = sqrt(device_norm2(m.n, b, w, s, &rc));
     if (rc) return rc;
+    NBX_TRY(finite_rhs(bn * bn));
     SolveStats tot{};
     int status = NBX_NOT_CONVERGED;
     uint32_t used = 0;
@@ -1305,6 +1326,7 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
         prev = ps.res0;
     }
     if (stats) *stats = tot;
+    NBX_TRY(finite_result(tot.rel_res));
     return status;
 }
 

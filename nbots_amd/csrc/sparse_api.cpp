@@ -671,7 +671,7 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
     Mirror* m = sync_device(A, &rc);
     if (rc) return rc;
     NBX_TRY(nbx::mat_prepare_sym(m->dev, m->stream));
-    m->ws.replay = !mo.eager_launch;
+    m->ws.replay = mo.replay;
     info->layout = m->dev.has_sym ? 1 : 0;
     double* db = m->ws.v64[3];
     double* dx = m->ws.v64[4];

@@ -14,7 +14,7 @@ MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
 
 def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,
                reliable_delta=None, precond_bits=None, simp=False, true_residual=None,
-               eager_launch=None):
+               replay=None):
     """nbx_solve_opts_t with the library's defaults (the SIMP loop's with
     simp=True), then the given fields."""
     o = SolveOpts()
@@ -35,8 +35,8 @@ def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_
         o.precond_bits = precond_bits
     if true_residual is not None:
         o.true_residual = int(bool(true_residual))
-    if eager_launch is not None:
-        o.eager_launch = int(bool(eager_launch))
+    if replay is not None:
+        o.replay = int(bool(replay))
     return o
 
 

@@ -58,7 +58,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("mixed_variant", C.c_int32), ("cheb_degree", C.c_uint32),
                 ("cheb_ratio", C.c_double), ("inner_rtol", C.c_double),
                 ("reliable_delta", C.c_double), ("precond_bits", C.c_int32),
-                ("true_residual", C.c_int32), ("eager_launch", C.c_int32),
+                ("true_residual", C.c_int32), ("replay", C.c_int32),
                 ("reserved", C.c_int32)]
 
 

@@ -34,7 +34,7 @@ def test_replay_matches_eager_launches(mode, kw):
     for eager in (True, False):
         print(f"[replay] {mode} eager={eager} 96x48 cantilever", flush=True)
         f = Fem.from_problem(p)
-        info = f.solve(m, tol=tol, max_iter=400000, opts=solve_opts(eager_launch=eager, **kw))
+        info = f.solve(m, tol=tol, max_iter=400000, opts=solve_opts(replay=not eager, **kw))
         res[eager] = (f.displacement(), info)
         f.destroy()
     (ue, ie), (ur, ir) = res[True], res[False]
@@ -57,7 +57,7 @@ def test_replay_simp_history_matches_eager():
             print(f"[replay] SIMP mode={mode} eager={eager} 96x48 MBB", flush=True)
             f = Fem.from_problem(p)
             s = Simp(f, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2)
-            o = solve_opts(simp=True, true_residual=True, eager_launch=eager, **extra)
+            o = solve_opts(simp=True, true_residual=True, replay=not eager, **extra)
             h = []
             for _ in range(4):
                 C, change, info = s.iterate(mode, tol=1e-8, opts=o)

@@ -226,3 +226,25 @@ def test_damage_solver_gauss_seidel_fallback_log():
         want = f.read()
     assert r["log"] == want
     assert want.count("Gauss-Seidel fails") == 4 and r["damage"].max() == 0.0
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="compiled reference not built")
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_pcg_non_finite_rhs_matches_reference(bad):
+    """A NaN or Inf in b (cg_precond_jacobi.c:44, :86-89): the compiled
+    reference leaves its loop when the squared residual stops comparing
+    greater than tol^2 (NaN at once; Inf after two iterations, when the lagged
+    sum turns NaN) and returns 0 with a non-finite tolerance_reached; the
+    restatement gives the same status, count and x."""
+    q = problem_from_golden(load_golden("traction_16x8"))
+    rp, col = oracle.pattern(q.N_nod, q.conn)
+    val, b, _ = oracle.assemble(q, rp, col)
+    oracle.set_bconditions(q, rp, col, val, b)
+    b[11] = bad
+    x, it, tr, st = oracle.pcg(rp, col, val, b, max_iter=500, tol=1e-8)
+    xr, itr, trr, str_ = oracle.ref_pcg(rp, col, val, b, max_iter=500, tol=1e-8)
+    assert (st, it) == (str_, itr) and st == 0
+    assert it == (0 if math.isnan(bad) else 2)
+    assert not math.isfinite(tr) and not math.isfinite(trr)
+    assert np.array_equal(np.isnan(x), np.isnan(xr))
+    assert np.array_equal(x[~np.isnan(x)], xr[~np.isnan(xr)])
