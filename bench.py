@@ -335,13 +335,13 @@ def pmc_probe(args):
 
 # ---- GPU solvers --------------------------------------------------------------------
 
-EAGER_LAUNCH = False   # --eager-launch: kernel by kernel, no captured-graph replay
+REPLAY = False   # --replay: the solvers' repeated steps as captured HIP graphs
 
 
 def _opts(*a, **kw):
     """nbx_solve_opts_t for every solve of this run (see nbots_amd.fem.solve_opts)."""
     from nbots_amd.fem import solve_opts
-    return solve_opts(*a, eager_launch=EAGER_LAUNCH, **kw)
+    return solve_opts(*a, replay=REPLAY, **kw)
 
 
 def time_solver(f, mode, opts, reps, label, max_iter=200000):
@@ -531,14 +531,14 @@ def main():
     ap.add_argument("--simp-iters", type=int, default=50)
     ap.add_argument("--simp-fp32ir-iters", type=int, default=10)
     ap.add_argument("--no-simp", action="store_true")
-    ap.add_argument("--eager-launch", action="store_true",
-                    help="launch the solvers' kernels one by one instead of replaying captured "
-                         "HIP graphs of their repeated steps (same results)")
+    ap.add_argument("--replay", action="store_true",
+                    help="replay the solvers' repeated steps as captured HIP graphs instead of "
+                         "launching kernel by kernel (same results)")
     ap.add_argument("--simp-only", action="store_true",
                     help="only the SIMP loop, its results as one JSON line (for profiling)")
     args = ap.parse_args()
-    global EAGER_LAUNCH
-    EAGER_LAUNCH = args.eager_launch
+    global REPLAY
+    REPLAY = args.replay
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only (asked for --gpus {args.gpus}); "
               "multi-GPU runs are not part of this build", file=sys.stderr)

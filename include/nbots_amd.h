@@ -311,14 +311,16 @@ typedef struct {
 				    true residual (restarts, above); 0 on the
 				    recursively updated one, as the reference
 				    does (default)                            */
-	int32_t eager_launch;    /* 0 (default): the FP64 PCG and the
-				    Chebyshev flexible CG capture their
-				    repeated steps (four PCG iterations, one
-				    outer iteration) once per solve as HIP
-				    graphs and replay them, one host call per
-				    step; 1: launch kernel by kernel.  The
-				    same kernels in the same order: the same
-				    results either way.                       */
+	int32_t replay;          /* 1: the FP64 PCG and the Chebyshev
+				    flexible CG capture their repeated steps
+				    (four PCG iterations, one outer
+				    iteration) once per solve as HIP graphs
+				    and replay them, one host call per step;
+				    0 (default): launch kernel by kernel.
+				    The same kernels in the same order: the
+				    same results either way.  On MI355X the
+				    host keeps ahead of these kernels, and
+				    replay runs up to 1.6 % slower.           */
 	int32_t reserved;        /* 0                                         */
 } nbx_solve_opts_t;
 /* Fills *opts with the defaults above. */
