@@ -96,14 +96,14 @@ def sym_spmv_bytes(n_nodes: int, n_valid: int) -> int:
 
 
 def cheb_step_bytes(n_nodes: int, n_valid: int, adds_z: bool) -> int:
-    """One Chebyshev step on the fp16 copy without diagonal blocks: per
-    off-diagonal block 8 B of values and a 2-B packed column; per node the
-    diagonal pair (4 B), the row length (1 B), the d gather (8 B), the
-    residual read and write (16 B), the new d (8 B) and, on the steps that
-    add to z, its read and write (16 B); per 64-node slice two offsets."""
+    """One Chebyshev step on the fp16 copy without diagonal blocks, every
+    slice as wide as the widest row (padding not counted): per off-diagonal
+    block 8 B of values and a 2-B packed column; per node the diagonal pair
+    (4 B), the d gather (8 B), the residual read and write (16 B), the new d
+    (8 B) and, on the steps that add to z, its read and write (16 B)."""
     off = n_valid - n_nodes
-    per_node = 4 + 1 + 8 + 16 + 8 + (16 if adds_z else 0)
-    return off * 10 + n_nodes * per_node + 8 * ((n_nodes + 63) // 64)
+    per_node = 4 + 8 + 16 + 8 + (16 if adds_z else 0)
+    return off * 10 + n_nodes * per_node
 
 
 def pcg_iter_bytes(n_nodes, n_blocks, prec, sym=False):
