@@ -1,0 +1,69 @@
+// Smoothed-aggregation algebraic multigrid for the 2-dof T3 systems: the
+// preconditioner of the NBX_MIXED_AMG_FCG solve (an FP64 flexible CG whose
+// preconditioner is one FP32 V-cycle).
+//
+// Method (Vanek, Mandel & Brezina 1996, smoothed aggregation; the near
+// null space of plane elasticity is the three rigid-body modes):
+//   * nodes are aggregated greedily on the matrix graph (a root with its
+//     whole free neighbourhood, then the rest joins a neighbouring
+//     aggregate); a node whose every unknown is eliminated by a Dirichlet
+//     condition has no coarse part;
+//   * the tentative prolongator T maps an aggregate's three coarse unknowns
+//     to the orthonormalised rigid-body modes on its nodes (QR per aggregate;
+//     the R factors are the next level's modes);
+//   * P = (I - omega D^-1 A) T with omega = 4 / (3 lambda_max(D^-1 A));
+//   * A_c = P^T A P, recursively, until at most AMG_COARSEST unknowns remain,
+//     whose matrix is inverted (dense, FP64) and applied as a dense product;
+//   * one V-cycle: Chebyshev smoothing in D^-1 A of degree AMG_DEGREE on
+//     [lambda/30, 1.1 lambda] before and after the coarse correction.
+// Everything is scalar CSR.  The aggregation, T and every pattern depend on
+// the matrix's graph, its Dirichlet rows and the node coordinates only, so
+// they are built once on the host (amg_setup.cpp); P, A_c and the
+// coarsest inverse follow the values on the device whenever K changes.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace nbx {
+
+constexpr uint32_t AMG_NONE = 0xffffffffu;
+constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and after
+constexpr uint32_t AMG_COARSEST = 200; // unknowns of the directly solved level
+constexpr int AMG_MAX_LEVELS = 12;
+
+// One level of the host-built symbolic hierarchy.
+struct AmgHostLevel {
+    uint32_t n = 0;                   // scalar unknowns
+    uint32_t bs = 0;                  // unknowns per node (2 on the finest level, 3 below)
+    std::vector<uint32_t> a_ptr, a_col;   // this level's operator pattern
+    // to the next coarser level (empty on the coarsest)
+    uint32_t nc = 0;                  // coarse unknowns (3 per aggregate)
+    std::vector<double> t;            // [3 n] T's values of row i: columns agg(i)*3 + m
+    std::vector<uint32_t> self;       // [n] P row i's position of column agg(i)*3, or NONE
+    std::vector<uint32_t> p_ptr, p_col;   // P pattern (n x nc)
+    std::vector<uint32_t> ap_map;     // [nnz(A)] P row i's position of agg(col)*3, or NONE
+    std::vector<uint32_t> w_ptr, w_col;   // W = A P pattern (n x nc)
+    std::vector<uint32_t> pt_ptr, pt_row, pt_pos;   // P^T: coarse column c -> (row, position)
+};
+
+struct AmgHost {
+    std::vector<AmgHostLevel> lv;     // lv[0] finest; lv.back() is the coarsest
+};
+
+// Build the symbolic hierarchy from the finest operator: CSR pattern
+// (rowptr[n+1], col ascending per row) with its values (to find the
+// Dirichlet-eliminated unknowns: rows whose off-diagonal values are all 0)
+// and the coordinates of the n/2 nodes (x, y interleaved).  Returns 0, or
+// -1 when the pattern is not a 2-dof node pattern.
+int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+                   const double* xy, AmgHost& out);
+
+// In-place inverse of a dense symmetric positive semi-definite n x n matrix
+// (row-major, FP64) by Cholesky with a pivot guard: an unknown whose pivot
+// falls below 1e-13 times the largest diagonal entry is dropped (its row and
+// column of the inverse are 0).  Returns the number of dropped unknowns.
+int amg_dense_inverse(uint32_t n, double* a);
+
+}  // namespace nbx

@@ -1,0 +1,383 @@
+// Host side of the smoothed-aggregation hierarchy: aggregation, tentative
+// prolongators and every pattern and index map the device numeric phase
+// uses (amg.hpp).  Plain C++: no HIP, so it also builds and runs under the
+// host sanitizers.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <iterator>
+
+#include "amg.hpp"
+
+namespace nbx {
+
+namespace {
+
+struct Graph {
+    std::vector<uint32_t> ptr, adj;
+};
+
+// Node graph of a level: node I's neighbours are the nodes J != I that hold
+// a column of one of I's rows.  An unknown marked dead (Dirichlet-eliminated
+// on the finest level, a rank-deficient coarse mode below) has no edges, and
+// on the finest level an edge needs a nonzero value (the elimination zeroes
+// the entries it decouples).
+Graph node_graph(const AmgHostLevel& L, const std::vector<uint8_t>& dead, const double* val) {
+    const uint32_t bs = L.bs, nodes = L.n / bs;
+    Graph g;
+    g.ptr.assign(nodes + 1, 0);
+    std::vector<uint32_t> tmp;
+    std::vector<std::vector<uint32_t>> rows(nodes);
+    for (uint32_t I = 0; I < nodes; ++I) {
+        tmp.clear();
+        for (uint32_t i = I * bs; i < I * bs + bs; ++i) {
+            if (dead[i]) continue;
+            for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
+                const uint32_t j = L.a_col[e], J = j / bs;
+                if (J == I || dead[j]) continue;
+                if (val && val[e] == 0.0) continue;
+                tmp.push_back(J);
+            }
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        rows[I] = tmp;
+        g.ptr[I + 1] = g.ptr[I] + (uint32_t)tmp.size();
+    }
+    g.adj.resize(g.ptr[nodes]);
+    for (uint32_t I = 0; I < nodes; ++I)
+        std::copy(rows[I].begin(), rows[I].end(), g.adj.begin() + g.ptr[I]);
+    return g;
+}
+
+// Greedy aggregation: (1) a node whose neighbourhood is nonempty and free
+// becomes a root and takes it; (2) a remaining node joins the aggregate of
+// its first neighbour aggregated in (1); (3) what is left forms aggregates
+// with its free neighbours.  Nodes without neighbours stay unaggregated.
+uint32_t aggregate(const Graph& g, std::vector<uint32_t>& agg) {
+    const uint32_t nodes = (uint32_t)g.ptr.size() - 1;
+    agg.assign(nodes, AMG_NONE);
+    uint32_t na = 0;
+    for (uint32_t I = 0; I < nodes; ++I) {
+        const uint32_t b = g.ptr[I], e = g.ptr[I + 1];
+        if (agg[I] != AMG_NONE || b == e) continue;
+        bool free_ = true;
+        for (uint32_t k = b; k < e && free_; ++k) free_ = agg[g.adj[k]] == AMG_NONE;
+        if (!free_) continue;
+        agg[I] = na;
+        for (uint32_t k = b; k < e; ++k) agg[g.adj[k]] = na;
+        ++na;
+    }
+    const std::vector<uint32_t> first = agg;
+    for (uint32_t I = 0; I < nodes; ++I) {
+        if (agg[I] != AMG_NONE) continue;
+        for (uint32_t k = g.ptr[I]; k < g.ptr[I + 1]; ++k)
+            if (first[g.adj[k]] != AMG_NONE) {
+                agg[I] = first[g.adj[k]];
+                break;
+            }
+    }
+    for (uint32_t I = 0; I < nodes; ++I) {
+        if (agg[I] != AMG_NONE || g.ptr[I] == g.ptr[I + 1]) continue;
+        agg[I] = na;
+        for (uint32_t k = g.ptr[I]; k < g.ptr[I + 1]; ++k)
+            if (agg[g.adj[k]] == AMG_NONE) agg[g.adj[k]] = na;
+        ++na;
+    }
+    return na;
+}
+
+// Sorted union of the given sorted lists.
+void merge_into(std::vector<uint32_t>& acc, const uint32_t* b, const uint32_t* e) {
+    std::vector<uint32_t> out;
+    out.reserve(acc.size() + (size_t)(e - b));
+    std::set_union(acc.begin(), acc.end(), b, e, std::back_inserter(out));
+    acc.swap(out);
+}
+
+uint32_t find(const uint32_t* b, const uint32_t* e, uint32_t c) {
+    const uint32_t* p = std::lower_bound(b, e, c);
+    return (p != e && *p == c) ? (uint32_t)(p - b) : AMG_NONE;
+}
+
+// One coarsening step: fills L's prolongator fields and returns the coarse
+// level's pattern, modes B (3 per coarse unknown) and dead unknowns.
+void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<uint8_t>& dead,
+             const double* val, AmgHostLevel& C, std::vector<double>& Bc,
+             std::vector<uint8_t>& dead_c) {
+    const uint32_t bs = L.bs, n = L.n, nodes = n / bs;
+    const Graph g = node_graph(L, dead, val);
+    std::vector<uint32_t> agg;
+    const uint32_t na = aggregate(g, agg);
+    L.nc = 3 * na;
+    // members of each aggregate, ascending
+    std::vector<uint32_t> mptr(na + 1, 0), mem(nodes);
+    for (uint32_t I = 0; I < nodes; ++I)
+        if (agg[I] != AMG_NONE) ++mptr[agg[I] + 1];
+    for (uint32_t a = 0; a < na; ++a) mptr[a + 1] += mptr[a];
+    {
+        std::vector<uint32_t> fill(mptr.begin(), mptr.end() - 1);
+        for (uint32_t I = 0; I < nodes; ++I)
+            if (agg[I] != AMG_NONE) mem[fill[agg[I]]++] = I;
+    }
+    // tentative prolongator: modified Gram-Schmidt (twice) on each
+    // aggregate's rows of B; a column that vanishes is a dead coarse unknown
+    L.t.assign(3 * (size_t)n, 0.0);
+    Bc.assign(3 * (size_t)L.nc, 0.0);
+    dead_c.assign(L.nc, 0);
+    std::vector<double> q;
+    for (uint32_t a = 0; a < na; ++a) {
+        const uint32_t m = (mptr[a + 1] - mptr[a]) * bs;
+        q.assign(3 * (size_t)m, 0.0);   // q[c * m + r]
+        for (uint32_t k = 0; k < mptr[a + 1] - mptr[a]; ++k)
+            for (uint32_t d = 0; d < bs; ++d) {
+                const uint32_t i = mem[mptr[a] + k] * bs + d;
+                for (int c = 0; c < 3; ++c) q[c * (size_t)m + k * bs + d] = dead[i] ? 0.0 : B[3 * (size_t)i + c];
+            }
+        double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int c = 0; c < 3; ++c) {
+            double* v = &q[c * (size_t)m];
+            double n0 = 0.0;
+            for (uint32_t r = 0; r < m; ++r) n0 += v[r] * v[r];
+            n0 = sqrt(n0);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < c; ++k) {
+                    const double* u = &q[k * (size_t)m];
+                    double s = 0.0;
+                    for (uint32_t r = 0; r < m; ++r) s += u[r] * v[r];
+                    for (uint32_t r = 0; r < m; ++r) v[r] -= s * u[r];
+                    R[k][c] += s;
+                }
+            double nv = 0.0;
+            for (uint32_t r = 0; r < m; ++r) nv += v[r] * v[r];
+            nv = sqrt(nv);
+            if (n0 > 0.0 && nv > 1e-10 * n0) {
+                for (uint32_t r = 0; r < m; ++r) v[r] /= nv;
+                R[c][c] = nv;
+            } else {
+                for (uint32_t r = 0; r < m; ++r) v[r] = 0.0;
+                dead_c[3 * a + c] = 1;
+            }
+        }
+        for (uint32_t k = 0; k < mptr[a + 1] - mptr[a]; ++k)
+            for (uint32_t d = 0; d < bs; ++d) {
+                const uint32_t i = mem[mptr[a] + k] * bs + d;
+                for (int c = 0; c < 3; ++c) L.t[3 * (size_t)i + c] = q[c * (size_t)m + k * bs + d];
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) Bc[3 * (size_t)(3 * a + r) + c] = dead_c[3 * a + r] ? 0.0 : R[r][c];
+    }
+    // P pattern per node (shared by its rows): the aggregates of the nodes
+    // its rows reach, three columns each
+    std::vector<uint32_t> pnode_ptr(nodes + 1, 0), pnode;
+    {
+        std::vector<uint32_t> tmp;
+        for (uint32_t I = 0; I < nodes; ++I) {
+            tmp.clear();
+            for (uint32_t i = I * bs; i < I * bs + bs; ++i)
+                for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
+                    const uint32_t A = agg[L.a_col[e] / bs];
+                    if (A != AMG_NONE) tmp.push_back(A);
+                }
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            for (uint32_t A : tmp)
+                for (uint32_t c = 0; c < 3; ++c) pnode.push_back(3 * A + c);
+            pnode_ptr[I + 1] = (uint32_t)pnode.size();
+        }
+    }
+    L.p_ptr.assign(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        L.p_ptr[i + 1] = L.p_ptr[i] + (pnode_ptr[i / bs + 1] - pnode_ptr[i / bs]);
+    L.p_col.resize(L.p_ptr[n]);
+    L.self.assign(n, AMG_NONE);
+    L.ap_map.assign(L.a_ptr[n], AMG_NONE);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t I = i / bs;
+        const uint32_t* b = &pnode[pnode_ptr[I]];
+        const uint32_t* e = b + (pnode_ptr[I + 1] - pnode_ptr[I]);
+        std::copy(b, e, L.p_col.begin() + L.p_ptr[i]);
+        if (agg[I] != AMG_NONE) L.self[i] = L.p_ptr[i] + find(b, e, 3 * agg[I]);
+        for (uint32_t k = L.a_ptr[i]; k < L.a_ptr[i + 1]; ++k) {
+            const uint32_t A = agg[L.a_col[k] / bs];
+            if (A != AMG_NONE) L.ap_map[k] = L.p_ptr[i] + find(b, e, 3 * A);
+        }
+    }
+    // W = A P pattern per node: the union of the P rows its rows reach
+    std::vector<uint32_t> wnode_ptr(nodes + 1, 0), wnode;
+    {
+        std::vector<uint32_t> acc, nb;
+        for (uint32_t I = 0; I < nodes; ++I) {
+            nb.clear();
+            for (uint32_t i = I * bs; i < I * bs + bs; ++i)
+                for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) nb.push_back(L.a_col[e] / bs);
+            std::sort(nb.begin(), nb.end());
+            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            acc.clear();
+            for (uint32_t J : nb)
+                merge_into(acc, pnode.data() + pnode_ptr[J], pnode.data() + pnode_ptr[J + 1]);
+            wnode.insert(wnode.end(), acc.begin(), acc.end());
+            wnode_ptr[I + 1] = (uint32_t)wnode.size();
+        }
+    }
+    L.w_ptr.assign(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        L.w_ptr[i + 1] = L.w_ptr[i] + (wnode_ptr[i / bs + 1] - wnode_ptr[i / bs]);
+    L.w_col.resize(L.w_ptr[n]);
+    for (uint32_t i = 0; i < n; ++i)
+        std::copy(wnode.begin() + wnode_ptr[i / bs], wnode.begin() + wnode_ptr[i / bs + 1],
+                  L.w_col.begin() + L.w_ptr[i]);
+    // P^T lists: coarse column c -> the rows i holding it and its position
+    L.pt_ptr.assign(L.nc + 1, 0);
+    for (uint32_t p = 0; p < L.p_ptr[n]; ++p) ++L.pt_ptr[L.p_col[p] + 1];
+    for (uint32_t c = 0; c < L.nc; ++c) L.pt_ptr[c + 1] += L.pt_ptr[c];
+    L.pt_row.resize(L.p_ptr[n]);
+    L.pt_pos.resize(L.p_ptr[n]);
+    {
+        std::vector<uint32_t> fill(L.pt_ptr.begin(), L.pt_ptr.end() - 1);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t p = L.p_ptr[i]; p < L.p_ptr[i + 1]; ++p) {
+                const uint32_t c = L.p_col[p];
+                L.pt_row[fill[c]] = i;
+                L.pt_pos[fill[c]++] = p;
+            }
+    }
+    // coarse pattern per aggregate: the union of the W rows of the nodes
+    // whose P rows hold it
+    C.n = L.nc;
+    C.bs = 3;
+    C.a_ptr.assign(C.n + 1, 0);
+    std::vector<uint32_t> cnode_ptr(na + 1, 0), cnode;
+    {
+        std::vector<uint32_t> acc, last(nodes, AMG_NONE);
+        for (uint32_t a = 0; a < na; ++a) {
+            acc.clear();
+            const uint32_t c = 3 * a;
+            for (uint32_t k = L.pt_ptr[c]; k < L.pt_ptr[c + 1]; ++k) {
+                const uint32_t I = L.pt_row[k] / bs;
+                if (last[I] == a) continue;   // the node's rows share one W pattern
+                last[I] = a;
+                merge_into(acc, wnode.data() + wnode_ptr[I], wnode.data() + wnode_ptr[I + 1]);
+            }
+            cnode.insert(cnode.end(), acc.begin(), acc.end());
+            cnode_ptr[a + 1] = (uint32_t)cnode.size();
+        }
+    }
+    for (uint32_t c = 0; c < C.n; ++c)
+        C.a_ptr[c + 1] = C.a_ptr[c] + (cnode_ptr[c / 3 + 1] - cnode_ptr[c / 3]);
+    C.a_col.resize(C.a_ptr[C.n]);
+    for (uint32_t c = 0; c < C.n; ++c)
+        std::copy(cnode.begin() + cnode_ptr[c / 3], cnode.begin() + cnode_ptr[c / 3 + 1],
+                  C.a_col.begin() + C.a_ptr[c]);
+}
+
+}  // namespace
+
+int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+                   const double* xy, AmgHost& out) {
+    out.lv.clear();
+    if (n == 0 || n % 2) return -1;
+    AmgHostLevel L0;
+    L0.n = n;
+    L0.bs = 2;
+    L0.a_ptr.assign(rowptr, rowptr + n + 1);
+    L0.a_col.assign(col, col + rowptr[n]);
+    // Dirichlet-eliminated unknowns: every off-diagonal value of the row is 0
+    std::vector<uint8_t> dead(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        bool any = false;
+        for (uint32_t e = rowptr[i]; e < rowptr[i + 1] && !any; ++e)
+            any = col[e] != i && val[e] != 0.0;
+        dead[i] = !any;
+    }
+    // rigid-body modes, centred on the mesh's centroid
+    const uint32_t nodes = n / 2;
+    double cx = 0.0, cy = 0.0;
+    for (uint32_t I = 0; I < nodes; ++I) {
+        cx += xy[2 * I];
+        cy += xy[2 * I + 1];
+    }
+    cx /= nodes;
+    cy /= nodes;
+    std::vector<double> B(3 * (size_t)n, 0.0);
+    for (uint32_t I = 0; I < nodes; ++I) {
+        const double x = xy[2 * I] - cx, y = xy[2 * I + 1] - cy;
+        double* bx = &B[3 * (size_t)(2 * I)];
+        double* by = &B[3 * (size_t)(2 * I + 1)];
+        bx[0] = 1.0;
+        bx[1] = 0.0;
+        bx[2] = -y;
+        by[0] = 0.0;
+        by[1] = 1.0;
+        by[2] = x;
+    }
+    out.lv.push_back(std::move(L0));
+    const double* v = val;
+    while (out.lv.back().n > AMG_COARSEST && (int)out.lv.size() < AMG_MAX_LEVELS) {
+        AmgHostLevel C;
+        std::vector<double> Bc;
+        std::vector<uint8_t> dead_c;
+        coarsen(out.lv.back(), B, dead, v, C, Bc, dead_c);
+        if (C.n == 0 || C.n >= out.lv.back().n) {
+            // nothing (or no reduction) to coarsen: this level is the coarsest
+            AmgHostLevel& L = out.lv.back();
+            L.nc = 0;
+            L.t.clear(); L.self.clear(); L.p_ptr.clear(); L.p_col.clear(); L.ap_map.clear();
+            L.w_ptr.clear(); L.w_col.clear(); L.pt_ptr.clear(); L.pt_row.clear(); L.pt_pos.clear();
+            break;
+        }
+        out.lv.push_back(std::move(C));
+        B.swap(Bc);
+        dead.swap(dead_c);
+        v = nullptr;   // below the finest level every structural entry counts
+    }
+    return 0;
+}
+
+int amg_dense_inverse(uint32_t n, double* a) {
+    // Cholesky, lower triangle in place
+    double dmax = 0.0;
+    for (uint32_t i = 0; i < n; ++i) dmax = fmax(dmax, a[(size_t)i * n + i]);
+    std::vector<uint8_t> drop(n, 0);
+    int dropped = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        double s = a[(size_t)k * n + k];
+        for (uint32_t p = 0; p < k; ++p) s -= a[(size_t)k * n + p] * a[(size_t)k * n + p];
+        if (!(s > 1e-13 * dmax)) {
+            drop[k] = 1;
+            ++dropped;
+            for (uint32_t i = k; i < n; ++i) a[(size_t)i * n + k] = 0.0;
+            continue;
+        }
+        const double lkk = sqrt(s);
+        a[(size_t)k * n + k] = lkk;
+        for (uint32_t i = k + 1; i < n; ++i) {
+            double t = a[(size_t)i * n + k];
+            for (uint32_t p = 0; p < k; ++p) t -= a[(size_t)i * n + p] * a[(size_t)k * n + p];
+            a[(size_t)i * n + k] = t / lkk;
+        }
+    }
+    // M = L^-1 (lower), rows of dropped unknowns 0
+    std::vector<double> M((size_t)n * n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (drop[i]) continue;
+        const double lii = a[(size_t)i * n + i];
+        for (uint32_t j = 0; j <= i; ++j) {
+            double t = (i == j) ? 1.0 : 0.0;
+            for (uint32_t p = j; p < i; ++p) t -= a[(size_t)i * n + p] * M[(size_t)p * n + j];
+            M[(size_t)i * n + j] = t / lii;
+        }
+    }
+    // A^-1 = M^T M
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j <= i; ++j) {
+            double t = 0.0;
+            for (uint32_t p = i; p < n; ++p) t += M[(size_t)p * n + i] * M[(size_t)p * n + j];
+            a[(size_t)i * n + j] = t;
+            a[(size_t)j * n + i] = t;
+        }
+    return dropped;
+}
+
+}  // namespace nbx
