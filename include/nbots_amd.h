@@ -292,9 +292,16 @@ enum {
 	NBX_MIXED_CG_RESTART = 0,   /* FP32 CG sweeps to inner_rtol inside FP64
 				       iterative refinement                   */
 	NBX_MIXED_CG_RELIABLE = 1,  /* FP32 CG, FP64 reliable updates          */
-	NBX_MIXED_CHEB_FCG = 2      /* FP64 flexible CG, Chebyshev-Jacobi
+	NBX_MIXED_CHEB_FCG = 2,     /* FP64 flexible CG, Chebyshev-Jacobi
 				       preconditioner in FP32 arithmetic on
 				       fp16 or FP32 matrix values (default)  */
+	NBX_MIXED_AMG_FCG = 3       /* FP64 flexible CG, one FP32 V-cycle of
+				       smoothed-aggregation multigrid (rigid-
+				       body modes from the node coordinates)
+				       as the preconditioner: nbx_fem_* systems
+				       only; the hierarchy is built once per
+				       pattern and Dirichlet plan, its values
+				       follow K                               */
 };
 typedef struct {
 	int32_t mixed_variant;   /* NBX_MIXED_*                               */
@@ -354,7 +361,8 @@ typedef struct {
 				 its block columns as packed 16-bit deltas
 				 (every |w - v| < 32768); bit 3: ... with
 				 every slice as wide as the widest row
-				 (rows placed by their index alone)         */
+				 (rows placed by their index alone); bit 4:
+				 the AMG preconditioner ran               */
 	double ms_spmv;       /* mean duration of sampled SpMV(+dot) kernels */
 } nbx_solve_info_t;
 

@@ -60,6 +60,17 @@ struct AmgHost {
 int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
                    const double* xy, AmgHost& out);
 
+// The power iteration's start vector: +-1 from a hash of the index, so that
+// every frequency is present (a smooth start underestimates lambda_max, and
+// a Chebyshev smoother whose interval ends below the spectrum diverges).
+inline double amg_sign(uint32_t i) {
+    uint32_t h = i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    return (h & 1u) ? 1.0 : -1.0;
+}
+
 // In-place inverse of a dense symmetric positive semi-definite n x n matrix
 // (row-major, FP64) by Cholesky with a pivot guard: an unknown whose pivot
 // falls below 1e-13 times the largest diagonal entry is dropped (its row and

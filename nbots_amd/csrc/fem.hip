@@ -786,6 +786,8 @@ int fem_apply_bc(FemSys& F, const BcPlan& plan) {
     }
     F.bc_applied = true;
     F.n_constrained = (uint32_t)plan.d_dof.size();
+    F.amg_symbolic_current = false;   // the eliminated unknowns may differ
+    F.K.amg_current = false;
     return mat_extract_diag(F.K, s);
 }
 
@@ -805,6 +807,7 @@ int fem_reapply_bc(FemSys& F) {
         NBX_HIP(hipGetLastError());
     }
     F.bc_applied = true;
+    F.K.amg_current = false;
     return mat_extract_diag(F.K, s);
 }
 

@@ -41,6 +41,7 @@ struct FemSys {
     ElemParams P;
     bool bc_applied = false;
     uint32_t n_constrained = 0;
+    bool amg_symbolic_current = false;  // K.amg's hierarchy matches the pattern and Dirichlet rows
     Workspace ws;
 };
 
@@ -68,5 +69,9 @@ int fem_damage_update(FemSys& F, double E, double nu, double Gf, double ft, cons
 int stress_from_strain(uint32_t ne, double E, double nu, const uint8_t* enabled_host,
                        const double* strain_host, double* stress_host);
 int fem_download_csr(const FemSys& F, uint32_t* rowptr, uint32_t* col, double* val);
+// The AMG preconditioner's hierarchy for the current K: built (host
+// symbolic phase) after a new pattern or Dirichlet plan, refreshed (device
+// numeric phase) after new values.
+int amg_prepare(FemSys& F);
 
 }  // namespace nbx

@@ -63,11 +63,14 @@ int fem_solve(nbx::FemSys& S, int mode, double tol, uint32_t max_iter, const nbx
         info->fp64_iters = st.fp64_iters;
         info->rel_res = st.rel_res;
     } else if (mode == NBX_SOLVE_MIXED) {
-        NBX_TRY(nbx::mat_prepare_mixed(S.K, S.s, mo));
+        if (mo.variant == nbx::AMG_FCG)
+            NBX_TRY(nbx::amg_prepare(S));
+        else
+            NBX_TRY(nbx::mat_prepare_mixed(S.K, S.s, mo));
         status = nbx::solve_mixed(S.K, S.F, S.u, tol, mo, max_iter,
                                   S.ws, S.s, &st);
         if (status > 1) return status;
-        info->layout |= st.layout & 14;   // the preconditioner's fp16 copy
+        info->layout |= st.layout & 30;   // the preconditioner's fp16 copy, or AMG
         info->iters = st.iters;
         info->outer = st.outer;
         info->fp64_iters = st.fp64_iters;

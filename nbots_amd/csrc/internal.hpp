@@ -8,6 +8,8 @@ namespace nbx {
 // Scalar state of a device-resident CG run; written only by the last
 // workgroup of a reduction kernel, read by the following kernels and polled
 // by the host.
+struct AmgDev;   // smoothed-aggregation hierarchy (amg.hip)
+
 struct PcgState {
     double rz;            // g.q (Jacobi-PCG) or r.r (scaled CG) of the current iterate
     double gg_cur;        // ||g_k||^2
@@ -39,6 +41,8 @@ struct DevMatrix {
     bool has_val32 = false;
     bool has_val16 = false;
     double lambda_max = 0.0;   // Gershgorin bound of the scaled matrix
+    AmgDev* amg = nullptr;     // the AMG preconditioner's hierarchy (FEM systems build it)
+    bool amg_current = false;  // ... and its numeric phase follows the current values
     // symmetric copy of the FP64 values (blocked matrices): its pattern is
     // built once per pattern; its values are refreshed, and the symmetry of
     // the values checked, when the solver first needs them after a change
@@ -108,7 +112,9 @@ struct SolveStats {
 //  CG_RELIABLE: FP32 CG with FP64 reliable updates (no restarts)
 //  CHEB_FCG: FP64 flexible CG preconditioned by a fixed-degree Chebyshev
 //            iteration on the FP32 Jacobi-scaled matrix
-enum MixedVariant { CG_RESTART = 0, CG_RELIABLE = 1, CHEB_FCG = 2 };
+//  AMG_FCG: FP64 flexible CG preconditioned by one FP32 smoothed-aggregation
+//           V-cycle (FEM systems: the hierarchy needs node coordinates)
+enum MixedVariant { CG_RESTART = 0, CG_RELIABLE = 1, CHEB_FCG = 2, AMG_FCG = 3 };
 struct MixedOpts {
     MixedVariant variant = CHEB_FCG;
     double inner_rtol = 1e-4;
@@ -122,6 +128,13 @@ struct MixedOpts {
 };
 
 int ws_alloc(Workspace& w, uint32_t n);
+
+void amg_free(AmgDev* a);
+// one V-cycle z = M^-1 r on the finest level (FP32 arrays of its length);
+// returns at once per kernel when st->done
+int amg_vcycle(AmgDev* a, const float* r, float* z, const PcgState* st, hipStream_t s);
+const double* amg_ones(const AmgDev* a);   // [n] 1.0: the flexible CG's unit scaling
+uint32_t amg_levels(const AmgDev* a);
 void ws_free(Workspace& w);
 void mat_free(DevMatrix& m);
 

@@ -453,6 +453,7 @@ static void off_free(DevMatrix& m) {
 void mat_free(DevMatrix& m) {
     sym_free(m);
     off_free(m);
+    amg_free(m.amg);
     void* ps[] = {m.s.slice_ptr, m.s.rowlen, m.s.bcol, m.s.val, m.s.val32, m.s.val16,
                   m.c.rowptr,    m.c.col,    m.c.val,  m.diag,  m.dis};
     for (void* p : ps)
@@ -587,6 +588,7 @@ int mat_extract_diag(DevMatrix& m, hipStream_t s) {
 }
 
 void mat_scaled_stale(DevMatrix& m) {
+    m.amg_current = false;
     m.has_scale = false;
     m.has_val32 = false;
     m.has_val16 = false;

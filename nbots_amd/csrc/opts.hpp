@@ -25,7 +25,7 @@ inline MixedOpts mixed_opts(double inner_rtol) {
 inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& base = MixedOpts()) {
     *o = base;
     if (!p) return NBX_OK;
-    if (p->mixed_variant < NBX_MIXED_CG_RESTART || p->mixed_variant > NBX_MIXED_CHEB_FCG) {
+    if (p->mixed_variant < NBX_MIXED_CG_RESTART || p->mixed_variant > NBX_MIXED_AMG_FCG) {
         set_last_error("unknown mixed-precision variant %d", (int)p->mixed_variant);
         return NBX_ERR_ARG;
     }

@@ -1174,7 +1174,26 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
                                     : 12.0 * (double)m.s.n_blocks;
     const bool cheb_cached =
         cheb_mat + (p64 ? 80.0 : 33.0) * (double)nu <= (double)INFINITY_CACHE_BYTES;
+    const bool amg = opt.variant == AMG_FCG;
+    const double2* ones = amg ? (const double2*)amg_ones(m.amg) : nullptr;
+    auto precondition_amg = [&](int first) {
+        // the update with a unit scaling writes r in FP32 (res), the V-cycle
+        // z = M^-1 r, the dot products and direction read z unscaled
+        hipLaunchKernelGGL(k_fcg_update<float2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv,
+                           ones, res, da, z, w.st, w.partials, nu, first, 1.0);
+        hipLaunchKernelGGL(k_fin_fcg_update, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        (void)amg_vcycle(m.amg, (const float*)res, (float*)z, w.st, s);
+        hipLaunchKernelGGL(k_fcg_dots<float2>, dim3(G), dim3(BLOCK), 0, s, z, ones, wv, r, w.st,
+                           w.partials, nu, first);
+        hipLaunchKernelGGL(k_fin_fcg_dots, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        hipLaunchKernelGGL(k_fcg_p<float2>, dim3(G), dim3(BLOCK), 0, s, z, ones, p, w.st, nu,
+                           first);
+    };
     auto precondition = [&](int first) {
+        if (amg) {
+            precondition_amg(first);
+            return;
+        }
         if (p64) {
             hipLaunchKernelGGL(k_fcg_update<double2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p,
                                wv, dis, res64, da64, z64, w.st, w.partials, nu, first, inv_theta);
@@ -1282,12 +1301,13 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         NBX_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
         stats->ms = ms;
         stats->iters = w.h_st->k;
-        stats->outer = w.h_st->k * (deg - 1);    // SpMVs in the preconditioner
+        // SpMVs in the preconditioner (Chebyshev), V-cycles (AMG)
+        stats->outer = amg ? w.h_st->k : w.h_st->k * (deg - 1);
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
-        stats->layout = (m.has_sym ? 1 : 0) | (off16 ? 2 : 0) | (packed ? 4 : 0) |
-                        (packed && m.o.width ? 8 : 0);
+        stats->layout = (m.has_sym ? 1 : 0) | (off16 && !amg ? 2 : 0) | (packed && !amg ? 4 : 0) |
+                        (packed && !amg && m.o.width ? 8 : 0) | (amg ? 16 : 0);
         stats->res0 = sqrt(w.h_st->gg0);
         const uint32_t ran = (w.h_st->k + every - 1) / every;
         NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));
@@ -1335,6 +1355,16 @@ int solve_mixed(const DevMatrix& m, const double* b, double* x, double rtol, con
     const bool cheb = opt.variant == CHEB_FCG;
     const bool fp16 = cheb && opt.precond_bits == 16;
     const bool fp64 = cheb && opt.precond_bits == 64;
+    if (opt.variant == AMG_FCG) {
+        if (!m.blocked || !m.amg || !m.amg_current) {
+            set_last_error("solve_mixed: no current AMG hierarchy (the AMG preconditioner is "
+                           "built for nbx_fem systems, whose node coordinates it needs)");
+            return NBX_ERR_ARG;
+        }
+        int rc = ws_alloc(w, m.n);
+        if (rc) return rc;
+        return solve_fcg_cheb(m, b, x, rtol, opt, max_iter, w, s, stats);
+    }
     if (!m.blocked || !m.has_scale ||
         !(fp64 || (fp16 ? m.has_off16 || m.has_val16 : m.has_val32))) {
         set_last_error("solve_mixed: matrix has no scaled %d-bit copy", fp16 ? 16 : 32);

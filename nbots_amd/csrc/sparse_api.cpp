@@ -700,7 +700,7 @@ static int sparse_solve(const nb_sparse_t* A, const double* b, double* x, int mo
         if (rc) return rc;
         status = nbx::solve_mixed(m->dev, db, dx, tol, mo, max_iter, m->ws, m->stream, &st);
         if (status > 1) return status;
-        info->layout |= st.layout & 14;   // the preconditioner's fp16 copy
+        info->layout |= st.layout & 30;   // the preconditioner's fp16 copy
         info->iters = st.iters;
         info->outer = st.outer;
         info->fp64_iters = st.fp64_iters;

@@ -9,7 +9,7 @@ import numpy as np
 from .lib import SolveInfo, SolveOpts, check, last_error, load
 
 SOLVE_REFERENCE, SOLVE_FP64, SOLVE_MIXED = 0, 1, 2
-MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG = 0, 1, 2
+MIXED_CG_RESTART, MIXED_CG_RELIABLE, MIXED_CHEB_FCG, MIXED_AMG_FCG = 0, 1, 2, 3
 
 
 def solve_opts(variant=MIXED_CHEB_FCG, cheb_degree=None, cheb_ratio=None, inner_rtol=None,

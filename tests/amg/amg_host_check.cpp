@@ -163,9 +163,9 @@ int main(int argc, char** argv) {
         for (uint32_t i = 0; i < L.n; ++i)
             for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e)
                 if (L.a_col[e] == i && M.a[e] > 0.0) M.dinv[i] = 1.0 / M.a[e];
-        // power iteration on D^-1 A from a fixed start
+        // power iteration on D^-1 A from a fixed +-1 start (every frequency)
         std::vector<double> x(L.n), y(L.n);
-        for (uint32_t i = 0; i < L.n; ++i) x[i] = 1.0 + 0.5 * sin(0.37 * i);
+        for (uint32_t i = 0; i < L.n; ++i) x[i] = amg_sign(i);
         double s = sqrt(dot(x, x));
         for (uint32_t i = 0; i < L.n; ++i) x[i] /= s;
         for (int it = 0; it < 15; ++it) {
