@@ -51,6 +51,8 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
 # 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md;
 # keyed by (config, symmetric layout)
+# iteration cap of the AMG-preconditioned solves (they take 25-40)
+AMG_MAX_ITER = 2000
 PMC_TRAFFIC = {("c3", False): 599187989, ("c3", True): 411256645}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
@@ -358,6 +360,34 @@ def time_solver(f, mode, opts, reps, label, max_iter=200000):
     return float(np.median(ts)), info
 
 
+def amg_solve(f, t_fp64, u64, Fl):
+    """FP64 flexible CG preconditioned by one FP32 smoothed-aggregation
+    V-cycle.  The first solve builds the hierarchy (host symbolic phase +
+    device numeric phase); the timed solves reuse it, as solves of one matrix
+    with several loads do.  A SIMP iteration pays the numeric phase again
+    (the moduli change), which the SIMP legs include."""
+    from nbots_amd.fem import SOLVE_MIXED, MIXED_AMG_FCG
+    o = _opts(MIXED_AMG_FCG, true_residual=True)
+    t0 = time.perf_counter()
+    first = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=AMG_MAX_ITER, opts=o)
+    t_first = time.perf_counter() - t0
+    log(f"[bench] AMG first solve (with setup): {first['iters']} it, {t_first:.3f}s, "
+        f"status {first['status']}")
+    t, info = time_solver(f, SOLVE_MIXED, o, 3, "AMG-preconditioned FCG", max_iter=AMG_MAX_ITER)
+    u = f.displacement()
+    return dict(
+        time_to_1e8_amg_s=round(t, 4),
+        amg=dict(seconds=round(t, 4), speedup_vs_fp64=round(t_fp64 / t, 3),
+                 iters=info["iters"], status=info["status"], rel_res=info["rel_res"],
+                 true_rel_res=f.true_residual(True),
+                 first_solve_s=round(t_first, 4), setup_s=round(t_first - t, 4),
+                 vs_fp64_disp_rel_l2=float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                 vs_fp64_compliance_rel=abs(float(Fl @ u) - float(Fl @ u64)) / abs(float(Fl @ u64)),
+                 solver=("FP64 flexible CG; preconditioner one FP32 V-cycle of smoothed-"
+                         "aggregation multigrid (rigid-body modes, Chebyshev-Jacobi smoothing "
+                         "of degree 3, dense coarsest level); setup = first solve - solve")))
+
+
 def decompose(f, args, t_fp64, iters64, u64, Fl):
     """The mixed-precision speedup split into the preconditioner's gain and
     the precision's, plus the FP32-CG + FP64-refinement scheme."""
@@ -383,6 +413,7 @@ def decompose(f, args, t_fp64, iters64, u64, Fl):
         mixed_true_rel_res=true16[0], mixed_true_rel_res_fp64_eval=true16[1],
         mixed_vs_fp64_disp_rel_l2=float(np.linalg.norm(u16 - u64) / np.linalg.norm(u64)),
         mixed_vs_fp64_compliance_rel=abs(float(Fl @ u16) - float(Fl @ u64)) / abs(float(Fl @ u64)))
+    out.update(amg_solve(f, t_fp64, u64, Fl))
     if args.no_decompose:
         return out
     o64 = _opts(MIXED_CHEB_FCG, cheb_degree=args.cheb_degree, cheb_ratio=args.cheb_ratio,
@@ -426,13 +457,15 @@ def simp_bench(args, opts16):
     --simp-fp32ir-iters iterations, FP32 CG in FP64 refinement."""
     from nbots_amd import synth
     from nbots_amd.fem import (Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG,
-                               MIXED_CG_RESTART)
+                               MIXED_CG_RESTART, MIXED_AMG_FCG)
     nx, ny = 1024, 512
     p = synth.mbb(nx, ny, rho=0.5)
     out = dict(simp_workload=f"{nx}x{ny} MBB half-beam, {p.N} DOF, {p.N_elem} T3, volfrac 0.5, "
                              "penal 3, filter radius 1.5, OC move 0.2, rho0 = 0.5",
                simp_iterations=args.simp_iters)
-    variants = [("mixed", SOLVE_MIXED, opts16, args.simp_iters),
+    variants = [("amg", SOLVE_MIXED, _opts(MIXED_AMG_FCG, simp=True, true_residual=True),
+                 args.simp_iters),
+                ("mixed", SOLVE_MIXED, opts16, args.simp_iters),
                 ("fp64", SOLVE_FP64, _opts(true_residual=True, simp=True), args.simp_iters)]
     if not args.no_decompose:
         variants.insert(1, ("fcg_fp64_precond", SOLVE_MIXED,
@@ -451,7 +484,8 @@ def simp_bench(args, opts16):
         hist, ch, its, solve_ms, it_s, fb = [], [], [], [], [], 0
         for it in range(n_it):
             t0 = time.perf_counter()
-            C, change, info = s.iterate(mode, tol=1e-8, opts=o)
+            C, change, info = s.iterate(mode, tol=1e-8, opts=o,
+                                        max_iter=AMG_MAX_ITER if name == "amg" else 0)
             it_s.append(time.perf_counter() - t0)
             hist.append(C)
             ch.append(change)
@@ -465,20 +499,27 @@ def simp_bench(args, opts16):
                           iters=its, setup_s=t_setup, change=ch, fallbacks=fb)
         s.destroy()
         f.destroy()
-    m, d = runs["mixed"], runs["fp64"]
+    a, m, d = runs["amg"], runs["mixed"], runs["fp64"]
     rel = np.abs(m["hist"] - d["hist"]) / np.abs(d["hist"])
+    rel_a = np.abs(a["hist"] - d["hist"]) / np.abs(d["hist"])
     ms = {k: 1e3 * float(v["it_s"].mean()) for k, v in runs.items()}
     out.update(
-        ms_per_simp_iter=round(ms["mixed"], 2),
+        ms_per_simp_iter=round(ms["amg"], 2),
+        simp_solver="FP64 flexible CG with one FP32 smoothed-aggregation V-cycle as preconditioner",
+        ms_per_simp_iter_cheb=round(ms["mixed"], 2),
         ms_per_simp_iter_fp64=round(ms["fp64"], 2),
-        simp_solve_ms_mean=round(m["solve_ms"], 2),
+        simp_solve_ms_mean=round(a["solve_ms"], 2),
+        simp_solve_ms_mean_cheb=round(m["solve_ms"], 2),
         simp_solve_ms_mean_fp64=round(d["solve_ms"], 2),
         simp_filter_setup_s=round(m["setup_s"], 3),
-        simp_solver_iters_mean=round(float(np.mean(m["iters"])), 1),
+        simp_solver_iters_mean=round(float(np.mean(a["iters"])), 1),
+        simp_solver_iters_mean_cheb=round(float(np.mean(m["iters"])), 1),
         simp_solver_iters_mean_fp64=round(float(np.mean(d["iters"])), 1),
+        simp_amg_first_iter_ms=round(1e3 * float(a["it_s"][0]), 2),
         simp_compliance_first_last=[float(d["hist"][0]), float(d["hist"][-1])],
-        simp_compliance_hist_max_rel_diff=float(rel.max()),
-        simp_change_last=float(m["change"][-1]))
+        simp_compliance_hist_max_rel_diff=float(rel_a.max()),
+        simp_compliance_hist_max_rel_diff_cheb=float(rel.max()),
+        simp_change_last=float(a["change"][-1]))
     if "fcg_fp64_precond" in runs:
         q = runs["fcg_fp64_precond"]
         out["simp_decomposition"] = dict(
