@@ -18,18 +18,21 @@ and, timed separately on the same GPU, time to 1e-8 of:
   * FP64 flexible CG with a degree-40 Chebyshev-Jacobi preconditioner whose
     matrix values, vectors and arithmetic are all FP64;
   * the same with the preconditioner's matrix in fp16 and its vectors in
-    FP32 (FP64 row sums): the library's default mixed-precision solver.
+    FP32 (FP64 row sums): the library's default mixed-precision solver;
+  * FP64 flexible CG preconditioned by one FP32 V-cycle of smoothed-
+    aggregation multigrid (the hierarchy's setup reported apart).
 The preconditioner factor (Jacobi-PCG / FP64-preconditioner FCG) and the
 precision factor (FP64-preconditioner FCG / fp16-preconditioner FCG) are
-reported apart; their product is the mixed solver's speedup.  A SIMP
-topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF,
-50 iterations) gives ms per SIMP iteration with each solver.
+reported apart; their product is the Chebyshev mixed solver's speedup.  A
+SIMP topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF,
+50 iterations) gives ms per SIMP iteration with each solver; the headline
+ms_per_simp_iter is the multigrid-preconditioned loop's, setup included.
 
     python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu]
                     [--simp-iters 50] [--no-simp] [--no-decompose]
     python bench.py --cpu-full        # the CPU baseline plan at full size
     python bench.py --pmc-probe [--probe-mixed]   # a short run for rocprofv3 --pmc
-    python bench.py --simp-only [--simp-iters N]  # the SIMP loop alone
+    python bench.py --simp-only [--simp-iters N] [--simp-amg-only]  # the SIMP loop alone
 """
 from __future__ import annotations
 
@@ -452,9 +455,11 @@ def simp_bench(args, opts16):
     """1024x512 MBB half-beam (1,051,650 DOF), volume fraction 0.5, penalty 3,
     density filter radius 1.5, OC move 0.2, from rho = 0.5: the SIMP loop
     with each solver (all to relative residual 1e-8, warm-started from the
-    previous displacement): the fp16-preconditioner flexible CG, the same
-    with an FP64 preconditioner, FP64 Jacobi-PCG, and, for the first
-    --simp-fp32ir-iters iterations, FP32 CG in FP64 refinement."""
+    previous displacement): the flexible CG preconditioned by one FP32
+    smoothed-aggregation V-cycle, the flexible CG with the fp16 Chebyshev
+    preconditioner, the same with an FP64 preconditioner, FP64 Jacobi-PCG,
+    and, for the first --simp-fp32ir-iters iterations, FP32 CG in FP64
+    refinement.  With --simp-amg-only, the multigrid loop alone."""
     from nbots_amd import synth
     from nbots_amd.fem import (Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG,
                                MIXED_CG_RESTART, MIXED_AMG_FCG)
@@ -467,7 +472,9 @@ def simp_bench(args, opts16):
                  args.simp_iters),
                 ("mixed", SOLVE_MIXED, opts16, args.simp_iters),
                 ("fp64", SOLVE_FP64, _opts(true_residual=True, simp=True), args.simp_iters)]
-    if not args.no_decompose:
+    if args.simp_amg_only:
+        variants = variants[:1]
+    elif not args.no_decompose:
         variants.insert(1, ("fcg_fp64_precond", SOLVE_MIXED,
                             _opts(MIXED_CHEB_FCG, precond_bits=64, simp=True,
                                        true_residual=True), args.simp_iters))
@@ -499,6 +506,14 @@ def simp_bench(args, opts16):
                           iters=its, setup_s=t_setup, change=ch, fallbacks=fb)
         s.destroy()
         f.destroy()
+    if args.simp_amg_only:
+        a = runs["amg"]
+        out.update(ms_per_simp_iter=round(1e3 * float(a["it_s"].mean()), 2),
+                   simp_solve_ms_mean=round(a["solve_ms"], 2),
+                   simp_solver_iters_mean=round(float(np.mean(a["iters"])), 1),
+                   simp_amg_first_iter_ms=round(1e3 * float(a["it_s"][0]), 2),
+                   simp_compliance_first_last=[float(a["hist"][0]), float(a["hist"][-1])])
+        return out
     a, m, d = runs["amg"], runs["mixed"], runs["fp64"]
     rel = np.abs(m["hist"] - d["hist"]) / np.abs(d["hist"])
     rel_a = np.abs(a["hist"] - d["hist"]) / np.abs(d["hist"])
@@ -575,6 +590,8 @@ def main():
     ap.add_argument("--replay", action="store_true",
                     help="replay the solvers' repeated steps as captured HIP graphs instead of "
                          "launching kernel by kernel (same results)")
+    ap.add_argument("--simp-amg-only", action="store_true",
+                    help="the SIMP loop with multigrid-preconditioned solves only")
     ap.add_argument("--simp-only", action="store_true",
                     help="only the SIMP loop, its results as one JSON line (for profiling)")
     args = ap.parse_args()

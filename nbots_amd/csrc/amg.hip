@@ -58,6 +58,9 @@ struct AmgDev {
     double* partials = nullptr;     // [MAX_GRID] reduction partials
     std::vector<double> host_inv;
     size_t bytes = 0;
+    // the finest level is smoothed on the matrix's fp16 scaled copy, in
+    // scaled variables; its P values are stored as D^1/2 P (FP32)
+    bool scaled16 = false;
 };
 
 namespace {
@@ -91,6 +94,17 @@ __global__ __launch_bounds__(BLOCK) void k_amg_to32(const double* __restrict__ i
                                                     float* __restrict__ out, size_t n) {
     const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e < n) out[e] = (float)in[e];
+}
+
+// the finest P for the scaled variables: row i times D_ii^1/2 = 1 / dis_i
+__global__ __launch_bounds__(BLOCK) void k_amg_p_scaled(const uint32_t* __restrict__ p_ptr,
+                                                        const double* __restrict__ p,
+                                                        const double* __restrict__ dis,
+                                                        float* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double sq = 1.0 / dis[i];
+    for (uint32_t q = p_ptr[i]; q < p_ptr[i + 1]; ++q) out[q] = (float)(p[q] * sq);
 }
 
 // D^-1 (0 where the diagonal is not positive: an unknown with no coarse or
@@ -379,8 +393,8 @@ __global__ __launch_bounds__(BLOCK) void k_amg_restrict(
     if (valid && lane == 0) bc[i] = s;
 }
 
-// x += P x_c
-template <int LANES>
+// x += P x_c (ADD), or x = P x_c
+template <bool ADD, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_amg_prolong(
     const uint32_t* __restrict__ p_ptr, const uint32_t* __restrict__ p_col,
     const float* __restrict__ p, const float* __restrict__ xc, float* __restrict__ x,
@@ -388,7 +402,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_prolong(
     if (st->done) return;
     AMG_ROW(LANES, n);
     const float px = csr_row<LANES>(p_ptr, p_col, p, xc, i, lane, valid);
-    if (valid && lane == 0) x[i] += px;
+    if (valid && lane == 0) x[i] = ADD ? x[i] + px : px;
 }
 
 // x = C^-1 b on the coarsest level (n <= AMG_COARSEST), one workgroup
@@ -563,8 +577,9 @@ static int amg_numeric(FemSys& S) {
         const unsigned g = grid_for(L.n);
         hipLaunchKernelGGL(k_amg_dinv, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
                            L.dinv64, L.dinv32, L.n);
-        hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.a_nnz)), dim3(BLOCK), 0, s, L.a64, L.a32,
-                           L.a_nnz);
+        if (!(l == 0 && A->scaled16))   // (that level's products read the fp16 copy)
+            hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.a_nnz)), dim3(BLOCK), 0, s, L.a64,
+                               L.a32, L.a_nnz);
         // lambda_max(D^-1 A): 15 power steps from a fixed start
         hipLaunchKernelGGL(k_amg_pow_init, dim3(g), dim3(BLOCK), 0, s, A->pow_x, L.n);
         hipLaunchKernelGGL(k_amg_sq, dim3(g), dim3(BLOCK), 0, s, A->pow_x, A->partials, L.n);
@@ -625,8 +640,12 @@ static int amg_numeric(FemSys& S) {
         const double omega = 4.0 / (3.0 * L.lmax);
         hipLaunchKernelGGL(k_amg_smooth_p, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
                            L.dinv64, L.t, L.self, L.p_ptr, L.ap_map, L.p64, omega, L.n);
-        hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.p_nnz)), dim3(BLOCK), 0, s, L.p64, L.p32,
-                           L.p_nnz);
+        if (l == 0 && A->scaled16)
+            hipLaunchKernelGGL(k_amg_p_scaled, dim3(g), dim3(BLOCK), 0, s, L.p_ptr, L.p64, S.K.dis,
+                               L.p32, L.n);
+        else
+            hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.p_nnz)), dim3(BLOCK), 0, s, L.p64,
+                               L.p32, L.p_nnz);
         hipLaunchKernelGGL(k_amg_w, dim3(grid_for((size_t)L.n * WAVE)), dim3(BLOCK), 0, s, L.a_ptr,
                            L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64, L.n);
         hipLaunchKernelGGL(k_amg_galerkin, dim3(grid_for((size_t)L.nc * WAVE)), dim3(BLOCK), 0, s,
@@ -643,24 +662,55 @@ int amg_prepare(FemSys& S) {
         S.amg_symbolic_current = true;
     }
     if (!S.K.amg_current) {
+        // the finest level's smoothing reads the fp16 scaled copy when the
+        // matrix allows one (values in fp16 range), its own FP32 CSR copy
+        // otherwise
+        AmgDev* A = S.K.amg;
+        A->scaled16 = false;
+        if (A->lv.size() > 1) {
+            const int rc = mat_make_off16(S.K, S.s);
+            if (rc != NBX_OK && rc != NBX_ERR_ARG) return rc;
+            A->scaled16 = rc == NBX_OK && S.K.has_off16 && S.K.has_scale;
+        }
         NBX_TRY(amg_numeric(S));
         S.K.amg_current = true;
     }
     return NBX_OK;
 }
 
+AmgFine amg_fine(const AmgDev* A) {
+    AmgFine f;
+    if (!A || !A->scaled16 || A->lv.size() < 2) return f;
+    const AmgDevLevel& L = A->lv[0];
+    f.scaled16 = true;
+    f.degree = AMG_DEGREE;
+    const double hi = 1.1 * L.lmax, lo = L.lmax / 30.0;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    f.inv_theta = 1.0 / theta;
+    for (int k = 1; k < AMG_DEGREE; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        f.c1[k] = rn * rho;
+        f.c2[k] = 2.0 * rn / delta;
+        rho = rn;
+    }
+    f.c = L.d0;
+    return f;
+}
+
 const double* amg_ones(const AmgDev* A) { return A->ones; }
 uint32_t amg_levels(const AmgDev* A) { return (uint32_t)A->lv.size(); }
 
 
-// z = V-cycle(r) on the finest level; b and x are FP32 arrays of the
-// finest level's length
-int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStream_t s) {
+// The V-cycle from level l0 down, x = M^-1 b on that level (b, x: that
+// level's arrays).
+static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const PcgState* st,
+                       hipStream_t s) {
     const size_t L = A->lv.size();
-    for (size_t l = 0; l + 1 < L; ++l) {
+    for (size_t l = l0; l + 1 < L; ++l) {
         AmgDevLevel& V = A->lv[l];
-        const float* bl = l == 0 ? b : V.b;
-        float* xl = l == 0 ? x : V.x;
+        const float* bl = l == l0 ? b : V.b;
+        float* xl = l == l0 ? x : V.x;
         hipLaunchKernelGGL((k_amg_sm_init<true, 1>), dim3(rows_grid(V.n, 1)), dim3(BLOCK), 0, s,
                            V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st,
                            V.n);
@@ -685,16 +735,16 @@ int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStrea
     {
         AmgDevLevel& C = A->lv[L - 1];
         hipLaunchKernelGGL(k_amg_dense, dim3(1), dim3(DENSE_BLOCK), 0, s, A->cinv,
-                           L == 1 ? b : C.b, L == 1 ? x : C.x, st, C.n);
+                           L == l0 + 1 ? b : C.b, L == l0 + 1 ? x : C.x, st, C.n);
     }
-    for (size_t l = L - 1; l-- > 0;) {
+    for (size_t l = L - 1; l-- > l0;) {
         AmgDevLevel& V = A->lv[l];
-        const float* bl = l == 0 ? b : V.b;
-        float* xl = l == 0 ? x : V.x;
+        const float* bl = l == l0 ? b : V.b;
+        float* xl = l == l0 ? x : V.x;
         with_lanes(V.lp, [&](auto ln) {
             constexpr int LN = decltype(ln)::value;
-            hipLaunchKernelGGL((k_amg_prolong<LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0, s,
-                               V.p_ptr, V.p_col, V.p32, A->lv[l + 1].x, xl, st, V.n);
+            hipLaunchKernelGGL((k_amg_prolong<true, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
+                               s, V.p_ptr, V.p_col, V.p32, A->lv[l + 1].x, xl, st, V.n);
         });
         with_lanes(V.la, [&](auto ln) {
             constexpr int LN = decltype(ln)::value;
@@ -715,6 +765,31 @@ int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStrea
             }
         });
     }
+}
+
+// z = V-cycle(r) on the finest level; b and x are FP32 arrays of the
+// finest level's length
+int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStream_t s) {
+    cycle_from(A, 0, b, x, st, s);
+    NBX_HIP(hipGetLastError());
+    return NBX_OK;
+}
+
+int amg_coarse_correction(AmgDev* A, const float* r, float* c, const PcgState* st,
+                          hipStream_t s) {
+    AmgDevLevel& V = A->lv[0];
+    AmgDevLevel& W = A->lv[1];
+    with_lanes(V.lr, [&](auto ln) {
+        constexpr int LN = decltype(ln)::value;
+        hipLaunchKernelGGL((k_amg_restrict<LN>), dim3(rows_grid(V.nc, LN)), dim3(BLOCK), 0, s,
+                           V.pt_ptr, V.pt_row, V.pt_pos, V.p32, r, W.b, st, V.nc);
+    });
+    cycle_from(A, 1, W.b, W.x, st, s);
+    with_lanes(V.lp, [&](auto ln) {
+        constexpr int LN = decltype(ln)::value;
+        hipLaunchKernelGGL((k_amg_prolong<false, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
+                           s, V.p_ptr, V.p_col, V.p32, W.x, c, st, V.n);
+    });
     NBX_HIP(hipGetLastError());
     return NBX_OK;
 }

@@ -32,6 +32,7 @@ constexpr uint32_t AMG_NONE = 0xffffffffu;
 constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and after
 constexpr uint32_t AMG_COARSEST = 200; // unknowns of the directly solved level
 constexpr int AMG_MAX_LEVELS = 12;
+static_assert(AMG_DEGREE <= 8, "AmgFine holds the coefficients of at most 8 steps");
 
 // One level of the host-built symbolic hierarchy.
 struct AmgHostLevel {

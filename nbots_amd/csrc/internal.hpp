@@ -135,6 +135,22 @@ void amg_free(AmgDev* a);
 int amg_vcycle(AmgDev* a, const float* r, float* z, const PcgState* st, hipStream_t s);
 const double* amg_ones(const AmgDev* a);   // [n] 1.0: the flexible CG's unit scaling
 uint32_t amg_levels(const AmgDev* a);
+// The finest level smoothed on the matrix's fp16 Jacobi-scaled copy (the
+// Chebyshev preconditioner's kernels, in scaled variables): its Chebyshev
+// coefficients and the vector for the coarse correction; `scaled16` false
+// when the hierarchy smooths every level on its own FP32 CSR copy.
+struct AmgFine {
+    bool scaled16 = false;
+    double inv_theta = 0.0;
+    double c1[8] = {}, c2[8] = {};   // steps 1 .. degree - 1
+    uint32_t degree = 0;
+    float* c = nullptr;              // [finest n]
+};
+AmgFine amg_fine(const AmgDev* a);
+// The coarse part of the scaled-variable cycle: b_1 = P~^T r (r the scaled
+// finest residual, P~ = D^1/2 P), the V-cycle on levels 1.., c = P~ x_1.
+int amg_coarse_correction(AmgDev* a, const float* r, float* c, const PcgState* st,
+                          hipStream_t s);
 void ws_free(Workspace& w);
 void mat_free(DevMatrix& m);
 

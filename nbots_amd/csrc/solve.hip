@@ -562,6 +562,38 @@ static void launch_cheb_step64(uint32_t k, uint32_t last, unsigned G, hipStream_
                            res, z, st, n, c1, c2);
 }
 
+// One V-cycle of the multigrid preconditioner whose finest level is
+// smoothed on the fp16 scaled copy K~ = D^-1/2 K D^-1/2, in scaled variables
+// (z~ = D^1/2 z, res~ = D^-1/2 res): Chebyshev smoothing in K~ is Chebyshev
+// smoothing in D^-1 K.  On entry the flexible-CG update has written
+// res = r~, din = r~ / theta, z = din (the first smoothing step from 0).
+//   pre-smoothing: steps 1 .. degree - 1;
+//   res = r~ - K~ z (a step with c1 = c2 = 0 completes the residual);
+//   c = P~ M_c^-1 P~^T res, the coarse levels' V-cycle (P~ = D^1/2 P);
+//   post-smoothing from z + c: its first step takes res -= K~ c,
+//   d = res / theta, z += c + d; then steps 1 .. degree - 1.
+template <bool NT, class M>
+static void amg16_cycle(const M& K, const AmgFine& f, AmgDev* amg, float2* res, float2* da,
+                        float2* db, float2* z, PcgState* st, uint32_t nu, unsigned G,
+                        hipStream_t s) {
+    const uint32_t last = f.degree - 1;
+    auto smooth = [&]() {
+        float2 *din = da, *dout = db;
+        for (uint32_t k = 1; k <= last; ++k) {
+            launch_cheb_step64<NT>(k, last, G, s, K, din, dout, res, z, st, nu, f.c1[k], f.c2[k]);
+            std::swap(din, dout);
+        }
+        return din;
+    };
+    const float2* dl = smooth();
+    hipLaunchKernelGGL((k_cheb_step64<NT, Z_SKIP, M>), dim3(G), dim3(BLOCK), 0, s, K, dl,
+                       dl == da ? db : da, res, z, st, nu, 0.0, 0.0);
+    (void)amg_coarse_correction(amg, (const float*)res, f.c, st, s);
+    hipLaunchKernelGGL((k_cheb_step64<NT, Z_PAIR, M>), dim3(G), dim3(BLOCK), 0, s, K,
+                       (const float2*)f.c, da, res, z, st, nu, 0.0, f.inv_theta);
+    (void)smooth();
+}
+
 // One step of the all-FP64 preconditioner (precond_bits = 64): the same
 // Chebyshev recurrence on the FP64 matrix itself (the symmetric copy the
 // FP64 products read, or the full one), vectors and arithmetic in FP64, in
@@ -1176,6 +1208,33 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         cheb_mat + (p64 ? 80.0 : 33.0) * (double)nu <= (double)INFINITY_CACHE_BYTES;
     const bool amg = opt.variant == AMG_FCG;
     const double2* ones = amg ? (const double2*)amg_ones(m.amg) : nullptr;
+    const AmgFine fine = amg ? amg_fine(m.amg) : AmgFine{};
+    // the fp16 copy's working set, as above, for the multigrid's finest level
+    const double fine_mat = m.off_packed ? 8.0 * (double)m.o.n_blocks +
+                                               4.0 * (double)m.o.n_pairs + 4.0 * (double)nu
+                                         : 12.0 * (double)m.o.n_blocks + 4.0 * (double)nu;
+    const bool fine_cached = fine_mat + 41.0 * (double)nu <= (double)INFINITY_CACHE_BYTES;
+    auto precondition_amg16 = [&](int first) {
+        hipLaunchKernelGGL(k_fcg_update<float2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv,
+                           dis, res, da, z, w.st, w.partials, nu, first, fine.inv_theta);
+        hipLaunchKernelGGL(k_fin_fcg_update, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        auto cycle = [&](const auto& K) {
+            if (fine_cached)
+                amg16_cycle<false>(K, fine, m.amg, res, da, db, z, w.st, nu, G, s);
+            else
+                amg16_cycle<true>(K, fine, m.amg, res, da, db, z, w.st, nu, G, s);
+        };
+        if (m.off_packed && m.o.width)
+            cycle(Off16U{m.o});
+        else if (m.off_packed)
+            cycle(Off16P{m.o});
+        else
+            cycle(m.o);
+        hipLaunchKernelGGL(k_fcg_dots<float2>, dim3(G), dim3(BLOCK), 0, s, z, dis, wv, r, w.st,
+                           w.partials, nu, first);
+        hipLaunchKernelGGL(k_fin_fcg_dots, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
+        hipLaunchKernelGGL(k_fcg_p<float2>, dim3(G), dim3(BLOCK), 0, s, z, dis, p, w.st, nu, first);
+    };
     auto precondition_amg = [&](int first) {
         // the update with a unit scaling writes r in FP32 (res), the V-cycle
         // z = M^-1 r, the dot products and direction read z unscaled
@@ -1190,6 +1249,10 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
                            first);
     };
     auto precondition = [&](int first) {
+        if (amg && fine.scaled16) {
+            precondition_amg16(first);
+            return;
+        }
         if (amg) {
             precondition_amg(first);
             return;
@@ -1306,8 +1369,12 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         stats->fp64_iters = w.h_st->k;
         stats->rel_res = bb > 0 ? w.h_st->tol_reached / sqrt(bb) : 0.0;
         stats->fallback = (int)w.h_st->breakdown;
-        stats->layout = (m.has_sym ? 1 : 0) | (off16 && !amg ? 2 : 0) | (packed && !amg ? 4 : 0) |
-                        (packed && !amg && m.o.width ? 8 : 0) | (amg ? 16 : 0);
+        // bits 2, 4, 8: the fp16 copy (packed columns, uniform width) that
+        // the Chebyshev preconditioner or the multigrid's finest level read
+        const bool o16 = amg ? fine.scaled16 : off16;
+        const bool pk = amg ? fine.scaled16 && m.off_packed : packed;
+        stats->layout = (m.has_sym ? 1 : 0) | (o16 ? 2 : 0) | (pk ? 4 : 0) |
+                        (pk && m.o.width ? 8 : 0) | (amg ? 16 : 0);
         stats->res0 = sqrt(w.h_st->gg0);
         const uint32_t ran = (w.h_st->k + every - 1) / every;
         NBX_TRY(sampled_ms(w, nsamp < ran ? nsamp : ran, &stats->ms_spmv_avg));

@@ -2,13 +2,15 @@
 wrote (tests/golden/c3_reference_solve.npz) and against the FP64 runs:
 
 * 2048x1024 lognormal cantilever (4,200,450 DOF): the default mixed solver
-  (FP64 flexible CG, fp16 Chebyshev preconditioner) and FP32 CG sweeps in
-  FP64 iterative refinement, each to a true relative residual 1e-8, reach
-  the reference's FP64 displacement within rel 1e-7 and the FP64 solve's
-  compliance within 1e-9;
+  (FP64 flexible CG, fp16 Chebyshev preconditioner), the same CG with one
+  FP32 smoothed-aggregation V-cycle as preconditioner, and FP32 CG sweeps
+  in FP64 iterative refinement, each to a true relative residual 1e-8,
+  reach the reference's FP64 displacement within rel 1e-7 and the FP64
+  solve's compliance within 1e-9;
 * the 1024x512 MBB half-beam SIMP loop (1,051,650 DOF), 50 iterations: the
-  compliance histories with mixed-precision and with FP64 solves agree
-  within 1e-6 and the designs stay within the volume bound.
+  compliance histories with mixed-precision solves (Chebyshev and
+  multigrid preconditioners) and with FP64 solves agree within 1e-6 and the
+  designs stay within the volume bound.
 """
 import numpy as np
 import pytest
@@ -31,15 +33,18 @@ def c3():
     f.destroy()
 
 
-@pytest.mark.parametrize("variant", ["cheb_fcg", "fp32_ir"])
+@pytest.mark.parametrize("variant", ["cheb_fcg", "amg_fcg", "fp32_ir"])
 def test_c3_mixed_against_reference(c3, variant):
-    from nbots_amd.fem import SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_CG_RESTART, solve_opts
+    from nbots_amd.fem import (SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_AMG_FCG, MIXED_CG_RESTART,
+                               solve_opts)
     f, u64, F = c3
     g = load_golden("c3_reference_solve")
-    o = (solve_opts(MIXED_CHEB_FCG, true_residual=True) if variant == "cheb_fcg" else
-         solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, true_residual=True))
+    o = {"cheb_fcg": lambda: solve_opts(MIXED_CHEB_FCG, true_residual=True),
+         "amg_fcg": lambda: solve_opts(MIXED_AMG_FCG, true_residual=True),
+         "fp32_ir": lambda: solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4, true_residual=True)}
     print(f"[c3] {variant}", flush=True)
-    info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000, opts=o)
+    info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=200000 if variant != "amg_fcg" else 2000,
+                   opts=o[variant]())
     assert info["status"] == 0, info
     assert f.true_residual(True) <= 1.0000001e-8
     u = f.displacement()
@@ -50,12 +55,15 @@ def test_c3_mixed_against_reference(c3, variant):
 
 def test_c5_simp_history_mixed_vs_fp64():
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG, solve_opts
+    from nbots_amd.fem import (Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG,
+                               MIXED_AMG_FCG, solve_opts)
     p = synth.mbb(1024, 512, rho=0.5)
     area = synth.elem_areas(p)
     hist = {}
     for name, mode, o in (("mixed", SOLVE_MIXED, solve_opts(MIXED_CHEB_FCG, simp=True,
                                                              true_residual=True)),
+                          ("amg", SOLVE_MIXED, solve_opts(MIXED_AMG_FCG, simp=True,
+                                                          true_residual=True)),
                           ("fp64", SOLVE_FP64, solve_opts(true_residual=True, simp=True))):
         print(f"[c5] {name}", flush=True)
         f = Fem.from_problem(p)
@@ -72,6 +80,8 @@ def test_c5_simp_history_mixed_vs_fp64():
         s.destroy()
         f.destroy()
     rel = np.abs(hist["mixed"] - hist["fp64"]) / np.abs(hist["fp64"])
-    print("compliance 0 / 49:", hist["fp64"][0], hist["fp64"][-1], "max rel diff", rel.max())
-    assert rel.max() <= 1e-6
+    rel_amg = np.abs(hist["amg"] - hist["fp64"]) / np.abs(hist["fp64"])
+    print("compliance 0 / 49:", hist["fp64"][0], hist["fp64"][-1], "max rel diff", rel.max(),
+          rel_amg.max())
+    assert rel.max() <= 1e-6 and rel_amg.max() <= 1e-6
     assert hist["fp64"][-1] < 0.25 * hist["fp64"][0]
