@@ -31,7 +31,7 @@ ms_per_simp_iter is the multigrid-preconditioned loop's, setup included.
     python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu]
                     [--simp-iters 50] [--no-simp] [--no-decompose]
     python bench.py --cpu-full        # the CPU baseline plan at full size
-    python bench.py --pmc-probe [--probe-mixed]   # a short run for rocprofv3 --pmc
+    python bench.py --pmc-probe [--probe-mixed | --probe-amg]   # short runs for rocprofv3
     python bench.py --simp-only [--simp-iters N] [--simp-amg-only]  # the SIMP loop alone
 """
 from __future__ import annotations
@@ -317,7 +317,16 @@ def pmc_probe(args):
     print(f"[probe] {cfg['nx']}x{cfg['ny']} cantilever: {N} DOF, {n_valid} blocks, "
           f"SpMV algorithmic bytes {spmv_bytes(n_nodes, n_valid, 64)} (FP64, full layout), "
           f"{sym_spmv_bytes(n_nodes, n_valid)} (symmetric layout)", flush=True)
-    if args.probe_mixed:
+    if args.probe_amg:
+        from nbots_amd.fem import MIXED_AMG_FCG
+        o = _opts(MIXED_AMG_FCG, true_residual=True)
+        for rep in range(3):
+            t0 = time.perf_counter()
+            info = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=AMG_MAX_ITER, opts=o)
+            print(f"[probe] AMG-preconditioned FCG {rep} ({'with' if rep == 0 else 'without'} "
+                  f"setup): {info['iters']} iterations, {time.perf_counter() - t0:.3f} s, "
+                  f"layout {info['layout']}", flush=True)
+    elif args.probe_mixed:
         print(f"[probe] Chebyshev step algorithmic bytes {cheb_step_bytes(n_nodes, n_valid, True)} "
               f"(adding to z), {cheb_step_bytes(n_nodes, n_valid, False)} (not)", flush=True)
         info = f.solve(SOLVE_MIXED, tol=0.0, max_iter=args.probe_iters)
@@ -574,6 +583,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="a short solve and a calibration kernel for rocprofv3 --pmc")
     ap.add_argument("--probe-mixed", action="store_true")
+    ap.add_argument("--probe-amg", action="store_true",
+                    help="with --pmc-probe: three AMG-preconditioned solves (the first builds "
+                         "the hierarchy)")
     ap.add_argument("--probe-iters", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--no-mixed", action="store_true")

@@ -21,12 +21,29 @@
 
 namespace nbx {
 
+// Block sliced ELL for the V-cycle's FP32 products: block rows of R
+// scalar rows (one node or aggregate), blocks of R x C values, 64 block
+// rows per slice; block k of block row I at slot slice_ptr[I/64] + 64 k +
+// I%64, its value (r, c) in plane r C + c (val[(r C + c) slots + slot]), so
+// a wavefront's loads of one block position are coalesced.  The levels'
+// operators (3 x 3 blocks below the finest level), P (2 x 3 on the finest
+// level, 3 x 3 below) and P^T (3 x 2, 3 x 3) take this form: all R rows of
+// a block row share one pattern of C-column groups.
+struct BellDev {
+    uint32_t rows = 0;
+    size_t slots = 0;
+    uint32_t* slice_ptr = nullptr;
+    uint16_t* len = nullptr;
+    uint32_t* bcol = nullptr;
+    float* val = nullptr;
+};
+
 struct AmgDevLevel {
-    uint32_t n = 0, nc = 0;
+    uint32_t n = 0, nc = 0, bs = 2;
     size_t a_nnz = 0, p_nnz = 0, w_nnz = 0;
     uint32_t *a_ptr = nullptr, *a_col = nullptr;
     double* a64 = nullptr;
-    float* a32 = nullptr;
+    float* a32 = nullptr;      // finest level only, when it is not smoothed on the fp16 copy
     double* dinv64 = nullptr;
     float* dinv32 = nullptr;
     double lmax = 0.0;
@@ -34,17 +51,17 @@ struct AmgDevLevel {
     uint32_t* self = nullptr;
     uint32_t *p_ptr = nullptr, *p_col = nullptr;
     double* p64 = nullptr;
-    float* p32 = nullptr;
     uint32_t* ap_map = nullptr;
     uint32_t *w_ptr = nullptr, *w_col = nullptr;
     double* w64 = nullptr;
     uint32_t *pt_ptr = nullptr, *pt_row = nullptr, *pt_pos = nullptr;
+    BellDev ab, pb, tb;        // the operator (below the finest level), P, P^T
     // V-cycle vectors: right-hand side and solution of this level (the
     // finest level's come from the caller), residual, two directions
     float *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
     // Chebyshev smoothing on [lambda/30, 1.1 lambda]
     float inv_theta = 0.f, c1[AMG_DEGREE] = {}, c2[AMG_DEGREE] = {};
-    int la = 8, lp = 8, lr = 16;   // lanes per row: A, P, P^T lists
+    int la = 8;                // lanes per row of the finest level's CSR smoothing
 };
 
 struct AmgDev {
@@ -94,17 +111,6 @@ __global__ __launch_bounds__(BLOCK) void k_amg_to32(const double* __restrict__ i
                                                     float* __restrict__ out, size_t n) {
     const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e < n) out[e] = (float)in[e];
-}
-
-// the finest P for the scaled variables: row i times D_ii^1/2 = 1 / dis_i
-__global__ __launch_bounds__(BLOCK) void k_amg_p_scaled(const uint32_t* __restrict__ p_ptr,
-                                                        const double* __restrict__ p,
-                                                        const double* __restrict__ dis,
-                                                        float* __restrict__ out, uint32_t n) {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const double sq = 1.0 / dis[i];
-    for (uint32_t q = p_ptr[i]; q < p_ptr[i + 1]; ++q) out[q] = (float)(p[q] * sq);
 }
 
 // D^-1 (0 where the diagonal is not positive: an unknown with no coarse or
@@ -213,6 +219,52 @@ __global__ __launch_bounds__(BLOCK) void k_amg_smooth_p(
         const double coef = od * a[e];
         const uint32_t j = a_col[e];
         for (int m = 0; m < 3; ++m) p[q + m] -= coef * t[3 * (size_t)j + m];
+    }
+}
+
+// Block values from a CSR whose R rows per block row share one pattern of
+// C-column groups: block k of block row I holds the rows' k-th groups
+// (scaled by 1 / dis of the row when dis is given: D^1/2 P on the finest
+// level in the scaled variables).
+template <int R, int C>
+__global__ __launch_bounds__(BLOCK) void k_bell_fill(BellDev B, const uint32_t* __restrict__ ptr,
+                                                     const double* __restrict__ v,
+                                                     const double* __restrict__ dis) {
+    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= B.rows) return;
+    const size_t base = (size_t)B.slice_ptr[I >> 6] + (I & 63);
+    const int len = B.len[I];
+    for (int r = 0; r < R; ++r) {
+        const size_t row = (size_t)R * I + r;
+        const uint32_t off = ptr[row];
+        const double sc = dis ? 1.0 / dis[row] : 1.0;
+        for (int k = 0; k < len; ++k)
+            for (int c = 0; c < C; ++c)
+                B.val[(size_t)(r * C + c) * B.slots + base + (size_t)k * SLICE] =
+                    (float)(v[off + (size_t)C * k + c] * sc);
+    }
+}
+// P^T's blocks from P's values through the P^T lists: block k of aggregate
+// a is node J = bcol; its value (c, r) is P(R J + r, 3 a + c), the
+// (R k + r)-th entry of column 3 a + c's list
+template <int R>
+__global__ __launch_bounds__(BLOCK) void k_bell_fill_t(BellDev B, const uint32_t* __restrict__ pt_ptr,
+                                                       const uint32_t* __restrict__ pt_pos,
+                                                       const double* __restrict__ p,
+                                                       const double* __restrict__ dis) {
+    const uint32_t a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= B.rows) return;
+    const size_t base = (size_t)B.slice_ptr[a >> 6] + (a & 63);
+    const int len = B.len[a];
+    for (int k = 0; k < len; ++k) {
+        const size_t sl = base + (size_t)k * SLICE;
+        const uint32_t J = B.bcol[sl];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < R; ++r) {
+                const uint32_t q = pt_pos[pt_ptr[3 * (size_t)a + c] + (size_t)R * k + r];
+                const double sc = dis ? 1.0 / dis[(size_t)R * J + r] : 1.0;
+                B.val[(size_t)(c * R + r) * B.slots + sl] = (float)(p[q] * sc);
+            }
     }
 }
 
@@ -377,32 +429,165 @@ __global__ __launch_bounds__(BLOCK) void k_amg_residual(
     if (valid && lane == 0) r[i] = b[i] - ax;
 }
 
-// b_c = P^T r, coarse row c from its (row, position) list
-template <int LANES>
-__global__ __launch_bounds__(BLOCK) void k_amg_restrict(
-    const uint32_t* __restrict__ pt_ptr, const uint32_t* __restrict__ pt_row,
-    const uint32_t* __restrict__ pt_pos, const float* __restrict__ p, const float* __restrict__ r,
-    float* __restrict__ bc, const PcgState* st, uint32_t nc) {
-    if (st->done) return;
-    AMG_ROW(LANES, nc);
-    float s = 0.f;
-    if (valid)
-        for (uint32_t k = pt_ptr[i] + lane; k < pt_ptr[i + 1]; k += LANES)
-            s += p[pt_pos[k]] * r[pt_row[k]];
-    s = group_sum<LANES>(s);
-    if (valid && lane == 0) bc[i] = s;
+// ---- block sliced ELL products (FP32) ---------------------------------------
+
+template <int R, int C>
+struct Bell {
+    size_t slots;
+    const uint32_t* __restrict__ slice_ptr;
+    const uint16_t* __restrict__ len;
+    const uint32_t* __restrict__ bcol;
+    const float* __restrict__ val;
+};
+template <int R, int C>
+Bell<R, C> bell(const BellDev& B) {
+    return Bell<R, C>{B.slots, B.slice_ptr, B.len, B.bcol, B.val};
 }
 
-// x += P x_c (ADD), or x = P x_c
-template <bool ADD, int LANES>
-__global__ __launch_bounds__(BLOCK) void k_amg_prolong(
-    const uint32_t* __restrict__ p_ptr, const uint32_t* __restrict__ p_col,
-    const float* __restrict__ p, const float* __restrict__ xc, float* __restrict__ x,
-    const PcgState* st, uint32_t n) {
+// y = (block row I) x, x indexed by C per block column.  Two blocks at a
+// time: their columns and values are loaded before the dependent gathers.
+template <int R, int C>
+__device__ __forceinline__ void bell_row(const Bell<R, C>& B, uint32_t I,
+                                         const float* __restrict__ x, float (&y)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r] = 0.f;
+    const size_t base = (size_t)B.slice_ptr[I >> 6] + (I & 63);
+    const int len = B.len[I];
+    for (int k0 = 0; k0 < len; k0 += 2) {
+        uint32_t J[2];
+        float a[2][R * C], xv[2][C];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (k0 + u < len) {
+                const size_t sl = base + (size_t)(k0 + u) * SLICE;
+                J[u] = B.bcol[sl];
+#pragma unroll
+                for (int m = 0; m < R * C; ++m) a[u][m] = B.val[(size_t)m * B.slots + sl];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (k0 + u < len)
+#pragma unroll
+                for (int c = 0; c < C; ++c) xv[u][c] = x[(size_t)C * J[u] + c];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (k0 + u < len)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int c = 0; c < C; ++c) y[r] += a[u][r * C + c] * xv[u][c];
+    }
+}
+
+// Chebyshev smoothing below the finest level, one thread per aggregate (3
+// rows).  The start (r = D^-1 b, d = r / theta, x = d) rides in the
+// restriction that writes b; the post-smoothing start takes A x.
+__global__ __launch_bounds__(BLOCK) void k_bsm_init(Bell<3, 3> A, const float* __restrict__ dinv,
+                                                    const float* __restrict__ b,
+                                                    const float* __restrict__ x,
+                                                    float* __restrict__ r, float* __restrict__ d,
+                                                    float inv_theta, const PcgState* st,
+                                                    uint32_t rows) {
     if (st->done) return;
-    AMG_ROW(LANES, n);
-    const float px = csr_row<LANES>(p_ptr, p_col, p, xc, i, lane, valid);
-    if (valid && lane == 0) x[i] = ADD ? x[i] + px : px;
+    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= rows) return;
+    float ax[3];
+    bell_row(A, I, x, ax);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const size_t i = 3 * (size_t)I + m;
+        const float ri = dinv[i] * (b[i] - ax[m]);
+        r[i] = ri;
+        d[i] = ri * inv_theta;
+    }
+}
+
+// r -= D^-1 A d_in; d_out = c1 d_in + c2 r; x += d_out (and d_in when ADD_IN)
+template <bool ADD_IN>
+__global__ __launch_bounds__(BLOCK) void k_bsm_step(Bell<3, 3> A, const float* __restrict__ dinv,
+                                                    const float* __restrict__ din,
+                                                    float* __restrict__ dout,
+                                                    float* __restrict__ r, float* __restrict__ x,
+                                                    float c1, float c2, const PcgState* st,
+                                                    uint32_t rows) {
+    if (st->done) return;
+    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= rows) return;
+    float ad[3];
+    bell_row(A, I, din, ad);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const size_t i = 3 * (size_t)I + m;
+        const float di = din[i];
+        const float ri = r[i] - dinv[i] * ad[m];
+        const float dn = c1 * di + c2 * ri;
+        r[i] = ri;
+        dout[i] = dn;
+        x[i] += ADD_IN ? di + dn : dn;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bresidual(Bell<3, 3> A, const float* __restrict__ b,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ r, const PcgState* st,
+                                                     uint32_t rows) {
+    if (st->done) return;
+    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= rows) return;
+    float ax[3];
+    bell_row(A, I, x, ax);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const size_t i = 3 * (size_t)I + m;
+        r[i] = b[i] - ax[m];
+    }
+}
+
+// b_c = P^T r, one thread per aggregate; INIT: also the next level's
+// smoothing start from x = 0 (r_c = D_c^-1 b_c, d_c = r_c / theta_c, x_c = d_c)
+template <int R, bool INIT>
+__global__ __launch_bounds__(BLOCK) void k_brestrict(Bell<3, R> T, const float* __restrict__ r,
+                                                     float* __restrict__ bc,
+                                                     const float* __restrict__ dinv_c,
+                                                     float* __restrict__ xc,
+                                                     float* __restrict__ rc,
+                                                     float* __restrict__ dc, float inv_theta_c,
+                                                     const PcgState* st, uint32_t aggs) {
+    if (st->done) return;
+    const uint32_t a = blockIdx.x * BLOCK + threadIdx.x;
+    if (a >= aggs) return;
+    float y[3];
+    bell_row(T, a, r, y);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const size_t i = 3 * (size_t)a + m;
+        bc[i] = y[m];
+        if constexpr (INIT) {
+            const float ri = dinv_c[i] * y[m];
+            const float di = ri * inv_theta_c;
+            rc[i] = ri;
+            dc[i] = di;
+            xc[i] = di;
+        }
+    }
+}
+
+// x += P x_c (ADD), or x = P x_c; one thread per node
+template <int R, bool ADD>
+__global__ __launch_bounds__(BLOCK) void k_bprolong(Bell<R, 3> P, const float* __restrict__ xc,
+                                                    float* __restrict__ x, const PcgState* st,
+                                                    uint32_t rows) {
+    if (st->done) return;
+    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= rows) return;
+    float y[R];
+    bell_row(P, I, xc, y);
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const size_t i = (size_t)R * I + m;
+        x[i] = ADD ? x[i] + y[m] : y[m];
+    }
 }
 
 // x = C^-1 b on the coarsest level (n <= AMG_COARSEST), one workgroup
@@ -450,15 +635,79 @@ void amg_free(AmgDev* A) {
     if (!A) return;
     for (AmgDevLevel& L : A->lv) {
         void* ps[] = {L.a_ptr, L.a_col, L.a64, L.a32, L.dinv64, L.dinv32, L.t, L.self, L.p_ptr,
-                      L.p_col, L.p64, L.p32, L.ap_map, L.w_ptr, L.w_col, L.w64, L.pt_ptr,
+                      L.p_col, L.p64, L.ap_map, L.w_ptr, L.w_col, L.w64, L.pt_ptr,
                       L.pt_row, L.pt_pos, L.b, L.x, L.r, L.d0, L.d1};
         for (void* p : ps)
             if (p) (void)hipFree(p);
+        for (BellDev* B : {&L.ab, &L.pb, &L.tb}) {
+            void* qs[] = {B->slice_ptr, B->len, B->bcol, B->val};
+            for (void* q : qs)
+                if (q) (void)hipFree(q);
+        }
     }
     void* ps[] = {A->fine_map, A->cinv, A->ones, A->pow_x, A->pow_y, A->scal, A->partials};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete A;
+}
+
+// The block sliced-ELL pattern of `rows` block rows: len_of(I) blocks in
+// row I, col_of(I, k) the k-th block's column; uploaded with room for R x C
+// FP32 values per slot.  False when a row has more blocks than the format
+// holds.
+template <int R, int C, class Len, class Col>
+static int bell_upload(uint32_t rows, Len len_of, Col col_of, BellDev& B, size_t& bytes) {
+    const uint32_t ns = (rows + SLICE - 1) / SLICE;
+    std::vector<uint32_t> sp(ns + 1, 0);
+    std::vector<uint16_t> len(rows, 0);
+    for (uint32_t sl = 0; sl < ns; ++sl) {
+        uint32_t w = 0;
+        for (uint32_t I = sl * SLICE; I < std::min<uint32_t>(rows, (sl + 1) * SLICE); ++I) {
+            const uint32_t l = len_of(I);
+            if (l > 0xffffu) {
+                set_last_error("AMG: a block row of %u blocks", l);
+                return NBX_ERR_ARG;
+            }
+            len[I] = (uint16_t)l;
+            w = std::max(w, l);
+        }
+        if ((uint64_t)sp[sl] + (uint64_t)w * SLICE > 0xffffffffull) {
+            set_last_error("AMG: block layout beyond 2^32 slots");
+            return NBX_ERR_ARG;
+        }
+        sp[sl + 1] = sp[sl] + w * SLICE;
+    }
+    std::vector<uint32_t> bcol(sp[ns], 0);
+    for (uint32_t I = 0; I < rows; ++I)
+        for (uint32_t k = 0; k < len[I]; ++k)
+            bcol[sp[I / SLICE] + (size_t)k * SLICE + I % SLICE] = col_of(I, k);
+    B.rows = rows;
+    B.slots = sp[ns];
+    NBX_TRY(up(&B.slice_ptr, sp, bytes));
+    NBX_TRY(up(&B.len, len, bytes));
+    NBX_TRY(up(&B.bcol, bcol, bytes));
+    NBX_TRY(alloc(&B.val, B.slots * R * C, bytes));
+    return NBX_OK;
+}
+
+// Whether the R rows of every block row share one pattern of complete
+// C-column groups (what the block layout stores).
+static bool grouped(uint32_t rows, int R, int C, const std::vector<uint32_t>& ptr,
+                    const std::vector<uint32_t>& col) {
+    for (uint32_t I = 0; I < rows; ++I) {
+        const uint32_t b0 = ptr[(size_t)R * I], l = ptr[(size_t)R * I + 1] - b0;
+        if (l % C) return false;
+        for (int r = 1; r < R; ++r) {
+            const uint32_t br = ptr[(size_t)R * I + r];
+            if (ptr[(size_t)R * I + r + 1] - br != l) return false;
+            for (uint32_t q = 0; q < l; ++q)
+                if (col[br + q] != col[b0 + q]) return false;
+        }
+        for (uint32_t q = 0; q < l; q += C)
+            for (int c = 1; c < C; ++c)
+                if (col[b0 + q + c] != col[b0 + q] + c || col[b0 + q] % C) return false;
+    }
+    return true;
 }
 
 // Symbolic hierarchy of the system's current K (pattern, Dirichlet rows,
@@ -511,16 +760,36 @@ static int amg_symbolic(FemSys& S) {
     S.K.amg = A;
     NBX_TRY(up(&A->fine_map, map, A->bytes));
     A->lv.resize(H.lv.size());
-    for (size_t l = 0; l < H.lv.size(); ++l) {
+    const size_t nl = H.lv.size();
+    for (size_t l = 0; l < nl; ++l) {
+        const AmgHostLevel& h = H.lv[l];
+        const uint32_t bs = l == 0 ? 2 : 3;
+        const bool ok = h.n % bs == 0 && grouped(h.n / bs, (int)bs, (int)bs, h.a_ptr, h.a_col) &&
+                        (l + 1 == nl || (h.nc % 3 == 0 &&
+                                          grouped(h.n / bs, (int)bs, 3, h.p_ptr, h.p_col)));
+        if (!ok) {
+            set_last_error("AMG: level %zu is not in %ux%u / %ux3 node blocks", l, bs, bs, bs);
+            return NBX_ERR_ARG;
+        }
+    }
+    for (size_t l = 0; l < nl; ++l) {
         const AmgHostLevel& h = H.lv[l];
         AmgDevLevel& L = A->lv[l];
         L.n = h.n;
         L.nc = h.nc;
+        L.bs = l == 0 ? 2 : 3;
         L.a_nnz = h.a_col.size();
         NBX_TRY(up(&L.a_ptr, h.a_ptr, A->bytes));
         NBX_TRY(up(&L.a_col, h.a_col, A->bytes));
         NBX_TRY(alloc(&L.a64, L.a_nnz, A->bytes));
-        NBX_TRY(alloc(&L.a32, L.a_nnz, A->bytes));
+        if (l == 0) NBX_TRY(alloc(&L.a32, L.a_nnz, A->bytes));
+        if (l > 0 && l + 1 < nl) {
+            const uint32_t* ap = h.a_ptr.data();
+            const uint32_t* ac = h.a_col.data();
+            auto alen = [&](uint32_t I) { return (ap[3 * (size_t)I + 1] - ap[3 * (size_t)I]) / 3; };
+            auto acol = [&](uint32_t I, uint32_t k) { return ac[ap[3 * (size_t)I] + 3 * k] / 3; };
+            NBX_TRY((bell_upload<3, 3>(h.n / 3, alen, acol, L.ab, A->bytes)));
+        }
         NBX_TRY(alloc(&L.dinv64, L.n, A->bytes));
         NBX_TRY(alloc(&L.dinv32, L.n, A->bytes));
         NBX_TRY(alloc(&L.r, L.n, A->bytes));
@@ -533,15 +802,28 @@ static int amg_symbolic(FemSys& S) {
         L.la = pick_lanes(L.a_nnz, L.n);
         if (l + 1 < H.lv.size()) {
             L.p_nnz = h.p_col.size();
-            L.lp = pick_lanes(L.p_nnz, L.n);
-            L.lr = pick_lanes(L.p_nnz, L.nc);
             L.w_nnz = h.w_col.size();
+            const uint32_t R = L.bs;
+            const uint32_t* pp = h.p_ptr.data();
+            const uint32_t* pc = h.p_col.data();
+            const uint32_t* tp = h.pt_ptr.data();
+            const uint32_t* tr = h.pt_row.data();
+            auto plen = [&](uint32_t I) { return (pp[R * (size_t)I + 1] - pp[R * (size_t)I]) / 3; };
+            auto pcol = [&](uint32_t I, uint32_t k) { return pc[pp[R * (size_t)I] + 3 * k] / 3; };
+            auto tlen = [&](uint32_t a) { return (tp[3 * (size_t)a + 1] - tp[3 * (size_t)a]) / R; };
+            auto tcol = [&](uint32_t a, uint32_t k) { return tr[tp[3 * (size_t)a] + R * k] / R; };
+            if (R == 2) {
+                NBX_TRY((bell_upload<2, 3>(L.n / 2, plen, pcol, L.pb, A->bytes)));
+                NBX_TRY((bell_upload<3, 2>(L.nc / 3, tlen, tcol, L.tb, A->bytes)));
+            } else {
+                NBX_TRY((bell_upload<3, 3>(L.n / 3, plen, pcol, L.pb, A->bytes)));
+                NBX_TRY((bell_upload<3, 3>(L.nc / 3, tlen, tcol, L.tb, A->bytes)));
+            }
             NBX_TRY(up(&L.t, h.t, A->bytes));
             NBX_TRY(up(&L.self, h.self, A->bytes));
             NBX_TRY(up(&L.p_ptr, h.p_ptr, A->bytes));
             NBX_TRY(up(&L.p_col, h.p_col, A->bytes));
             NBX_TRY(alloc(&L.p64, L.p_nnz, A->bytes));
-            NBX_TRY(alloc(&L.p32, L.p_nnz, A->bytes));
             NBX_TRY(up(&L.ap_map, h.ap_map, A->bytes));
             NBX_TRY(up(&L.w_ptr, h.w_ptr, A->bytes));
             NBX_TRY(up(&L.w_col, h.w_col, A->bytes));
@@ -577,7 +859,10 @@ static int amg_numeric(FemSys& S) {
         const unsigned g = grid_for(L.n);
         hipLaunchKernelGGL(k_amg_dinv, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
                            L.dinv64, L.dinv32, L.n);
-        if (!(l == 0 && A->scaled16))   // (that level's products read the fp16 copy)
+        // the finest level's smoothing reads the fp16 copy or this FP32 CSR
+        // copy; the coarser levels' operators are filled in block form by
+        // the level above
+        if (l == 0 && !A->scaled16)
             hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.a_nnz)), dim3(BLOCK), 0, s, L.a64,
                                L.a32, L.a_nnz);
         // lambda_max(D^-1 A): 15 power steps from a fixed start
@@ -640,17 +925,28 @@ static int amg_numeric(FemSys& S) {
         const double omega = 4.0 / (3.0 * L.lmax);
         hipLaunchKernelGGL(k_amg_smooth_p, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
                            L.dinv64, L.t, L.self, L.p_ptr, L.ap_map, L.p64, omega, L.n);
-        if (l == 0 && A->scaled16)
-            hipLaunchKernelGGL(k_amg_p_scaled, dim3(g), dim3(BLOCK), 0, s, L.p_ptr, L.p64, S.K.dis,
-                               L.p32, L.n);
-        else
-            hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.p_nnz)), dim3(BLOCK), 0, s, L.p64,
-                               L.p32, L.p_nnz);
+        // P and P^T in block form (D^1/2 P on the finest level when it is
+        // smoothed in the scaled variables)
+        const double* pdis = l == 0 && A->scaled16 ? S.K.dis : nullptr;
+        if (L.bs == 2) {
+            hipLaunchKernelGGL((k_bell_fill<2, 3>), dim3(grid_for(L.pb.rows)), dim3(BLOCK), 0, s,
+                               L.pb, L.p_ptr, L.p64, pdis);
+            hipLaunchKernelGGL((k_bell_fill_t<2>), dim3(grid_for(L.tb.rows)), dim3(BLOCK), 0, s,
+                               L.tb, L.pt_ptr, L.pt_pos, L.p64, pdis);
+        } else {
+            hipLaunchKernelGGL((k_bell_fill<3, 3>), dim3(grid_for(L.pb.rows)), dim3(BLOCK), 0, s,
+                               L.pb, L.p_ptr, L.p64, pdis);
+            hipLaunchKernelGGL((k_bell_fill_t<3>), dim3(grid_for(L.tb.rows)), dim3(BLOCK), 0, s,
+                               L.tb, L.pt_ptr, L.pt_pos, L.p64, pdis);
+        }
         hipLaunchKernelGGL(k_amg_w, dim3(grid_for((size_t)L.n * WAVE)), dim3(BLOCK), 0, s, L.a_ptr,
                            L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64, L.n);
         hipLaunchKernelGGL(k_amg_galerkin, dim3(grid_for((size_t)L.nc * WAVE)), dim3(BLOCK), 0, s,
                            L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64, C.a_ptr,
                            C.a_col, C.a64, L.nc);
+        if (C.ab.rows)
+            hipLaunchKernelGGL((k_bell_fill<3, 3>), dim3(grid_for(C.ab.rows)), dim3(BLOCK), 0, s,
+                               C.ab, C.a_ptr, C.a64, (const double*)nullptr);
         NBX_HIP(hipGetLastError());
     }
     return NBX_OK;
@@ -702,8 +998,55 @@ const double* amg_ones(const AmgDev* A) { return A->ones; }
 uint32_t amg_levels(const AmgDev* A) { return (uint32_t)A->lv.size(); }
 
 
+// b_{l+1} = P_l^T r_l, with the next level's smoothing start unless that
+// level is the coarsest
+static void restrict_to(AmgDev* A, size_t l, const float* r, const PcgState* st, hipStream_t s) {
+    AmgDevLevel& V = A->lv[l];
+    AmgDevLevel& W = A->lv[l + 1];
+    const bool init = l + 2 < A->lv.size();
+    const unsigned g = grid_for(V.tb.rows);
+    auto go = [&](auto R) {
+        constexpr int RR = decltype(R)::value;
+        if (init)
+            hipLaunchKernelGGL((k_brestrict<RR, true>), dim3(g), dim3(BLOCK), 0, s,
+                               bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0, W.inv_theta,
+                               st, V.tb.rows);
+        else
+            hipLaunchKernelGGL((k_brestrict<RR, false>), dim3(g), dim3(BLOCK), 0, s,
+                               bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0, W.inv_theta,
+                               st, V.tb.rows);
+    };
+    if (V.bs == 2)
+        go(std::integral_constant<int, 2>{});
+    else
+        go(std::integral_constant<int, 3>{});
+}
+
+// x_l (+)= P_l x_{l+1}
+static void prolong_from(AmgDev* A, size_t l, float* x, bool add, const PcgState* st,
+                         hipStream_t s) {
+    AmgDevLevel& V = A->lv[l];
+    const float* xc = A->lv[l + 1].x;
+    const unsigned g = grid_for(V.pb.rows);
+    auto go = [&](auto R) {
+        constexpr int RR = decltype(R)::value;
+        if (add)
+            hipLaunchKernelGGL((k_bprolong<RR, true>), dim3(g), dim3(BLOCK), 0, s,
+                               bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
+        else
+            hipLaunchKernelGGL((k_bprolong<RR, false>), dim3(g), dim3(BLOCK), 0, s,
+                               bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
+    };
+    if (V.bs == 2)
+        go(std::integral_constant<int, 2>{});
+    else
+        go(std::integral_constant<int, 3>{});
+}
+
 // The V-cycle from level l0 down, x = M^-1 b on that level (b, x: that
-// level's arrays).
+// level's arrays).  Below the finest level the restriction has already
+// started each level's pre-smoothing (r, d0, x); the finest level, when it
+// is smoothed here (not on the fp16 copy), runs on its FP32 CSR copy.
 static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const PcgState* st,
                        hipStream_t s) {
     const size_t L = A->lv.size();
@@ -711,26 +1054,34 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
         AmgDevLevel& V = A->lv[l];
         const float* bl = l == l0 ? b : V.b;
         float* xl = l == l0 ? x : V.x;
-        hipLaunchKernelGGL((k_amg_sm_init<true, 1>), dim3(rows_grid(V.n, 1)), dim3(BLOCK), 0, s,
-                           V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st,
-                           V.n);
         float *din = V.d0, *dout = V.d1;
-        with_lanes(V.la, [&](auto ln) {
-            constexpr int LN = decltype(ln)::value;
+        if (l == 0) {
+            hipLaunchKernelGGL((k_amg_sm_init<true, 1>), dim3(rows_grid(V.n, 1)), dim3(BLOCK), 0,
+                               s, V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl, V.r, V.d0,
+                               V.inv_theta, st, V.n);
+            with_lanes(V.la, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                for (int k = 1; k < AMG_DEGREE; ++k) {
+                    hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)),
+                                       dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, din,
+                                       dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
+                    std::swap(din, dout);
+                }
+                hipLaunchKernelGGL((k_amg_residual<LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
+                                   s, V.a_ptr, V.a_col, V.a32, bl, xl, V.r, st, V.n);
+            });
+        } else {
+            const unsigned g = grid_for(V.ab.rows);
             for (int k = 1; k < AMG_DEGREE; ++k) {
-                hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK),
-                                   0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, din, dout, V.r, xl,
-                                   V.c1[k], V.c2[k], st, V.n);
+                hipLaunchKernelGGL((k_bsm_step<false>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                   V.c2[k], st, V.ab.rows);
                 std::swap(din, dout);
             }
-            hipLaunchKernelGGL((k_amg_residual<LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0, s,
-                               V.a_ptr, V.a_col, V.a32, bl, xl, V.r, st, V.n);
-        });
-        with_lanes(V.lr, [&](auto ln) {
-            constexpr int LN = decltype(ln)::value;
-            hipLaunchKernelGGL((k_amg_restrict<LN>), dim3(rows_grid(V.nc, LN)), dim3(BLOCK), 0, s,
-                               V.pt_ptr, V.pt_row, V.pt_pos, V.p32, V.r, A->lv[l + 1].b, st, V.nc);
-        });
+            hipLaunchKernelGGL(k_bresidual, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab), bl, xl,
+                               V.r, st, V.ab.rows);
+        }
+        restrict_to(A, l, V.r, st, s);
     }
     {
         AmgDevLevel& C = A->lv[L - 1];
@@ -741,29 +1092,42 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
         AmgDevLevel& V = A->lv[l];
         const float* bl = l == l0 ? b : V.b;
         float* xl = l == l0 ? x : V.x;
-        with_lanes(V.lp, [&](auto ln) {
-            constexpr int LN = decltype(ln)::value;
-            hipLaunchKernelGGL((k_amg_prolong<true, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
-                               s, V.p_ptr, V.p_col, V.p32, A->lv[l + 1].x, xl, st, V.n);
-        });
-        with_lanes(V.la, [&](auto ln) {
-            constexpr int LN = decltype(ln)::value;
-            hipLaunchKernelGGL((k_amg_sm_init<false, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
-                               s, V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl, V.r, V.d0, V.inv_theta,
-                               st, V.n);
-            float *din = V.d0, *dout = V.d1;
+        prolong_from(A, l, xl, true, st, s);
+        float *din = V.d0, *dout = V.d1;
+        if (l == 0) {
+            with_lanes(V.la, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                hipLaunchKernelGGL((k_amg_sm_init<false, LN>), dim3(rows_grid(V.n, LN)),
+                                   dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl,
+                                   V.r, V.d0, V.inv_theta, st, V.n);
+                for (int k = 1; k < AMG_DEGREE; ++k) {
+                    if (k == 1)
+                        hipLaunchKernelGGL((k_amg_sm_step<true, LN>), dim3(rows_grid(V.n, LN)),
+                                           dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32,
+                                           din, dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
+                    else
+                        hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)),
+                                           dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32,
+                                           din, dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
+                    std::swap(din, dout);
+                }
+            });
+        } else {
+            const unsigned g = grid_for(V.ab.rows);
+            hipLaunchKernelGGL(k_bsm_init, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab), V.dinv32,
+                               bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
             for (int k = 1; k < AMG_DEGREE; ++k) {
                 if (k == 1)
-                    hipLaunchKernelGGL((k_amg_sm_step<true, LN>), dim3(rows_grid(V.n, LN)),
-                                       dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, din,
-                                       dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
+                    hipLaunchKernelGGL((k_bsm_step<true>), dim3(g), dim3(BLOCK), 0, s,
+                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                       V.c2[k], st, V.ab.rows);
                 else
-                    hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)),
-                                       dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, din,
-                                       dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
+                    hipLaunchKernelGGL((k_bsm_step<false>), dim3(g), dim3(BLOCK), 0, s,
+                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                       V.c2[k], st, V.ab.rows);
                 std::swap(din, dout);
             }
-        });
+        }
     }
 }
 
@@ -777,19 +1141,9 @@ int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStrea
 
 int amg_coarse_correction(AmgDev* A, const float* r, float* c, const PcgState* st,
                           hipStream_t s) {
-    AmgDevLevel& V = A->lv[0];
-    AmgDevLevel& W = A->lv[1];
-    with_lanes(V.lr, [&](auto ln) {
-        constexpr int LN = decltype(ln)::value;
-        hipLaunchKernelGGL((k_amg_restrict<LN>), dim3(rows_grid(V.nc, LN)), dim3(BLOCK), 0, s,
-                           V.pt_ptr, V.pt_row, V.pt_pos, V.p32, r, W.b, st, V.nc);
-    });
-    cycle_from(A, 1, W.b, W.x, st, s);
-    with_lanes(V.lp, [&](auto ln) {
-        constexpr int LN = decltype(ln)::value;
-        hipLaunchKernelGGL((k_amg_prolong<false, LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
-                           s, V.p_ptr, V.p_col, V.p32, W.x, c, st, V.n);
-    });
+    restrict_to(A, 0, r, st, s);
+    cycle_from(A, 1, A->lv[1].b, A->lv[1].x, st, s);
+    prolong_from(A, 0, c, false, st, s);
     NBX_HIP(hipGetLastError());
     return NBX_OK;
 }
