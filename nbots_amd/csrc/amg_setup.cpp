@@ -6,7 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
-#include <iterator>
+#include <thread>
 
 #include "amg.hpp"
 
@@ -18,6 +18,60 @@ struct Graph {
     std::vector<uint32_t> ptr, adj;
 };
 
+// Host threads for the setup (at most 16: a one-GPU job's share of a host).
+unsigned setup_threads(uint32_t items) {
+    const unsigned h = std::thread::hardware_concurrency();
+    const unsigned t = std::min(h == 0 ? 1u : h, 16u);
+    return std::max(1u, std::min(t, items / 4096u));
+}
+
+// f(lo, hi) over contiguous ranges of [0, n), one per thread.
+template <class F>
+void par_for(uint32_t n, F f) {
+    const unsigned nt = setup_threads(n);
+    if (nt <= 1) {
+        f(0u, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back(f, (uint32_t)((uint64_t)n * t / nt), (uint32_t)((uint64_t)n * (t + 1) / nt));
+    for (std::thread& x : th) x.join();
+}
+
+// Per-item lists built in parallel: f(i, out) appends item i's list to
+// out.  The result (ptr[n + 1], data) does not depend on the threads.
+template <class F>
+void build_lists(uint32_t n, F f, std::vector<uint32_t>& ptr, std::vector<uint32_t>& data) {
+    const unsigned nt = setup_threads(n);
+    std::vector<std::vector<uint32_t>> part(nt);
+    std::vector<uint32_t> len(n);
+    std::vector<uint32_t> lo(nt + 1);
+    for (unsigned t = 0; t <= nt; ++t) lo[t] = (uint32_t)((uint64_t)n * t / nt);
+    auto fill = [&](unsigned t) {
+        std::vector<uint32_t>& out = part[t];
+        for (uint32_t i = lo[t]; i < lo[t + 1]; ++i) {
+            const size_t before = out.size();
+            f(i, out);
+            len[i] = (uint32_t)(out.size() - before);
+        }
+    };
+    if (nt == 1) {
+        fill(0);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(fill, t);
+        for (std::thread& x : th) x.join();
+    }
+    ptr.assign(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + len[i];
+    data.resize(ptr[n]);
+    for (unsigned t = 0; t < nt; ++t)
+        std::copy(part[t].begin(), part[t].end(), data.begin() + ptr[lo[t]]);
+}
+
+
 // Node graph of a level: node I's neighbours are the nodes J != I that hold
 // a column of one of I's rows.  An unknown marked dead (Dirichlet-eliminated
 // on the finest level, a rank-deficient coarse mode below) has no edges, and
@@ -26,28 +80,23 @@ struct Graph {
 Graph node_graph(const AmgHostLevel& L, const std::vector<uint8_t>& dead, const double* val) {
     const uint32_t bs = L.bs, nodes = L.n / bs;
     Graph g;
-    g.ptr.assign(nodes + 1, 0);
-    std::vector<uint32_t> tmp;
-    std::vector<std::vector<uint32_t>> rows(nodes);
-    for (uint32_t I = 0; I < nodes; ++I) {
-        tmp.clear();
-        for (uint32_t i = I * bs; i < I * bs + bs; ++i) {
-            if (dead[i]) continue;
-            for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
-                const uint32_t j = L.a_col[e], J = j / bs;
-                if (J == I || dead[j]) continue;
-                if (val && val[e] == 0.0) continue;
-                tmp.push_back(J);
+    build_lists(
+        nodes,
+        [&](uint32_t I, std::vector<uint32_t>& out) {
+            const size_t b = out.size();
+            for (uint32_t i = I * bs; i < I * bs + bs; ++i) {
+                if (dead[i]) continue;
+                for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
+                    const uint32_t j = L.a_col[e], J = j / bs;
+                    if (J == I || dead[j]) continue;
+                    if (val && val[e] == 0.0) continue;
+                    out.push_back(J);
+                }
             }
-        }
-        std::sort(tmp.begin(), tmp.end());
-        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-        rows[I] = tmp;
-        g.ptr[I + 1] = g.ptr[I] + (uint32_t)tmp.size();
-    }
-    g.adj.resize(g.ptr[nodes]);
-    for (uint32_t I = 0; I < nodes; ++I)
-        std::copy(rows[I].begin(), rows[I].end(), g.adj.begin() + g.ptr[I]);
+            std::sort(out.begin() + (ptrdiff_t)b, out.end());
+            out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
+        },
+        g.ptr, g.adj);
     return g;
 }
 
@@ -88,14 +137,6 @@ uint32_t aggregate(const Graph& g, std::vector<uint32_t>& agg) {
     return na;
 }
 
-// Sorted union of the given sorted lists.
-void merge_into(std::vector<uint32_t>& acc, const uint32_t* b, const uint32_t* e) {
-    std::vector<uint32_t> out;
-    out.reserve(acc.size() + (size_t)(e - b));
-    std::set_union(acc.begin(), acc.end(), b, e, std::back_inserter(out));
-    acc.swap(out);
-}
-
 uint32_t find(const uint32_t* b, const uint32_t* e, uint32_t c) {
     const uint32_t* p = std::lower_bound(b, e, c);
     return (p != e && *p == c) ? (uint32_t)(p - b) : AMG_NONE;
@@ -126,8 +167,9 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     L.t.assign(3 * (size_t)n, 0.0);
     Bc.assign(3 * (size_t)L.nc, 0.0);
     dead_c.assign(L.nc, 0);
+    par_for(na, [&](uint32_t a0, uint32_t a1) {
     std::vector<double> q;
-    for (uint32_t a = 0; a < na; ++a) {
+    for (uint32_t a = a0; a < a1; ++a) {
         const uint32_t m = (mptr[a + 1] - mptr[a]) * bs;
         q.assign(3 * (size_t)m, 0.0);   // q[c * m + r]
         for (uint32_t k = 0; k < mptr[a + 1] - mptr[a]; ++k)
@@ -168,108 +210,134 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) Bc[3 * (size_t)(3 * a + r) + c] = dead_c[3 * a + r] ? 0.0 : R[r][c];
     }
+    });
     // P pattern per node (shared by its rows): the aggregates of the nodes
     // its rows reach, three columns each
-    std::vector<uint32_t> pnode_ptr(nodes + 1, 0), pnode;
-    {
-        std::vector<uint32_t> tmp;
-        for (uint32_t I = 0; I < nodes; ++I) {
-            tmp.clear();
+    std::vector<uint32_t> anode_ptr, anode;   // the aggregates, ascending
+    build_lists(
+        nodes,
+        [&](uint32_t I, std::vector<uint32_t>& out) {
+            const size_t b = out.size();
             for (uint32_t i = I * bs; i < I * bs + bs; ++i)
                 for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
                     const uint32_t A = agg[L.a_col[e] / bs];
-                    if (A != AMG_NONE) tmp.push_back(A);
+                    if (A != AMG_NONE) out.push_back(A);
                 }
-            std::sort(tmp.begin(), tmp.end());
-            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            for (uint32_t A : tmp)
-                for (uint32_t c = 0; c < 3; ++c) pnode.push_back(3 * A + c);
-            pnode_ptr[I + 1] = (uint32_t)pnode.size();
-        }
-    }
+            std::sort(out.begin() + (ptrdiff_t)b, out.end());
+            out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
+        },
+        anode_ptr, anode);
     L.p_ptr.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i)
-        L.p_ptr[i + 1] = L.p_ptr[i] + (pnode_ptr[i / bs + 1] - pnode_ptr[i / bs]);
+        L.p_ptr[i + 1] = L.p_ptr[i] + 3 * (anode_ptr[i / bs + 1] - anode_ptr[i / bs]);
     L.p_col.resize(L.p_ptr[n]);
     L.self.assign(n, AMG_NONE);
     L.ap_map.assign(L.a_ptr[n], AMG_NONE);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t I = i / bs;
-        const uint32_t* b = &pnode[pnode_ptr[I]];
-        const uint32_t* e = b + (pnode_ptr[I + 1] - pnode_ptr[I]);
-        std::copy(b, e, L.p_col.begin() + L.p_ptr[i]);
-        if (agg[I] != AMG_NONE) L.self[i] = L.p_ptr[i] + find(b, e, 3 * agg[I]);
-        for (uint32_t k = L.a_ptr[i]; k < L.a_ptr[i + 1]; ++k) {
-            const uint32_t A = agg[L.a_col[k] / bs];
-            if (A != AMG_NONE) L.ap_map[k] = L.p_ptr[i] + find(b, e, 3 * A);
+    par_for(nodes, [&](uint32_t I0, uint32_t I1) {
+        for (uint32_t I = I0; I < I1; ++I) {
+            const uint32_t* b = &anode[anode_ptr[I]];
+            const uint32_t* e = &anode[0] + anode_ptr[I + 1];
+            for (uint32_t i = I * bs; i < I * bs + bs; ++i) {
+                uint32_t q = L.p_ptr[i];
+                for (const uint32_t* A = b; A != e; ++A)
+                    for (uint32_t c = 0; c < 3; ++c) L.p_col[q++] = 3 * *A + c;
+                if (agg[I] != AMG_NONE) L.self[i] = L.p_ptr[i] + 3 * find(b, e, agg[I]);
+                for (uint32_t k = L.a_ptr[i]; k < L.a_ptr[i + 1]; ++k) {
+                    const uint32_t A = agg[L.a_col[k] / bs];
+                    if (A != AMG_NONE) L.ap_map[k] = L.p_ptr[i] + 3 * find(b, e, A);
+                }
+            }
         }
-    }
+    });
     // W = A P pattern per node: the union of the P rows its rows reach
-    std::vector<uint32_t> wnode_ptr(nodes + 1, 0), wnode;
-    {
-        std::vector<uint32_t> acc, nb;
-        for (uint32_t I = 0; I < nodes; ++I) {
-            nb.clear();
+    // (aggregates; three columns each)
+    std::vector<uint32_t> wnode_ptr, wnode;
+    build_lists(
+        nodes,
+        [&](uint32_t I, std::vector<uint32_t>& out) {
+            const size_t b = out.size();
+            uint32_t prev = AMG_NONE;
             for (uint32_t i = I * bs; i < I * bs + bs; ++i)
-                for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) nb.push_back(L.a_col[e] / bs);
-            std::sort(nb.begin(), nb.end());
-            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
-            acc.clear();
-            for (uint32_t J : nb)
-                merge_into(acc, pnode.data() + pnode_ptr[J], pnode.data() + pnode_ptr[J + 1]);
-            wnode.insert(wnode.end(), acc.begin(), acc.end());
-            wnode_ptr[I + 1] = (uint32_t)wnode.size();
-        }
-    }
+                for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
+                    const uint32_t J = L.a_col[e] / bs;
+                    if (J == prev) continue;   // a node's columns are adjacent
+                    prev = J;
+                    out.insert(out.end(), anode.begin() + anode_ptr[J],
+                               anode.begin() + anode_ptr[J + 1]);
+                }
+            std::sort(out.begin() + (ptrdiff_t)b, out.end());
+            out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
+        },
+        wnode_ptr, wnode);
     L.w_ptr.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i)
-        L.w_ptr[i + 1] = L.w_ptr[i] + (wnode_ptr[i / bs + 1] - wnode_ptr[i / bs]);
+        L.w_ptr[i + 1] = L.w_ptr[i] + 3 * (wnode_ptr[i / bs + 1] - wnode_ptr[i / bs]);
     L.w_col.resize(L.w_ptr[n]);
-    for (uint32_t i = 0; i < n; ++i)
-        std::copy(wnode.begin() + wnode_ptr[i / bs], wnode.begin() + wnode_ptr[i / bs + 1],
-                  L.w_col.begin() + L.w_ptr[i]);
-    // P^T lists: coarse column c -> the rows i holding it and its position
+    par_for(n, [&](uint32_t i0, uint32_t i1) {
+        for (uint32_t i = i0; i < i1; ++i) {
+            uint32_t q = L.w_ptr[i];
+            for (uint32_t k = wnode_ptr[i / bs]; k < wnode_ptr[i / bs + 1]; ++k)
+                for (uint32_t c = 0; c < 3; ++c) L.w_col[q++] = 3 * wnode[k] + c;
+        }
+    });
+    // the nodes whose P rows hold each aggregate, ascending
+    std::vector<uint32_t> hptr(na + 1, 0), hold(anode.size());
+    for (uint32_t A : anode) ++hptr[A + 1];
+    for (uint32_t a = 0; a < na; ++a) hptr[a + 1] += hptr[a];
+    {
+        std::vector<uint32_t> fill(hptr.begin(), hptr.end() - 1);
+        for (uint32_t I = 0; I < nodes; ++I)
+            for (uint32_t k = anode_ptr[I]; k < anode_ptr[I + 1]; ++k) hold[fill[anode[k]]++] = I;
+    }
+    // P^T lists: coarse column c -> the rows i holding it (ascending) and
+    // its position in P
     L.pt_ptr.assign(L.nc + 1, 0);
-    for (uint32_t p = 0; p < L.p_ptr[n]; ++p) ++L.pt_ptr[L.p_col[p] + 1];
-    for (uint32_t c = 0; c < L.nc; ++c) L.pt_ptr[c + 1] += L.pt_ptr[c];
+    for (uint32_t c = 0; c < L.nc; ++c)
+        L.pt_ptr[c + 1] = L.pt_ptr[c] + bs * (hptr[c / 3 + 1] - hptr[c / 3]);
     L.pt_row.resize(L.p_ptr[n]);
     L.pt_pos.resize(L.p_ptr[n]);
-    {
-        std::vector<uint32_t> fill(L.pt_ptr.begin(), L.pt_ptr.end() - 1);
-        for (uint32_t i = 0; i < n; ++i)
-            for (uint32_t p = L.p_ptr[i]; p < L.p_ptr[i + 1]; ++p) {
-                const uint32_t c = L.p_col[p];
-                L.pt_row[fill[c]] = i;
-                L.pt_pos[fill[c]++] = p;
+    par_for(na, [&](uint32_t a0, uint32_t a1) {
+        for (uint32_t a = a0; a < a1; ++a)
+            for (uint32_t m = 0; m < 3; ++m) {
+                uint32_t q = L.pt_ptr[3 * a + m];
+                for (uint32_t k = hptr[a]; k < hptr[a + 1]; ++k) {
+                    const uint32_t I = hold[k];
+                    const uint32_t pos = 3 * find(&anode[anode_ptr[I]], &anode[0] + anode_ptr[I + 1], a) + m;
+                    for (uint32_t i = I * bs; i < I * bs + bs; ++i) {
+                        L.pt_row[q] = i;
+                        L.pt_pos[q++] = L.p_ptr[i] + pos;
+                    }
+                }
             }
-    }
+    });
     // coarse pattern per aggregate: the union of the W rows of the nodes
     // whose P rows hold it
     C.n = L.nc;
     C.bs = 3;
-    C.a_ptr.assign(C.n + 1, 0);
-    std::vector<uint32_t> cnode_ptr(na + 1, 0), cnode;
-    {
-        std::vector<uint32_t> acc, last(nodes, AMG_NONE);
-        for (uint32_t a = 0; a < na; ++a) {
-            acc.clear();
-            const uint32_t c = 3 * a;
-            for (uint32_t k = L.pt_ptr[c]; k < L.pt_ptr[c + 1]; ++k) {
-                const uint32_t I = L.pt_row[k] / bs;
-                if (last[I] == a) continue;   // the node's rows share one W pattern
-                last[I] = a;
-                merge_into(acc, wnode.data() + wnode_ptr[I], wnode.data() + wnode_ptr[I + 1]);
+    std::vector<uint32_t> cnode_ptr, cnode;
+    build_lists(
+        na,
+        [&](uint32_t a, std::vector<uint32_t>& out) {
+            const size_t b = out.size();
+            for (uint32_t k = hptr[a]; k < hptr[a + 1]; ++k) {
+                const uint32_t I = hold[k];
+                out.insert(out.end(), wnode.begin() + wnode_ptr[I], wnode.begin() + wnode_ptr[I + 1]);
             }
-            cnode.insert(cnode.end(), acc.begin(), acc.end());
-            cnode_ptr[a + 1] = (uint32_t)cnode.size();
-        }
-    }
+            std::sort(out.begin() + (ptrdiff_t)b, out.end());
+            out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
+        },
+        cnode_ptr, cnode);
+    C.a_ptr.assign(C.n + 1, 0);
     for (uint32_t c = 0; c < C.n; ++c)
-        C.a_ptr[c + 1] = C.a_ptr[c] + (cnode_ptr[c / 3 + 1] - cnode_ptr[c / 3]);
+        C.a_ptr[c + 1] = C.a_ptr[c] + 3 * (cnode_ptr[c / 3 + 1] - cnode_ptr[c / 3]);
     C.a_col.resize(C.a_ptr[C.n]);
-    for (uint32_t c = 0; c < C.n; ++c)
-        std::copy(cnode.begin() + cnode_ptr[c / 3], cnode.begin() + cnode_ptr[c / 3 + 1],
-                  C.a_col.begin() + C.a_ptr[c]);
+    par_for(C.n, [&](uint32_t c0, uint32_t c1) {
+        for (uint32_t c = c0; c < c1; ++c) {
+            uint32_t q = C.a_ptr[c];
+            for (uint32_t k = cnode_ptr[c / 3]; k < cnode_ptr[c / 3 + 1]; ++k)
+                for (uint32_t m = 0; m < 3; ++m) C.a_col[q++] = 3 * cnode[k] + m;
+        }
+    });
 }
 
 }  // namespace
