@@ -268,83 +268,132 @@ __global__ __launch_bounds__(BLOCK) void k_bell_fill_t(BellDev B, const uint32_t
     }
 }
 
-// W = A P and A_c = P^T W take one wavefront per output row.  Each lane owns
-// up to 64 consecutive entries of the row at a time; the wavefront walks the
-// row's terms in order, loads the rows it multiplies 64 entries at a time,
-// and every lane picks (by broadcast of each entry) the ones whose column it
-// owns.  So each output entry sums its products in the pattern's order, with
-// no atomics and no lookups; the host built every output row to contain
-// every product's column.
-__device__ __forceinline__ uint32_t bcast_u32(uint32_t v, int t) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, t);
-}
-__device__ __forceinline__ double bcast_f64(double v, int t) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, t);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), t);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
+// W = A P and A_c = P^T W, one thread per output block.  The host built
+// every pattern in groups: A's rows of one node (R rows: R = 2 on the finest
+// level, 3 below) share its pattern of R-column groups, P's, W's and A_c's
+// rows share patterns of 3-column groups (aggregates), all ascending; W's
+// row holds every aggregate of every P row its A row reaches, A_c's row
+// every aggregate of every W row of a node holding it.  An output block sums
+// its block products in a fixed order (the neighbour blocks, or the nodes
+// holding the aggregate, ascending): no atomics, and runs are reproducible.
 
-// acc += s * (the row [b, e) of (col, val) restricted to column mycol)
-__device__ __forceinline__ double pick_row(const uint32_t* __restrict__ col,
-                                           const double* __restrict__ val, uint32_t b, uint32_t e,
-                                           uint32_t mycol, double s, double acc, int lane) {
-    for (uint32_t o = b; o < e; o += WAVE) {
-        const uint32_t q = o + (uint32_t)lane;
-        const uint32_t qc = q < e ? col[q] : AMG_NONE;
-        const double qv = q < e ? val[q] : 0.0;
-        const int cnt = (int)((e - o) < (uint32_t)WAVE ? (e - o) : (uint32_t)WAVE);
-        for (int t = 0; t < cnt; ++t) {
-            const uint32_t ct = bcast_u32(qc, t);
-            const double vt = bcast_f64(qv, t);
-            if (mycol == ct) acc += s * vt;
-        }
+// the block row whose blocks start at or before block t: the last i with
+// ptr[stride i] <= per t (ptr ascending, rows in [0, rows))
+__device__ __forceinline__ uint32_t block_row(const uint32_t* __restrict__ ptr, uint32_t stride,
+                                              size_t per, size_t t, uint32_t rows) {
+    uint32_t lo = 0, hi = rows;   // invariant: ptr[stride lo] <= per t < ptr[stride hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if ((size_t)ptr[(size_t)stride * mid] <= per * t) lo = mid; else hi = mid;
     }
-    return acc;
+    return lo;
 }
-
-__global__ __launch_bounds__(BLOCK) void k_amg_w(
-    const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_col,
-    const double* __restrict__ a, const uint32_t* __restrict__ p_ptr,
-    const uint32_t* __restrict__ p_col, const double* __restrict__ p,
-    const uint32_t* __restrict__ w_ptr, const uint32_t* __restrict__ w_col,
-    double* __restrict__ w, uint32_t n) {
-    const uint32_t i = (uint32_t)(((size_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE);
-    const int lane = (int)(threadIdx.x % WAVE);
-    if (i >= n) return;   // uniform in the wavefront
-    const uint32_t wb = w_ptr[i], we = w_ptr[i + 1];
-    for (uint32_t o = wb; o < we; o += WAVE) {
-        const uint32_t my = o + (uint32_t)lane;
-        const uint32_t mycol = my < we ? w_col[my] : AMG_NONE;
-        double acc = 0.0;
-        for (uint32_t e = a_ptr[i]; e < a_ptr[i + 1]; ++e) {
-            const uint32_t j = a_col[e];
-            acc = pick_row(p_col, p, p_ptr[j], p_ptr[j + 1], mycol, a[e], acc, lane);
-        }
-        if (my < we) w[my] = acc;
+// position of group b3 (a multiple of 3) in the ascending groups
+// col[b], col[b + 3], ..., col[b + 3 (len - 1)], or len
+__device__ __forceinline__ uint32_t find_group(const uint32_t* __restrict__ col, uint32_t b,
+                                               uint32_t len, uint32_t b3) {
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (col[b + 3 * mid] < b3) lo = mid + 1; else hi = mid;
     }
+    return lo < len && col[b + 3 * lo] == b3 ? lo : len;
 }
 
+// W(I, b) = sum over neighbour blocks J of A(I, J) P(J, b)
+template <int R>
+__global__ __launch_bounds__(BLOCK) void k_amg_w(const uint32_t* __restrict__ a_ptr,
+                                                 const uint32_t* __restrict__ a_col,
+                                                 const double* __restrict__ a,
+                                                 const uint32_t* __restrict__ p_ptr,
+                                                 const uint32_t* __restrict__ p_col,
+                                                 const double* __restrict__ p,
+                                                 const uint32_t* __restrict__ w_ptr,
+                                                 const uint32_t* __restrict__ w_col,
+                                                 double* __restrict__ w, uint32_t nodes,
+                                                 size_t blocks) {
+    const size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= blocks) return;
+    const uint32_t I = block_row(w_ptr, R, 3 * R, t, nodes);
+    const uint32_t j = (uint32_t)(t - w_ptr[(size_t)R * I] / (3 * R));
+    const uint32_t b3 = w_col[w_ptr[(size_t)R * I] + 3 * j];
+    uint32_t ar[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) ar[r] = a_ptr[(size_t)R * I + r];
+    const uint32_t al = (a_ptr[(size_t)R * I + 1] - ar[0]) / R;
+    double acc[R][3] = {};
+    for (uint32_t k = 0; k < al; ++k) {
+        const uint32_t J = a_col[ar[0] + R * k] / R;
+        const uint32_t p0 = p_ptr[(size_t)R * J];
+        const uint32_t kk = find_group(p_col, p0, (p_ptr[(size_t)R * J + 1] - p0) / 3, b3);
+        if (kk == (p_ptr[(size_t)R * J + 1] - p0) / 3) continue;
+        double pk[R][3];
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) pk[c][m] = p[p_ptr[(size_t)R * J + c] + 3 * kk + m];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < R; ++c) s += a[ar[r] + R * k + c] * pk[c][m];
+                acc[r][m] += s;
+            }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) w[w_ptr[(size_t)R * I + r] + 3 * j + m] = acc[r][m];
+}
+
+// A_c(a, b) = sum over the nodes J holding aggregate a of P(J, a)^T W(J, b)
+template <int R>
 __global__ __launch_bounds__(BLOCK) void k_amg_galerkin(
     const uint32_t* __restrict__ pt_ptr, const uint32_t* __restrict__ pt_row,
     const uint32_t* __restrict__ pt_pos, const double* __restrict__ p,
     const uint32_t* __restrict__ w_ptr, const uint32_t* __restrict__ w_col,
     const double* __restrict__ w, const uint32_t* __restrict__ c_ptr,
-    const uint32_t* __restrict__ c_col, double* __restrict__ ac, uint32_t nc) {
-    const uint32_t c = (uint32_t)(((size_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE);
-    const int lane = (int)(threadIdx.x % WAVE);
-    if (c >= nc) return;   // uniform in the wavefront
-    const uint32_t cb = c_ptr[c], ce = c_ptr[c + 1];
-    for (uint32_t o = cb; o < ce; o += WAVE) {
-        const uint32_t my = o + (uint32_t)lane;
-        const uint32_t mycol = my < ce ? c_col[my] : AMG_NONE;
-        double acc = 0.0;
-        for (uint32_t k = pt_ptr[c]; k < pt_ptr[c + 1]; ++k) {
-            const uint32_t i = pt_row[k];
-            acc = pick_row(w_col, w, w_ptr[i], w_ptr[i + 1], mycol, p[pt_pos[k]], acc, lane);
-        }
-        if (my < ce) ac[my] = acc;
+    const uint32_t* __restrict__ c_col, double* __restrict__ ac, uint32_t aggs, size_t blocks) {
+    const size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= blocks) return;
+    const uint32_t A = block_row(c_ptr, 3, 9, t, aggs);
+    const uint32_t j = (uint32_t)(t - c_ptr[3 * (size_t)A] / 9);
+    const uint32_t b3 = c_col[c_ptr[3 * (size_t)A] + 3 * j];
+    uint32_t tr[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) tr[m] = pt_ptr[3 * (size_t)A + m];
+    const uint32_t hl = (pt_ptr[3 * (size_t)A + 1] - tr[0]) / R;
+    double acc[3][3] = {};
+    for (uint32_t h = 0; h < hl; ++h) {
+        const uint32_t J = pt_row[tr[0] + R * h] / R;
+        const uint32_t w0 = w_ptr[(size_t)R * J];
+        const uint32_t wl = (w_ptr[(size_t)R * J + 1] - w0) / 3;
+        const uint32_t q = find_group(w_col, w0, wl, b3);
+        if (q == wl) continue;
+        double wq[R][3], ph[R][3];
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                wq[c][n] = w[w_ptr[(size_t)R * J + c] + 3 * q + n];
+                ph[c][n] = p[pt_pos[tr[n] + R * h + c]];
+            }
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < R; ++c) s += ph[c][m] * wq[c][n];
+                acc[m][n] += s;
+            }
     }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) ac[c_ptr[3 * (size_t)A + m] + 3 * j + n] = acc[m][n];
 }
 
 // ---- V-cycle (FP32) ----------------------------------------------------------
@@ -766,7 +815,8 @@ static int amg_symbolic(FemSys& S) {
         const uint32_t bs = l == 0 ? 2 : 3;
         const bool ok = h.n % bs == 0 && grouped(h.n / bs, (int)bs, (int)bs, h.a_ptr, h.a_col) &&
                         (l + 1 == nl || (h.nc % 3 == 0 &&
-                                          grouped(h.n / bs, (int)bs, 3, h.p_ptr, h.p_col)));
+                                          grouped(h.n / bs, (int)bs, 3, h.p_ptr, h.p_col) &&
+                                          grouped(h.n / bs, (int)bs, 3, h.w_ptr, h.w_col)));
         if (!ok) {
             set_last_error("AMG: level %zu is not in %ux%u / %ux3 node blocks", l, bs, bs, bs);
             return NBX_ERR_ARG;
@@ -939,11 +989,23 @@ static int amg_numeric(FemSys& S) {
             hipLaunchKernelGGL((k_bell_fill_t<3>), dim3(grid_for(L.tb.rows)), dim3(BLOCK), 0, s,
                                L.tb, L.pt_ptr, L.pt_pos, L.p64, pdis);
         }
-        hipLaunchKernelGGL(k_amg_w, dim3(grid_for((size_t)L.n * WAVE)), dim3(BLOCK), 0, s, L.a_ptr,
-                           L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64, L.n);
-        hipLaunchKernelGGL(k_amg_galerkin, dim3(grid_for((size_t)L.nc * WAVE)), dim3(BLOCK), 0, s,
-                           L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64, C.a_ptr,
-                           C.a_col, C.a64, L.nc);
+        const uint32_t nodes = L.n / L.bs, aggs = L.nc / 3;
+        const size_t wblocks = L.w_nnz / (3 * L.bs), cblocks = C.a_nnz / 9;
+        if (L.bs == 2) {
+            hipLaunchKernelGGL(k_amg_w<2>, dim3(grid_for(wblocks)), dim3(BLOCK), 0, s, L.a_ptr,
+                               L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64,
+                               nodes, wblocks);
+            hipLaunchKernelGGL(k_amg_galerkin<2>, dim3(grid_for(cblocks)), dim3(BLOCK), 0, s,
+                               L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64,
+                               C.a_ptr, C.a_col, C.a64, aggs, cblocks);
+        } else {
+            hipLaunchKernelGGL(k_amg_w<3>, dim3(grid_for(wblocks)), dim3(BLOCK), 0, s, L.a_ptr,
+                               L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64,
+                               nodes, wblocks);
+            hipLaunchKernelGGL(k_amg_galerkin<3>, dim3(grid_for(cblocks)), dim3(BLOCK), 0, s,
+                               L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64,
+                               C.a_ptr, C.a_col, C.a64, aggs, cblocks);
+        }
         if (C.ab.rows)
             hipLaunchKernelGGL((k_bell_fill<3, 3>), dim3(grid_for(C.ab.rows)), dim3(BLOCK), 0, s,
                                C.ab, C.a_ptr, C.a64, (const double*)nullptr);
