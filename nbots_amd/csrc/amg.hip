@@ -31,6 +31,7 @@ namespace nbx {
 // a block row share one pattern of C-column groups.
 struct BellDev {
     uint32_t rows = 0;
+    int lanes = 1;             // lanes per block row in the V-cycle's products
     size_t slots = 0;
     uint32_t* slice_ptr = nullptr;
     uint16_t* len = nullptr;
@@ -493,22 +494,24 @@ Bell<R, C> bell(const BellDev& B) {
     return Bell<R, C>{B.slots, B.slice_ptr, B.len, B.bcol, B.val};
 }
 
-// y = (block row I) x, x indexed by C per block column.  Two blocks at a
-// time: their columns and values are loaded before the dependent gathers.
-template <int R, int C>
-__device__ __forceinline__ void bell_row(const Bell<R, C>& B, uint32_t I,
+// y = (block row I) x, x indexed by C per block column, by a group of LANES
+// lanes: lane l takes blocks l, l + LANES, ..., two at a time (their columns
+// and values loaded before the dependent gathers), and the group adds the
+// lanes' sums by a butterfly; every lane gets the result.
+template <int R, int C, int LANES>
+__device__ __forceinline__ void bell_row(const Bell<R, C>& B, uint32_t I, uint32_t lane,
                                          const float* __restrict__ x, float (&y)[R]) {
 #pragma unroll
     for (int r = 0; r < R; ++r) y[r] = 0.f;
     const size_t base = (size_t)B.slice_ptr[I >> 6] + (I & 63);
     const int len = B.len[I];
-    for (int k0 = 0; k0 < len; k0 += 2) {
+    for (int k0 = (int)lane; k0 < len; k0 += 2 * LANES) {
         uint32_t J[2];
         float a[2][R * C], xv[2][C];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            if (k0 + u < len) {
-                const size_t sl = base + (size_t)(k0 + u) * SLICE;
+            if (k0 + u * LANES < len) {
+                const size_t sl = base + (size_t)(k0 + u * LANES) * SLICE;
                 J[u] = B.bcol[sl];
 #pragma unroll
                 for (int m = 0; m < R * C; ++m) a[u][m] = B.val[(size_t)m * B.slots + sl];
@@ -516,22 +519,32 @@ __device__ __forceinline__ void bell_row(const Bell<R, C>& B, uint32_t I,
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-            if (k0 + u < len)
+            if (k0 + u * LANES < len)
 #pragma unroll
                 for (int c = 0; c < C; ++c) xv[u][c] = x[(size_t)C * J[u] + c];
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-            if (k0 + u < len)
+            if (k0 + u * LANES < len)
 #pragma unroll
                 for (int r = 0; r < R; ++r)
 #pragma unroll
                     for (int c = 0; c < C; ++c) y[r] += a[u][r * C + c] * xv[u][c];
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[r] = group_sum<LANES>(y[r]);
 }
 
-// Chebyshev smoothing below the finest level, one thread per aggregate (3
+// the group's block row and lane (a group's lanes share its validity, so a
+// group leaves as a whole)
+#define BELL_ROW(LANES, rows)                                                       \
+    const size_t gid_ = (size_t)blockIdx.x * BLOCK + threadIdx.x;                  \
+    const uint32_t I = (uint32_t)(gid_ / LANES), lane = (uint32_t)(gid_ % LANES); \
+    if (I >= (rows)) return
+
+// Chebyshev smoothing below the finest level, a group per aggregate (3
 // rows).  The start (r = D^-1 b, d = r / theta, x = d) rides in the
 // restriction that writes b; the post-smoothing start takes A x.
+template <int LANES>
 __global__ __launch_bounds__(BLOCK) void k_bsm_init(Bell<3, 3> A, const float* __restrict__ dinv,
                                                     const float* __restrict__ b,
                                                     const float* __restrict__ x,
@@ -539,10 +552,10 @@ __global__ __launch_bounds__(BLOCK) void k_bsm_init(Bell<3, 3> A, const float* _
                                                     float inv_theta, const PcgState* st,
                                                     uint32_t rows) {
     if (st->done) return;
-    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
-    if (I >= rows) return;
+    BELL_ROW(LANES, rows);
     float ax[3];
-    bell_row(A, I, x, ax);
+    bell_row<3, 3, LANES>(A, I, lane, x, ax);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
         const size_t i = 3 * (size_t)I + m;
@@ -553,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_bsm_init(Bell<3, 3> A, const float* _
 }
 
 // r -= D^-1 A d_in; d_out = c1 d_in + c2 r; x += d_out (and d_in when ADD_IN)
-template <bool ADD_IN>
+template <bool ADD_IN, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_bsm_step(Bell<3, 3> A, const float* __restrict__ dinv,
                                                     const float* __restrict__ din,
                                                     float* __restrict__ dout,
@@ -561,10 +574,10 @@ __global__ __launch_bounds__(BLOCK) void k_bsm_step(Bell<3, 3> A, const float* _
                                                     float c1, float c2, const PcgState* st,
                                                     uint32_t rows) {
     if (st->done) return;
-    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
-    if (I >= rows) return;
+    BELL_ROW(LANES, rows);
     float ad[3];
-    bell_row(A, I, din, ad);
+    bell_row<3, 3, LANES>(A, I, lane, din, ad);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
         const size_t i = 3 * (size_t)I + m;
@@ -577,15 +590,16 @@ __global__ __launch_bounds__(BLOCK) void k_bsm_step(Bell<3, 3> A, const float* _
     }
 }
 
+template <int LANES>
 __global__ __launch_bounds__(BLOCK) void k_bresidual(Bell<3, 3> A, const float* __restrict__ b,
                                                      const float* __restrict__ x,
                                                      float* __restrict__ r, const PcgState* st,
                                                      uint32_t rows) {
     if (st->done) return;
-    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
-    if (I >= rows) return;
+    BELL_ROW(LANES, rows);
     float ax[3];
-    bell_row(A, I, x, ax);
+    bell_row<3, 3, LANES>(A, I, lane, x, ax);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
         const size_t i = 3 * (size_t)I + m;
@@ -593,9 +607,9 @@ __global__ __launch_bounds__(BLOCK) void k_bresidual(Bell<3, 3> A, const float* 
     }
 }
 
-// b_c = P^T r, one thread per aggregate; INIT: also the next level's
+// b_c = P^T r, a group per aggregate; INIT: also the next level's
 // smoothing start from x = 0 (r_c = D_c^-1 b_c, d_c = r_c / theta_c, x_c = d_c)
-template <int R, bool INIT>
+template <int R, bool INIT, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_brestrict(Bell<3, R> T, const float* __restrict__ r,
                                                      float* __restrict__ bc,
                                                      const float* __restrict__ dinv_c,
@@ -604,13 +618,13 @@ __global__ __launch_bounds__(BLOCK) void k_brestrict(Bell<3, R> T, const float* 
                                                      float* __restrict__ dc, float inv_theta_c,
                                                      const PcgState* st, uint32_t aggs) {
     if (st->done) return;
-    const uint32_t a = blockIdx.x * BLOCK + threadIdx.x;
-    if (a >= aggs) return;
+    BELL_ROW(LANES, aggs);
     float y[3];
-    bell_row(T, a, r, y);
+    bell_row<3, R, LANES>(T, I, lane, r, y);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        const size_t i = 3 * (size_t)a + m;
+        const size_t i = 3 * (size_t)I + m;
         bc[i] = y[m];
         if constexpr (INIT) {
             const float ri = dinv_c[i] * y[m];
@@ -622,16 +636,16 @@ __global__ __launch_bounds__(BLOCK) void k_brestrict(Bell<3, R> T, const float* 
     }
 }
 
-// x += P x_c (ADD), or x = P x_c; one thread per node
-template <int R, bool ADD>
+// x += P x_c (ADD), or x = P x_c; a group per node
+template <int R, bool ADD, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_bprolong(Bell<R, 3> P, const float* __restrict__ xc,
                                                     float* __restrict__ x, const PcgState* st,
                                                     uint32_t rows) {
     if (st->done) return;
-    const uint32_t I = blockIdx.x * BLOCK + threadIdx.x;
-    if (I >= rows) return;
+    BELL_ROW(LANES, rows);
     float y[R];
-    bell_row(P, I, xc, y);
+    bell_row<R, 3, LANES>(P, I, lane, xc, y);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < R; ++m) {
         const size_t i = (size_t)R * I + m;
@@ -732,6 +746,12 @@ static int bell_upload(uint32_t rows, Len len_of, Col col_of, BellDev& B, size_t
             bcol[sp[I / SLICE] + (size_t)k * SLICE + I % SLICE] = col_of(I, k);
     B.rows = rows;
     B.slots = sp[ns];
+    // about three blocks per lane (1, 2, 4, 8 or 16 lanes)
+    size_t used = 0;
+    for (uint32_t I = 0; I < rows; ++I) used += len[I];
+    const double avg = rows ? (double)used / rows : 0.0;
+    B.lanes = 1;
+    while (B.lanes < 16 && avg / (2 * B.lanes) >= 2.5) B.lanes *= 2;
     NBX_TRY(up(&B.slice_ptr, sp, bytes));
     NBX_TRY(up(&B.len, len, bytes));
     NBX_TRY(up(&B.bcol, bcol, bytes));
@@ -1060,28 +1080,47 @@ const double* amg_ones(const AmgDev* A) { return A->ones; }
 uint32_t amg_levels(const AmgDev* A) { return (uint32_t)A->lv.size(); }
 
 
+// f(integral_constant LANES) for a block layout's lanes per row
+template <class F>
+static void with_bl(int lanes, F&& f) {
+    switch (lanes) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 16>{}); break;
+    }
+}
+// f(integral_constant R) for a level's rows per node
+template <class F>
+static void with_bs(uint32_t bs, F&& f) {
+    if (bs == 2)
+        f(std::integral_constant<int, 2>{});
+    else
+        f(std::integral_constant<int, 3>{});
+}
+
 // b_{l+1} = P_l^T r_l, with the next level's smoothing start unless that
 // level is the coarsest
 static void restrict_to(AmgDev* A, size_t l, const float* r, const PcgState* st, hipStream_t s) {
     AmgDevLevel& V = A->lv[l];
     AmgDevLevel& W = A->lv[l + 1];
     const bool init = l + 2 < A->lv.size();
-    const unsigned g = grid_for(V.tb.rows);
-    auto go = [&](auto R) {
+    with_bs(V.bs, [&](auto R) {
         constexpr int RR = decltype(R)::value;
-        if (init)
-            hipLaunchKernelGGL((k_brestrict<RR, true>), dim3(g), dim3(BLOCK), 0, s,
-                               bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0, W.inv_theta,
-                               st, V.tb.rows);
-        else
-            hipLaunchKernelGGL((k_brestrict<RR, false>), dim3(g), dim3(BLOCK), 0, s,
-                               bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0, W.inv_theta,
-                               st, V.tb.rows);
-    };
-    if (V.bs == 2)
-        go(std::integral_constant<int, 2>{});
-    else
-        go(std::integral_constant<int, 3>{});
+        with_bl(V.tb.lanes, [&](auto ln) {
+            constexpr int LN = decltype(ln)::value;
+            const unsigned g = grid_for((size_t)V.tb.rows * LN);
+            if (init)
+                hipLaunchKernelGGL((k_brestrict<RR, true, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0,
+                                   W.inv_theta, st, V.tb.rows);
+            else
+                hipLaunchKernelGGL((k_brestrict<RR, false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<3, RR>(V.tb), r, W.b, W.dinv32, W.x, W.r, W.d0,
+                                   W.inv_theta, st, V.tb.rows);
+        });
+    });
 }
 
 // x_l (+)= P_l x_{l+1}
@@ -1089,20 +1128,19 @@ static void prolong_from(AmgDev* A, size_t l, float* x, bool add, const PcgState
                          hipStream_t s) {
     AmgDevLevel& V = A->lv[l];
     const float* xc = A->lv[l + 1].x;
-    const unsigned g = grid_for(V.pb.rows);
-    auto go = [&](auto R) {
+    with_bs(V.bs, [&](auto R) {
         constexpr int RR = decltype(R)::value;
-        if (add)
-            hipLaunchKernelGGL((k_bprolong<RR, true>), dim3(g), dim3(BLOCK), 0, s,
-                               bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
-        else
-            hipLaunchKernelGGL((k_bprolong<RR, false>), dim3(g), dim3(BLOCK), 0, s,
-                               bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
-    };
-    if (V.bs == 2)
-        go(std::integral_constant<int, 2>{});
-    else
-        go(std::integral_constant<int, 3>{});
+        with_bl(V.pb.lanes, [&](auto ln) {
+            constexpr int LN = decltype(ln)::value;
+            const unsigned g = grid_for((size_t)V.pb.rows * LN);
+            if (add)
+                hipLaunchKernelGGL((k_bprolong<RR, true, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
+            else
+                hipLaunchKernelGGL((k_bprolong<RR, false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<RR, 3>(V.pb), xc, x, st, V.pb.rows);
+        });
+    });
 }
 
 // The V-cycle from level l0 down, x = M^-1 b on that level (b, x: that
@@ -1133,15 +1171,18 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
                                    s, V.a_ptr, V.a_col, V.a32, bl, xl, V.r, st, V.n);
             });
         } else {
-            const unsigned g = grid_for(V.ab.rows);
-            for (int k = 1; k < AMG_DEGREE; ++k) {
-                hipLaunchKernelGGL((k_bsm_step<false>), dim3(g), dim3(BLOCK), 0, s,
-                                   bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
-                                   V.c2[k], st, V.ab.rows);
-                std::swap(din, dout);
-            }
-            hipLaunchKernelGGL(k_bresidual, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab), bl, xl,
-                               V.r, st, V.ab.rows);
+            with_bl(V.ab.lanes, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                const unsigned g = grid_for((size_t)V.ab.rows * LN);
+                for (int k = 1; k < AMG_DEGREE; ++k) {
+                    hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                       V.c2[k], st, V.ab.rows);
+                    std::swap(din, dout);
+                }
+                hipLaunchKernelGGL(k_bresidual<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
+                                   bl, xl, V.r, st, V.ab.rows);
+            });
         }
         restrict_to(A, l, V.r, st, s);
     }
@@ -1175,20 +1216,23 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
                 }
             });
         } else {
-            const unsigned g = grid_for(V.ab.rows);
-            hipLaunchKernelGGL(k_bsm_init, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab), V.dinv32,
-                               bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
-            for (int k = 1; k < AMG_DEGREE; ++k) {
-                if (k == 1)
-                    hipLaunchKernelGGL((k_bsm_step<true>), dim3(g), dim3(BLOCK), 0, s,
-                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
-                                       V.c2[k], st, V.ab.rows);
-                else
-                    hipLaunchKernelGGL((k_bsm_step<false>), dim3(g), dim3(BLOCK), 0, s,
-                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
-                                       V.c2[k], st, V.ab.rows);
-                std::swap(din, dout);
-            }
+            with_bl(V.ab.lanes, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                const unsigned g = grid_for((size_t)V.ab.rows * LN);
+                hipLaunchKernelGGL(k_bsm_init<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
+                                   V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
+                for (int k = 1; k < AMG_DEGREE; ++k) {
+                    if (k == 1)
+                        hipLaunchKernelGGL((k_bsm_step<true, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                           bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl,
+                                           V.c1[k], V.c2[k], st, V.ab.rows);
+                    else
+                        hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                           bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl,
+                                           V.c1[k], V.c2[k], st, V.ab.rows);
+                    std::swap(din, dout);
+                }
+            });
         }
     }
 }
