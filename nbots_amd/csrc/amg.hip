@@ -44,7 +44,6 @@ struct AmgDevLevel {
     size_t a_nnz = 0, p_nnz = 0, w_nnz = 0;
     uint32_t *a_ptr = nullptr, *a_col = nullptr;
     double* a64 = nullptr;
-    float* a32 = nullptr;      // finest level only, when it is not smoothed on the fp16 copy
     double* dinv64 = nullptr;
     float* dinv32 = nullptr;
     double lmax = 0.0;
@@ -62,7 +61,6 @@ struct AmgDevLevel {
     float *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
     // Chebyshev smoothing on [lambda/30, 1.1 lambda]
     float inv_theta = 0.f, c1[AMG_DEGREE] = {}, c2[AMG_DEGREE] = {};
-    int la = 8;                // lanes per row of the finest level's CSR smoothing
 };
 
 struct AmgDev {
@@ -76,8 +74,9 @@ struct AmgDev {
     double* partials = nullptr;     // [MAX_GRID] reduction partials
     std::vector<double> host_inv;
     size_t bytes = 0;
-    // the finest level is smoothed on the matrix's fp16 scaled copy, in
-    // scaled variables; its P values are stored as D^1/2 P (FP32)
+    // more than one level: the finest is smoothed on the matrix's fp16
+    // scaled copy, in scaled variables, and its P values are stored as
+    // D^1/2 P (FP32)
     bool scaled16 = false;
 };
 
@@ -106,12 +105,6 @@ __global__ __launch_bounds__(BLOCK) void k_amg_fine_vals(const double* __restric
                                                          double* __restrict__ a, size_t nnz) {
     const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     if (e < nnz) a[e] = kval[map[e]];
-}
-
-__global__ __launch_bounds__(BLOCK) void k_amg_to32(const double* __restrict__ in,
-                                                    float* __restrict__ out, size_t n) {
-    const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (e < n) out[e] = (float)in[e];
 }
 
 // D^-1 (0 where the diagonal is not positive: an unknown with no coarse or
@@ -398,10 +391,9 @@ __global__ __launch_bounds__(BLOCK) void k_amg_galerkin(
 }
 
 // ---- V-cycle (FP32) ----------------------------------------------------------
-// Rows are taken by groups of LANES consecutive lanes: each lane sums every
-// LANES-th entry of its row, the group adds the lanes' sums by a butterfly,
-// and the group's first lane does the row's update.  A row's entries are
-// contiguous, so a group's loads are too.  The order of the additions is
+// A group of LANES consecutive lanes takes a block row: each lane sums every
+// LANES-th block, the group adds the lanes' sums by a butterfly, and the
+// group's first lane does the row's update.  The order of the additions is
 // fixed by the pattern, so runs are reproducible.
 
 template <int LANES>
@@ -409,74 +401,6 @@ __device__ __forceinline__ float group_sum(float s) {
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     return s;
-}
-
-template <int LANES>
-__device__ __forceinline__ float csr_row(const uint32_t* __restrict__ ptr,
-                                         const uint32_t* __restrict__ col,
-                                         const float* __restrict__ a, const float* __restrict__ x,
-                                         uint32_t i, uint32_t lane, bool valid) {
-    float s = 0.f;
-    if (valid)
-        for (uint32_t e = ptr[i] + lane; e < ptr[i + 1]; e += LANES) s += a[e] * x[col[e]];
-    return group_sum<LANES>(s);
-}
-
-// the group's row and lane; `valid` is false for the groups past the last row
-// (they still take part in the butterfly, with zeros)
-#define AMG_ROW(LANES, n)                                                           \
-    const size_t gid_ = (size_t)blockIdx.x * BLOCK + threadIdx.x;                  \
-    const uint32_t i = (uint32_t)(gid_ / LANES), lane = (uint32_t)(gid_ % LANES); \
-    const bool valid = i < (n)
-
-// smoothing start: r = D^-1 (b - A x) (ZERO: x = 0, r = D^-1 b), d = r / theta;
-// ZERO also sets x = d (otherwise the first step adds d)
-template <bool ZERO, int LANES>
-__global__ __launch_bounds__(BLOCK) void k_amg_sm_init(
-    const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
-    const float* __restrict__ a, const float* __restrict__ dinv, const float* __restrict__ b,
-    float* __restrict__ x, float* __restrict__ r, float* __restrict__ d, float inv_theta,
-    const PcgState* st, uint32_t n) {
-    if (st->done) return;
-    AMG_ROW(LANES, n);
-    const float ax = ZERO ? 0.f : csr_row<LANES>(ptr, col, a, x, i, lane, valid);
-    if (!valid || lane != 0) return;
-    const float ri = dinv[i] * (b[i] - ax);
-    const float di = ri * inv_theta;
-    r[i] = ri;
-    d[i] = di;
-    if (ZERO) x[i] = di;
-}
-
-// one step: r -= D^-1 A d_in; d_out = c1 d_in + c2 r; x += d_out (and d_in
-// when ADD_IN: the first step after a start that left x alone)
-template <bool ADD_IN, int LANES>
-__global__ __launch_bounds__(BLOCK) void k_amg_sm_step(
-    const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
-    const float* __restrict__ a, const float* __restrict__ dinv, const float* __restrict__ din,
-    float* __restrict__ dout, float* __restrict__ r, float* __restrict__ x, float c1, float c2,
-    const PcgState* st, uint32_t n) {
-    if (st->done) return;
-    AMG_ROW(LANES, n);
-    const float ad = csr_row<LANES>(ptr, col, a, din, i, lane, valid);
-    if (!valid || lane != 0) return;
-    const float di = din[i];
-    const float ri = r[i] - dinv[i] * ad;
-    const float dn = c1 * di + c2 * ri;
-    r[i] = ri;
-    dout[i] = dn;
-    x[i] += ADD_IN ? di + dn : dn;
-}
-
-template <int LANES>
-__global__ __launch_bounds__(BLOCK) void k_amg_residual(
-    const uint32_t* __restrict__ ptr, const uint32_t* __restrict__ col,
-    const float* __restrict__ a, const float* __restrict__ b, const float* __restrict__ x,
-    float* __restrict__ r, const PcgState* st, uint32_t n) {
-    if (st->done) return;
-    AMG_ROW(LANES, n);
-    const float ax = csr_row<LANES>(ptr, col, a, x, i, lane, valid);
-    if (valid && lane == 0) r[i] = b[i] - ax;
 }
 
 // ---- block sliced ELL products (FP32) ---------------------------------------
@@ -672,32 +596,11 @@ __global__ __launch_bounds__(DENSE_BLOCK) void k_amg_dense(const float* __restri
 
 }  // namespace
 
-namespace {
-
-// lanes per row for a row length (entries per row on average)
-int pick_lanes(size_t entries, uint32_t rows) {
-    const double avg = rows ? (double)entries / rows : 0.0;
-    return avg <= 6.0 ? 4 : avg <= 14.0 ? 8 : avg <= 40.0 ? 16 : 32;
-}
-
-template <class F>
-void with_lanes(int lanes, F&& f) {
-    switch (lanes) {
-        case 4: f(std::integral_constant<int, 4>{}); break;
-        case 8: f(std::integral_constant<int, 8>{}); break;
-        case 16: f(std::integral_constant<int, 16>{}); break;
-        default: f(std::integral_constant<int, 32>{}); break;
-    }
-}
-
-unsigned rows_grid(uint32_t rows, int lanes) { return grid_for((size_t)rows * (size_t)lanes); }
-
-}  // namespace
 
 void amg_free(AmgDev* A) {
     if (!A) return;
     for (AmgDevLevel& L : A->lv) {
-        void* ps[] = {L.a_ptr, L.a_col, L.a64, L.a32, L.dinv64, L.dinv32, L.t, L.self, L.p_ptr,
+        void* ps[] = {L.a_ptr, L.a_col, L.a64, L.dinv64, L.dinv32, L.t, L.self, L.p_ptr,
                       L.p_col, L.p64, L.ap_map, L.w_ptr, L.w_col, L.w64, L.pt_ptr,
                       L.pt_row, L.pt_pos, L.b, L.x, L.r, L.d0, L.d1};
         for (void* p : ps)
@@ -852,7 +755,6 @@ static int amg_symbolic(FemSys& S) {
         NBX_TRY(up(&L.a_ptr, h.a_ptr, A->bytes));
         NBX_TRY(up(&L.a_col, h.a_col, A->bytes));
         NBX_TRY(alloc(&L.a64, L.a_nnz, A->bytes));
-        if (l == 0) NBX_TRY(alloc(&L.a32, L.a_nnz, A->bytes));
         if (l > 0 && l + 1 < nl) {
             const uint32_t* ap = h.a_ptr.data();
             const uint32_t* ac = h.a_col.data();
@@ -869,7 +771,6 @@ static int amg_symbolic(FemSys& S) {
             NBX_TRY(alloc(&L.b, L.n, A->bytes));
             NBX_TRY(alloc(&L.x, L.n, A->bytes));
         }
-        L.la = pick_lanes(L.a_nnz, L.n);
         if (l + 1 < H.lv.size()) {
             L.p_nnz = h.p_col.size();
             L.w_nnz = h.w_col.size();
@@ -929,12 +830,9 @@ static int amg_numeric(FemSys& S) {
         const unsigned g = grid_for(L.n);
         hipLaunchKernelGGL(k_amg_dinv, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
                            L.dinv64, L.dinv32, L.n);
-        // the finest level's smoothing reads the fp16 copy or this FP32 CSR
-        // copy; the coarser levels' operators are filled in block form by
-        // the level above
-        if (l == 0 && !A->scaled16)
-            hipLaunchKernelGGL(k_amg_to32, dim3(grid_for(L.a_nnz)), dim3(BLOCK), 0, s, L.a64,
-                               L.a32, L.a_nnz);
+        // (the finest level's smoothing reads the fp16 scaled copy; the
+        // coarser levels' operators are filled in block form by the level
+        // above)
         // lambda_max(D^-1 A): 15 power steps from a fixed start
         hipLaunchKernelGGL(k_amg_pow_init, dim3(g), dim3(BLOCK), 0, s, A->pow_x, L.n);
         hipLaunchKernelGGL(k_amg_sq, dim3(g), dim3(BLOCK), 0, s, A->pow_x, A->partials, L.n);
@@ -1046,9 +944,13 @@ int amg_prepare(FemSys& S) {
         AmgDev* A = S.K.amg;
         A->scaled16 = false;
         if (A->lv.size() > 1) {
-            const int rc = mat_make_off16(S.K, S.s);
-            if (rc != NBX_OK && rc != NBX_ERR_ARG) return rc;
-            A->scaled16 = rc == NBX_OK && S.K.has_off16 && S.K.has_scale;
+            NBX_TRY(mat_make_off16(S.K, S.s));
+            if (!S.K.has_off16 || !S.K.has_scale) {
+                set_last_error("AMG: the matrix has no fp16 Jacobi-scaled copy for the finest "
+                               "level's smoothing");
+                return NBX_ERR_ARG;
+            }
+            A->scaled16 = true;
         }
         NBX_TRY(amg_numeric(S));
         S.K.amg_current = true;
@@ -1143,10 +1045,11 @@ static void prolong_from(AmgDev* A, size_t l, float* x, bool add, const PcgState
     });
 }
 
-// The V-cycle from level l0 down, x = M^-1 b on that level (b, x: that
-// level's arrays).  Below the finest level the restriction has already
-// started each level's pre-smoothing (r, d0, x); the finest level, when it
-// is smoothed here (not on the fp16 copy), runs on its FP32 CSR copy.
+// The V-cycle from level l0 >= 1 down, x = M^-1 b on that level (b, x: that
+// level's arrays; the finest level is smoothed by the flexible CG's own
+// kernels on the fp16 copy).  The restriction has already started each
+// level's pre-smoothing (r, d0, x).  A one-level hierarchy is its dense
+// solve alone (l0 = 0).
 static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const PcgState* st,
                        hipStream_t s) {
     const size_t L = A->lv.size();
@@ -1155,35 +1058,18 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
         const float* bl = l == l0 ? b : V.b;
         float* xl = l == l0 ? x : V.x;
         float *din = V.d0, *dout = V.d1;
-        if (l == 0) {
-            hipLaunchKernelGGL((k_amg_sm_init<true, 1>), dim3(rows_grid(V.n, 1)), dim3(BLOCK), 0,
-                               s, V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl, V.r, V.d0,
-                               V.inv_theta, st, V.n);
-            with_lanes(V.la, [&](auto ln) {
-                constexpr int LN = decltype(ln)::value;
-                for (int k = 1; k < AMG_DEGREE; ++k) {
-                    hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)),
-                                       dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, din,
-                                       dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
-                    std::swap(din, dout);
-                }
-                hipLaunchKernelGGL((k_amg_residual<LN>), dim3(rows_grid(V.n, LN)), dim3(BLOCK), 0,
-                                   s, V.a_ptr, V.a_col, V.a32, bl, xl, V.r, st, V.n);
-            });
-        } else {
-            with_bl(V.ab.lanes, [&](auto ln) {
-                constexpr int LN = decltype(ln)::value;
-                const unsigned g = grid_for((size_t)V.ab.rows * LN);
-                for (int k = 1; k < AMG_DEGREE; ++k) {
-                    hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
-                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
-                                       V.c2[k], st, V.ab.rows);
-                    std::swap(din, dout);
-                }
-                hipLaunchKernelGGL(k_bresidual<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
-                                   bl, xl, V.r, st, V.ab.rows);
-            });
-        }
+        with_bl(V.ab.lanes, [&](auto ln) {
+            constexpr int LN = decltype(ln)::value;
+            const unsigned g = grid_for((size_t)V.ab.rows * LN);
+            for (int k = 1; k < AMG_DEGREE; ++k) {
+                hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                   bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                   V.c2[k], st, V.ab.rows);
+                std::swap(din, dout);
+            }
+            hipLaunchKernelGGL(k_bresidual<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab), bl,
+                               xl, V.r, st, V.ab.rows);
+        });
         restrict_to(A, l, V.r, st, s);
     }
     {
@@ -1197,49 +1083,32 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
         float* xl = l == l0 ? x : V.x;
         prolong_from(A, l, xl, true, st, s);
         float *din = V.d0, *dout = V.d1;
-        if (l == 0) {
-            with_lanes(V.la, [&](auto ln) {
-                constexpr int LN = decltype(ln)::value;
-                hipLaunchKernelGGL((k_amg_sm_init<false, LN>), dim3(rows_grid(V.n, LN)),
-                                   dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32, bl, xl,
-                                   V.r, V.d0, V.inv_theta, st, V.n);
-                for (int k = 1; k < AMG_DEGREE; ++k) {
-                    if (k == 1)
-                        hipLaunchKernelGGL((k_amg_sm_step<true, LN>), dim3(rows_grid(V.n, LN)),
-                                           dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32,
-                                           din, dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
-                    else
-                        hipLaunchKernelGGL((k_amg_sm_step<false, LN>), dim3(rows_grid(V.n, LN)),
-                                           dim3(BLOCK), 0, s, V.a_ptr, V.a_col, V.a32, V.dinv32,
-                                           din, dout, V.r, xl, V.c1[k], V.c2[k], st, V.n);
-                    std::swap(din, dout);
-                }
-            });
-        } else {
-            with_bl(V.ab.lanes, [&](auto ln) {
-                constexpr int LN = decltype(ln)::value;
-                const unsigned g = grid_for((size_t)V.ab.rows * LN);
-                hipLaunchKernelGGL(k_bsm_init<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
-                                   V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
-                for (int k = 1; k < AMG_DEGREE; ++k) {
-                    if (k == 1)
-                        hipLaunchKernelGGL((k_bsm_step<true, LN>), dim3(g), dim3(BLOCK), 0, s,
-                                           bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl,
-                                           V.c1[k], V.c2[k], st, V.ab.rows);
-                    else
-                        hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
-                                           bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl,
-                                           V.c1[k], V.c2[k], st, V.ab.rows);
-                    std::swap(din, dout);
-                }
-            });
-        }
+        with_bl(V.ab.lanes, [&](auto ln) {
+            constexpr int LN = decltype(ln)::value;
+            const unsigned g = grid_for((size_t)V.ab.rows * LN);
+            hipLaunchKernelGGL(k_bsm_init<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
+                               V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
+            for (int k = 1; k < AMG_DEGREE; ++k) {
+                if (k == 1)
+                    hipLaunchKernelGGL((k_bsm_step<true, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                       V.c2[k], st, V.ab.rows);
+                else
+                    hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
+                                       bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
+                                       V.c2[k], st, V.ab.rows);
+                std::swap(din, dout);
+            }
+        });
     }
 }
 
-// z = V-cycle(r) on the finest level; b and x are FP32 arrays of the
-// finest level's length
+// z = C^-1 r for a one-level hierarchy (the finest level is the coarsest)
 int amg_vcycle(AmgDev* A, const float* b, float* x, const PcgState* st, hipStream_t s) {
+    if (A->lv.size() != 1) {
+        set_last_error("AMG: the V-cycle of a multi-level hierarchy runs on the fp16 copy");
+        return NBX_ERR_ARG;
+    }
     cycle_from(A, 0, b, x, st, s);
     NBX_HIP(hipGetLastError());
     return NBX_OK;

@@ -130,15 +130,16 @@ struct MixedOpts {
 int ws_alloc(Workspace& w, uint32_t n);
 
 void amg_free(AmgDev* a);
-// one V-cycle z = M^-1 r on the finest level (FP32 arrays of its length);
-// returns at once per kernel when st->done
+// z = M^-1 r for a one-level hierarchy (at most 200 unknowns: the dense
+// coarsest solve alone; FP32 arrays of its length); returns at once when
+// st->done
 int amg_vcycle(AmgDev* a, const float* r, float* z, const PcgState* st, hipStream_t s);
 const double* amg_ones(const AmgDev* a);   // [n] 1.0: the flexible CG's unit scaling
 uint32_t amg_levels(const AmgDev* a);
 // The finest level smoothed on the matrix's fp16 Jacobi-scaled copy (the
 // Chebyshev preconditioner's kernels, in scaled variables): its Chebyshev
 // coefficients and the vector for the coarse correction; `scaled16` false
-// when the hierarchy smooths every level on its own FP32 CSR copy.
+// for a one-level hierarchy (amg_vcycle).
 struct AmgFine {
     bool scaled16 = false;
     double inv_theta = 0.0;

@@ -1236,8 +1236,9 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
         hipLaunchKernelGGL(k_fcg_p<float2>, dim3(G), dim3(BLOCK), 0, s, z, dis, p, w.st, nu, first);
     };
     auto precondition_amg = [&](int first) {
-        // the update with a unit scaling writes r in FP32 (res), the V-cycle
-        // z = M^-1 r, the dot products and direction read z unscaled
+        // a one-level hierarchy: the update with a unit scaling writes r in
+        // FP32 (res), the dense solve z = C^-1 r, the dot products and
+        // direction read z unscaled
         hipLaunchKernelGGL(k_fcg_update<float2>, dim3(G), dim3(BLOCK), 0, s, m.s, bv, xv, r, p, wv,
                            ones, res, da, z, w.st, w.partials, nu, first, 1.0);
         hipLaunchKernelGGL(k_fin_fcg_update, dim3(1), dim3(FIN), 0, s, w.st, w.partials, G, first);
