@@ -332,9 +332,9 @@ typedef struct {
 } nbx_solve_opts_t;
 /* Fills *opts with the defaults above. */
 void nbx_solve_opts_default(nbx_solve_opts_t *opts);
-/* Fills *opts with the SIMP loop's defaults (nbx_simp_iterate); at present
- * the same as nbx_solve_opts_default (degree 40 on [lambda_max/3e4,
- * lambda_max] is the best point swept for both).                        */
+/* Fills *opts with the SIMP loop's defaults (nbx_simp_iterate): the
+ * multigrid-preconditioned variant NBX_MIXED_AMG_FCG, the Chebyshev fields
+ * as nbx_solve_opts_default (degree 40 on [lambda_max/3e4, lambda_max]).  */
 void nbx_simp_opts_default(nbx_solve_opts_t *opts);
 
 typedef struct {
@@ -509,10 +509,10 @@ void nbx_simp_destroy(nbx_simp_t *simp);
 /* One iteration: reassemble with E(xPhys), re-apply the boundary
  * conditions, solve (mode, relative tol, warm start from the previous
  * displacement), compliance of the current design, sensitivities, filter
- * and OC update.  change = max |x_new - x|.  opts NULL, or Chebyshev
- * fields left at 0, take the SIMP defaults (nbx_simp_opts_default): the
- * void elements (E = Emin) leave K far worse conditioned than a solid
- * plate.                                                                */
+ * and OC update.  change = max |x_new - x|.  opts NULL takes the SIMP
+ * defaults (nbx_simp_opts_default: multigrid-preconditioned mixed solves),
+ * and Chebyshev fields left at 0 take theirs: the void elements (E = Emin)
+ * leave K far worse conditioned than a solid plate.                      */
 int nbx_simp_iterate(nbx_simp_t *simp, int mode, double tol,
 		     uint32_t max_iter, const nbx_solve_opts_t *opts,
 		     double *compliance, double *change,

@@ -45,13 +45,18 @@ inline int mixed_opts(const nbx_solve_opts_t* p, MixedOpts* o, const MixedOpts& 
     return NBX_OK;
 }
 
-// Defaults of the SIMP loop's mixed solves.  Its systems carry void
-// elements (E = Emin = 1e-9 E0); the plate defaults (degree 40 on
-// [lmax/3e4, lmax]) are also the best point swept there (1.05M-DOF MBB
-// beam, 25 iterations: 261 ms per iteration; degree 32: 265, degree 48 on
-// [lmax/1e5, lmax]: 263, degree 24: 271).
+// Defaults of the SIMP loop's mixed solves: the multigrid-preconditioned
+// flexible CG (1.05M-DOF MBB beam, 50 iterations: 45 ms per iteration
+// against 231 with the Chebyshev preconditioner).  Its systems carry void
+// elements (E = Emin = 1e-9 E0) that spread the spectrum, which a
+// polynomial pays for in products and a multigrid hierarchy much less.
+// The Chebyshev fields keep the plate defaults (degree 40 on [lmax/3e4,
+// lmax], also the best point swept on the beam: 261 ms per iteration over
+// 25; degree 32: 265, degree 48 on [lmax/1e5, lmax]: 263, degree 24: 271)
+// for a caller that selects that variant.
 inline MixedOpts simp_mixed_defaults() {
     MixedOpts o;
+    o.variant = AMG_FCG;
     o.cheb_degree = 40;
     o.cheb_ratio = 3e4;
     return o;

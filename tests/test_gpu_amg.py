@@ -134,6 +134,24 @@ def test_amg_simp_history_matches_fp64():
     assert rel.max() <= 1e-6, rel.max()
 
 
+def test_simp_default_solver_is_amg():
+    """nbx_simp_iterate without options (and nbx_simp_opts_default) take the
+    multigrid-preconditioned mixed solve."""
+    from nbots_amd import synth
+    from nbots_amd.fem import Fem, Simp, SOLVE_MIXED
+    from nbots_amd.lib import SolveOpts, load
+    o = SolveOpts()
+    load().nbx_simp_opts_default(o)
+    assert o.mixed_variant == 3
+    f = Fem.from_problem(synth.mbb(48, 24, rho=0.5))
+    s = Simp(f, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2)
+    for _ in range(2):
+        C, change, info = s.iterate(SOLVE_MIXED, tol=1e-8)
+        assert info["status"] == 0 and info["layout"] & 16, info
+    s.destroy()
+    f.destroy()
+
+
 def test_amg_refused_without_coordinates():
     import ctypes as C
     import oracle

@@ -49,24 +49,26 @@ def test_replay_matches_eager_launches(mode, kw):
 
 def test_replay_simp_history_matches_eager():
     from nbots_amd import synth
-    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED, solve_opts
+    from nbots_amd.fem import Fem, Simp, SOLVE_FP64, SOLVE_MIXED, MIXED_AMG_FCG, solve_opts
     p = synth.mbb(96, 48, rho=0.5)
     hist = {}
+    cases = ((SOLVE_MIXED, {}), (SOLVE_FP64, {}), ("amg", {"variant": MIXED_AMG_FCG}))
     for eager in (True, False):
-        for mode, extra in ((SOLVE_MIXED, {}), (SOLVE_FP64, {})):
+        for mode, extra in cases:
             print(f"[replay] SIMP mode={mode} eager={eager} 96x48 MBB", flush=True)
             f = Fem.from_problem(p)
             s = Simp(f, rmin=1.5, volfrac=0.5, penal=3.0, E0=1.0, Emin=1e-9, move=0.2)
             o = solve_opts(simp=True, true_residual=True, replay=not eager, **extra)
             h = []
             for _ in range(4):
-                C, change, info = s.iterate(mode, tol=1e-8, opts=o)
+                C, change, info = s.iterate(SOLVE_MIXED if mode == "amg" else mode, tol=1e-8,
+                                            opts=o)
                 h.append((C, change, info["iters"], info["rel_res"]))
             x, xp, dc, _ = s.state()
             hist[(mode, eager)] = (h, x, xp)
             s.destroy()
             f.destroy()
-    for mode in (SOLVE_MIXED, SOLVE_FP64):
+    for mode, _ in cases:
         he, xe, xpe = hist[(mode, True)]
         hr, xr, xpr = hist[(mode, False)]
         assert he == hr
