@@ -963,12 +963,12 @@ AmgFine amg_fine(const AmgDev* A) {
     if (!A || !A->scaled16 || A->lv.size() < 2) return f;
     const AmgDevLevel& L = A->lv[0];
     f.scaled16 = true;
-    f.degree = AMG_DEGREE;
+    f.degree = AMG_FINE_DEGREE;
     const double hi = 1.1 * L.lmax, lo = L.lmax / 30.0;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho = 1.0 / sigma;
     f.inv_theta = 1.0 / theta;
-    for (int k = 1; k < AMG_DEGREE; ++k) {
+    for (int k = 1; k < AMG_FINE_DEGREE; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
         f.c1[k] = rn * rho;
         f.c2[k] = 2.0 * rn / delta;

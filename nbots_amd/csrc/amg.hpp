@@ -14,8 +14,10 @@
 //   * P = (I - omega D^-1 A) T with omega = 4 / (3 lambda_max(D^-1 A));
 //   * A_c = P^T A P, recursively, until at most AMG_COARSEST unknowns remain,
 //     whose matrix is inverted (dense, FP64) and applied as a dense product;
-//   * one V-cycle: Chebyshev smoothing in D^-1 A of degree AMG_DEGREE on
-//     [lambda/30, 1.1 lambda] before and after the coarse correction.
+//   * one V-cycle: Chebyshev smoothing in D^-1 A on [lambda/30, 1.1 lambda]
+//     before and after the coarse correction, of degree AMG_DEGREE below the
+//     finest level and AMG_FINE_DEGREE on it (degree 1: one Jacobi step
+//     weighted 1/theta, theta the interval's centre).
 // Everything is scalar CSR.  The aggregation, T and every pattern depend on
 // the matrix's graph, its Dirichlet rows and the node coordinates only, so
 // they are built once on the host (amg_setup.cpp); P, A_c and the
@@ -30,9 +32,15 @@ namespace nbx {
 
 constexpr uint32_t AMG_NONE = 0xffffffffu;
 constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and after
+// ... on the finest level, whose products are the cycle's largest: with
+// degree 1 the solves take as many outer iterations as with 3 (4.2M-DOF
+// cantilever: 64 against 66; 1.05M-DOF SIMP beam: 94-95 against 90 per
+// solve) for a third of the finest products (C3 solve 0.055 -> 0.046 s)
+constexpr int AMG_FINE_DEGREE = 1;
 constexpr uint32_t AMG_COARSEST = 200; // unknowns of the directly solved level
 constexpr int AMG_MAX_LEVELS = 12;
-static_assert(AMG_DEGREE <= 8, "AmgFine holds the coefficients of at most 8 steps");
+static_assert(AMG_DEGREE <= 8 && AMG_FINE_DEGREE >= 1 && AMG_FINE_DEGREE <= 8,
+              "AmgFine holds the coefficients of at most 8 steps");
 
 // One level of the host-built symbolic hierarchy.
 struct AmgHostLevel {

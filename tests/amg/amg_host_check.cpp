@@ -55,15 +55,17 @@ void spmv(const AmgHostLevel& L, const std::vector<double>& a, const std::vector
 }
 
 struct Cheb {
-    double theta, c1[AMG_DEGREE], c2[AMG_DEGREE];
+    int degree;
+    double theta, c1[8], c2[8];
 };
-Cheb cheb(double lmax) {
+Cheb cheb(double lmax, int degree) {
     Cheb c;
+    c.degree = degree;
     const double hi = 1.1 * lmax, lo = lmax / 30.0;
     c.theta = 0.5 * (hi + lo);
     const double delta = 0.5 * (hi - lo), sigma = c.theta / delta;
     double rho = 1.0 / sigma;
-    for (int k = 1; k < AMG_DEGREE; ++k) {
+    for (int k = 1; k < degree; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
         c.c1[k] = rn * rho;
         c.c2[k] = 2.0 * rn / delta;
@@ -72,9 +74,10 @@ Cheb cheb(double lmax) {
     return c;
 }
 
-void smooth(const AmgHostLevel& L, const Num& N, const std::vector<double>& b, std::vector<double>& x,
-            bool zero) {
-    const Cheb c = cheb(N.lmax);
+// degree AMG_FINE_DEGREE on the finest level (l = 0), AMG_DEGREE below
+void smooth(const AmgHostLevel& L, const Num& N, size_t l, const std::vector<double>& b,
+            std::vector<double>& x, bool zero) {
+    const Cheb c = cheb(N.lmax, l == 0 ? AMG_FINE_DEGREE : AMG_DEGREE);
     std::vector<double> r(L.n), d(L.n), y(L.n);
     if (!zero) spmv(L, N.a, x, y);
     for (uint32_t i = 0; i < L.n; ++i) {
@@ -82,7 +85,7 @@ void smooth(const AmgHostLevel& L, const Num& N, const std::vector<double>& b, s
         d[i] = r[i] / c.theta;
         x[i] = (zero ? 0.0 : x[i]) + d[i];
     }
-    for (int k = 1; k < AMG_DEGREE; ++k) {
+    for (int k = 1; k < c.degree; ++k) {
         spmv(L, N.a, d, y);
         for (uint32_t i = 0; i < L.n; ++i) {
             r[i] -= N.dinv[i] * y[i];
@@ -104,7 +107,7 @@ void vcycle(const AmgHost& H, const std::vector<Num>& N, size_t l, const std::ve
         }
         return;
     }
-    smooth(L, N[l], b, x, true);
+    smooth(L, N[l], l, b, x, true);
     std::vector<double> res(L.n), y(L.n);
     spmv(L, N[l].a, x, y);
     for (uint32_t i = 0; i < L.n; ++i) res[i] = b[i] - y[i];
@@ -120,7 +123,7 @@ void vcycle(const AmgHost& H, const std::vector<Num>& N, size_t l, const std::ve
         for (uint32_t q = L.p_ptr[i]; q < L.p_ptr[i + 1]; ++q) s += N[l].p[q] * xc[L.p_col[q]];
         x[i] += s;
     }
-    smooth(L, N[l], b, x, false);
+    smooth(L, N[l], l, b, x, false);
 }
 
 double dot(const std::vector<double>& a, const std::vector<double>& b) {
