@@ -13,7 +13,10 @@
 // Triangles of any size: volume V = sum_e a_e xPhys_e (a = element area), so
 // dv_e = a_e.  Every filter sum runs over a row's entries in ascending
 // column order with separate multiply and add (-ffp-contract=off), so H x
-// and H (dc ./ Hs) are bit-identical with a sequential CPU loop.
+// and H (dc ./ Hs) are bit-identical with a sequential CPU loop.  The
+// bisection tests V through the identity V = x_new . H (a ./ Hs) (H is
+// symmetric), which needs no filter pass per step; only the final design
+// is filtered.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,38 +70,52 @@ __global__ void k_oc_init(OcState* st, double target) {
     *st = s;
 }
 
-// candidate design for the current multiplier
-__global__ void k_oc_x(const OcState* __restrict__ st, const double* __restrict__ x,
-                       const double* __restrict__ dc, const double* __restrict__ dv,
-                       double move, uint32_t ne, double* __restrict__ xnew) {
-    if (st->done) return;
-    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= ne) return;
-    const double lmid = st->lmid;
-    const double xe = x[e];
-    double v = xe * sqrt(-dc[e] / dv[e] / lmid);
+// the OC candidate of one element for the multiplier lmid
+__device__ __forceinline__ double oc_candidate(double xe, double dce, double dve, double lmid,
+                                               double move) {
+    double v = xe * sqrt(-dce / dve / lmid);
     if (v > xe + move) v = xe + move;
     if (v > 1.0) v = 1.0;
     if (v < xe - move) v = xe - move;
     if (v < 0.0) v = 0.0;
-    xnew[e] = v;
+    return v;
 }
 
-// its filtered densities and per-workgroup partial volumes
-__global__ __launch_bounds__(BLOCK) void k_oc_vol(const OcState* __restrict__ st, DevFilter H,
-                                                  const double* __restrict__ xnew,
-                                                  const double* __restrict__ area,
-                                                  double* __restrict__ xphys, double* partials) {
+// One bisection step: the filtered volume of the candidate design,
+// sum_e a_e (H x_new)_e / Hs_e, which is x_new . (H (a ./ Hs)) = x_new . dv
+// because H is symmetric; so a step reads x, dc and dv once and filters
+// nothing (per-workgroup partials).
+__global__ __launch_bounds__(BLOCK) void k_oc_step(const OcState* __restrict__ st,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ dc,
+                                                   const double* __restrict__ dv, double move,
+                                                   uint32_t ne, double* partials) {
     if (st->done) return;   // uniform over the grid
+    const double lmid = st->lmid;
     double acc[1] = {0.0};
-    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < H.ne; e += gridDim.x * BLOCK) {
-        const double xp = filt_row<false>(H, e, xnew) / H.Hs[e];
-        xphys[e] = xp;
-        acc[0] += area[e] * xp;
+    for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < ne; e += gridDim.x * BLOCK) {
+        const double dve = dv[e];
+        acc[0] += dve * oc_candidate(x[e], dc[e], dve, lmid, move);
     }
     __shared__ double red[1][BLOCK / WAVE];
     block_sum<1>(acc, red);
     put_partials<1>(acc, partials);
+}
+
+// the design of the last multiplier the bisection tried, and its filtered
+// densities
+__global__ void k_oc_x(const OcState* __restrict__ st, const double* __restrict__ x,
+                       const double* __restrict__ dc, const double* __restrict__ dv,
+                       double move, uint32_t ne, double* __restrict__ xnew) {
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= ne) return;
+    xnew[e] = oc_candidate(x[e], dc[e], dv[e], st->lmid, move);
+}
+__global__ void k_oc_phys(DevFilter H, const double* __restrict__ xnew,
+                          double* __restrict__ xphys) {
+    const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= H.ne) return;
+    xphys[e] = filt_row<false>(H, e, xnew) / H.Hs[e];
 }
 
 // volume test, next bracket and multiplier
@@ -364,10 +381,8 @@ int simp_update(Simp& o, FemSys& S, double* change) {
             return NBX_ERR_BREAKDOWN;
         }
         for (int k = 0; k < BATCH; ++k) {
-            hipLaunchKernelGGL(k_oc_x, dim3(grid_for(ne)), dim3(BLOCK), 0, s, o.st, o.x, o.dcf, o.dvf,
-                               o.move, ne, o.xnew);
-            hipLaunchKernelGGL(k_oc_vol, dim3(g), dim3(BLOCK), 0, s, o.st, H, o.xnew, o.area, o.xphys,
-                               S.ws.partials);
+            hipLaunchKernelGGL(k_oc_step, dim3(g), dim3(BLOCK), 0, s, o.st, o.x, o.dcf, o.dvf,
+                               o.move, ne, S.ws.partials);
             hipLaunchKernelGGL(k_oc_fin, dim3(1), dim3(FIN), 0, s, o.st, S.ws.partials, g);
         }
         NBX_HIP(hipGetLastError());
@@ -376,6 +391,9 @@ int simp_update(Simp& o, FemSys& S, double* change) {
         if (h.done) break;
     }
     o.last_oc_steps = h.steps;
+    hipLaunchKernelGGL(k_oc_x, dim3(grid_for(ne)), dim3(BLOCK), 0, s, o.st, o.x, o.dcf, o.dvf,
+                       o.move, ne, o.xnew);
+    hipLaunchKernelGGL(k_oc_phys, dim3(grid_for(ne)), dim3(BLOCK), 0, s, H, o.xnew, o.xphys);
     hipLaunchKernelGGL(k_absdiff_max, dim3(g), dim3(BLOCK), 0, s, o.xnew, o.x, ne, S.ws.partials);
     hipLaunchKernelGGL(k_fin_absmax, dim3(1), dim3(FIN), 0, s, S.ws.partials, g, o.scal);
     NBX_HIP(hipGetLastError());
