@@ -263,6 +263,25 @@ def test_filter_sens_is_the_transpose(name):
         assert abs(bad - rhs) > 1e-6 * abs(rhs)     # the planted mistake fails
 
 
+@pytest.mark.parametrize("name", ["mbb_16x8", "jittered_24x12"])
+def test_filtered_volume_identity(name):
+    """The device's OC bisection tests the filtered volume of a candidate
+    design as x_new . H (a ./ Hs) instead of sum_e a_e (H x_new)_e / Hs_e
+    (no filter pass per step): the transpose identity above with g = a.
+    Candidates from the OC formula itself, at several multipliers."""
+    p, H = _filter_case(name)
+    area = synth.elem_areas(p)
+    rng = np.random.default_rng(11)
+    n = area.size
+    x = rng.uniform(0.05, 1.0, n)
+    dc = -rng.uniform(0.1, 10.0, n)
+    dv = oracle.filter_sens(H, area)
+    for lmid in (0.3, 1.0, 7.0):
+        xn = np.clip(np.clip(x * np.sqrt(-dc / dv / lmid), x - 0.2, x + 0.2), 0.0, 1.0)
+        v_filter = area @ oracle.filter_apply(H, xn)
+        assert abs(xn @ dv - v_filter) <= 1e-13 * v_filter
+
+
 def _mbb_compliance(x, H, nx=16, ny=8, penal=3.0, E0=1.0, Emin=1e-9):
     """C(x) = F.u of the MBB beam with E(xPhys), xPhys = filter_apply(x); K
     and F from the oracle's assembly and BCs, u from a dense solve.  Also
