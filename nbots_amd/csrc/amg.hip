@@ -619,8 +619,8 @@ void amg_free(AmgDev* A) {
 
 // The block sliced-ELL pattern of `rows` block rows: len_of(I) blocks in
 // row I, col_of(I, k) the k-th block's column; uploaded with room for R x C
-// FP32 values per slot.  False when a row has more blocks than the format
-// holds.
+// FP32 values per slot.  NBX_ERR_ARG when a row has more blocks than the
+// format holds (65,535) or the layout more than 2^32 slots.
 template <int R, int C, class Len, class Col>
 static int bell_upload(uint32_t rows, Len len_of, Col col_of, BellDev& B, size_t& bytes) {
     const uint32_t ns = (rows + SLICE - 1) / SLICE;

@@ -18,10 +18,13 @@
 //     before and after the coarse correction, of degree AMG_DEGREE below the
 //     finest level and AMG_FINE_DEGREE on it (degree 1: one Jacobi step
 //     weighted 1/theta, theta the interval's centre).
-// Everything is scalar CSR.  The aggregation, T and every pattern depend on
-// the matrix's graph, its Dirichlet rows and the node coordinates only, so
-// they are built once on the host (amg_setup.cpp); P, A_c and the
-// coarsest inverse follow the values on the device whenever K changes.
+// The host builds every pattern as scalar CSR, grouped by node (a node's
+// rows share one pattern of complete column groups).  The aggregation, T
+// and every pattern depend on the matrix's graph, its Dirichlet rows and the
+// node coordinates only, so they are built once on the host
+// (amg_setup.cpp); P, A_c and the coarsest inverse follow the values on the
+// device whenever K changes (amg.hip, which also keeps the V-cycle's block
+// copies).
 #pragma once
 
 #include <stdint.h>
