@@ -396,8 +396,9 @@ def amg_solve(f, t_fp64, u64, Fl):
                  vs_fp64_disp_rel_l2=float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
                  vs_fp64_compliance_rel=abs(float(Fl @ u) - float(Fl @ u64)) / abs(float(Fl @ u64)),
                  solver=("FP64 flexible CG; preconditioner one FP32 V-cycle of smoothed-"
-                         "aggregation multigrid (rigid-body modes, Chebyshev-Jacobi smoothing "
-                         "of degree 3, dense coarsest level); setup = first solve - solve")))
+                         "aggregation multigrid (rigid-body modes; one weighted Jacobi step on "
+                         "the finest level's fp16 scaled copy, degree-3 Chebyshev-Jacobi "
+                         "smoothing below; dense coarsest level); setup = first solve - solve")))
 
 
 def decompose(f, args, t_fp64, iters64, u64, Fl):
