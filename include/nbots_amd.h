@@ -512,7 +512,9 @@ void nbx_simp_destroy(nbx_simp_t *simp);
  * and OC update.  change = max |x_new - x|.  opts NULL takes the SIMP
  * defaults (nbx_simp_opts_default: multigrid-preconditioned mixed solves),
  * and Chebyshev fields left at 0 take theirs: the void elements (E = Emin)
- * leave K far worse conditioned than a solid plate.                      */
+ * leave K far worse conditioned than a solid plate.  max_iter 0: 20 N
+ * iterations, 5,000 outer iterations for the multigrid-preconditioned
+ * solve.                                                                  */
 int nbx_simp_iterate(nbx_simp_t *simp, int mode, double tol,
 		     uint32_t max_iter, const nbx_solve_opts_t *opts,
 		     double *compliance, double *change,

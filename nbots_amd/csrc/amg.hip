@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <new>
 #include <type_traits>
 #include <vector>
 
@@ -700,7 +701,12 @@ static int amg_symbolic(FemSys& S) {
     NBX_TRY(fem_download_csr(S, rp.data(), col.data(), val.data()));
     NBX_HIP(copy_on(xy.data(), S.nod, xy.size() * sizeof(double), hipMemcpyDeviceToHost, S.s));
     AmgHost H;
-    if (amg_build_host(n, rp.data(), col.data(), val.data(), xy.data(), H)) {
+    const int hb = amg_build_host(n, rp.data(), col.data(), val.data(), xy.data(), H);
+    if (hb == -2) {
+        set_last_error("AMG: host memory ran out building the hierarchy");
+        return NBX_ERR_ARG;
+    }
+    if (hb) {
         set_last_error("AMG: the matrix is not a 2-dof node pattern");
         return NBX_ERR_ARG;
     }
@@ -934,7 +940,16 @@ static int amg_numeric(FemSys& S) {
 
 int amg_prepare(FemSys& S) {
     if (!S.K.amg || !S.amg_symbolic_current) {
-        NBX_TRY(amg_symbolic(S));
+        // (the host arrays of a large system can exhaust host memory: an
+        // error, not an exception through the C interface)
+        int rc;
+        try {
+            rc = amg_symbolic(S);
+        } catch (const std::bad_alloc&) {
+            set_last_error("AMG: host memory ran out building the hierarchy");
+            rc = NBX_ERR_ARG;
+        }
+        if (rc != NBX_OK) return rc;
         S.amg_symbolic_current = true;
     }
     if (!S.K.amg_current) {

@@ -67,8 +67,9 @@ struct AmgHost {
 // Build the symbolic hierarchy from the finest operator: CSR pattern
 // (rowptr[n+1], col ascending per row) with its values (to find the
 // Dirichlet-eliminated unknowns: rows whose off-diagonal values are all 0)
-// and the coordinates of the n/2 nodes (x, y interleaved).  Returns 0, or
-// -1 when the pattern is not a 2-dof node pattern.
+// and the coordinates of the n/2 nodes (x, y interleaved).  Returns 0, -1
+// when the pattern is not a 2-dof node pattern, or -2 when host memory ran
+// out (the setup runs on up to 16 threads).
 int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
                    const double* xy, AmgHost& out);
 

@@ -6,6 +6,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <thread>
 
 #include "amg.hpp"
@@ -25,19 +27,39 @@ unsigned setup_threads(uint32_t items) {
     return std::max(1u, std::min(t, items / 4096u));
 }
 
+// Runs body(t) for t in [0, nt) on nt threads (the caller's among them);
+// an exception in any of them (out of host memory) is rethrown here after
+// all have joined.
+template <class B>
+void run_threads(unsigned nt, B body) {
+    if (nt <= 1) {
+        body(0u);
+        return;
+    }
+    std::vector<std::exception_ptr> err(nt);
+    auto guarded = [&](unsigned t) {
+        try {
+            body(t);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(guarded, t);
+    guarded(0);
+    for (std::thread& x : th) x.join();
+    for (const std::exception_ptr& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 // f(lo, hi) over contiguous ranges of [0, n), one per thread.
 template <class F>
 void par_for(uint32_t n, F f) {
     const unsigned nt = setup_threads(n);
-    if (nt <= 1) {
-        f(0u, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    th.reserve(nt);
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back(f, (uint32_t)((uint64_t)n * t / nt), (uint32_t)((uint64_t)n * (t + 1) / nt));
-    for (std::thread& x : th) x.join();
+    run_threads(nt, [&](unsigned t) {
+        f((uint32_t)((uint64_t)n * t / nt), (uint32_t)((uint64_t)n * (t + 1) / nt));
+    });
 }
 
 // Per-item lists built in parallel: f(i, out) appends item i's list to
@@ -57,13 +79,7 @@ void build_lists(uint32_t n, F f, std::vector<uint32_t>& ptr, std::vector<uint32
             len[i] = (uint32_t)(out.size() - before);
         }
     };
-    if (nt == 1) {
-        fill(0);
-    } else {
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < nt; ++t) th.emplace_back(fill, t);
-        for (std::thread& x : th) x.join();
-    }
+    run_threads(nt, fill);
     ptr.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + len[i];
     data.resize(ptr[n]);
@@ -342,8 +358,8 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
 
 }  // namespace
 
-int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
-                   const double* xy, AmgHost& out) {
+static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+                      const double* xy, AmgHost& out) {
     out.lv.clear();
     if (n == 0 || n % 2) return -1;
     AmgHostLevel L0;
@@ -401,6 +417,16 @@ int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, cons
         v = nullptr;   // below the finest level every structural entry counts
     }
     return 0;
+}
+
+int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+                   const double* xy, AmgHost& out) {
+    try {
+        return build_host(n, rowptr, col, val, xy, out);
+    } catch (const std::bad_alloc&) {
+        out.lv.clear();
+        return -2;
+    }
 }
 
 int amg_dense_inverse(uint32_t n, double* a) {

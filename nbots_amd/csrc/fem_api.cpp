@@ -363,7 +363,12 @@ int nbx_simp_iterate(nbx_simp_t* h, int mode, double tol, uint32_t max_iter,
     NBX_TRY(nbx::simp_moduli(h->o, S));
     NBX_TRY(nbx::fem_assemble_device(S));
     NBX_TRY(nbx::fem_reapply_bc(S));
-    const int rc = fem_solve(S, mode, tol, max_iter ? max_iter : 20 * 2 * S.nn, mo, info);
+    // max_iter 0: 20 N iterations, or 5,000 outer iterations of the
+    // multigrid-preconditioned solve (which takes tens; a cap in N would let a
+    // solve that has stopped progressing run for minutes)
+    const bool amg = mode == NBX_SOLVE_MIXED && mo.variant == nbx::AMG_FCG;
+    const uint32_t cap = max_iter ? max_iter : amg ? 5000u : 20 * 2 * S.nn;
+    const int rc = fem_solve(S, mode, tol, cap, mo, info);
     if (rc > 1) return rc;
     double C = 0.0;
     NBX_TRY(nbx::fem_compliance(S, h->o.xphys, h->o.penal, h->o.E0, h->o.Emin, S.ce, S.dc, &C));
