@@ -7,6 +7,9 @@ wrote (tests/golden/c3_reference_solve.npz) and against the FP64 runs:
   in FP64 iterative refinement, each to a true relative residual 1e-8,
   reach the reference's FP64 displacement within rel 1e-7 and the FP64
   solve's compliance within 1e-9;
+* 1024x512 lognormal cantilever (1,051,650 DOF): the compiled reference's
+  Jacobi-PCG run on the GPU box's host, and every device solver within 1e-7 of
+  its whole displacement (rel-L2);
 * the 1024x512 MBB half-beam SIMP loop (1,051,650 DOF), 50 iterations: the
   compliance histories with mixed-precision solves (Chebyshev and
   multigrid preconditioners) and with FP64 solves agree within 1e-6 and the
@@ -51,6 +54,46 @@ def test_c3_mixed_against_reference(c3, variant):
     check_against_reference_solve(u, F, g, 1e-7)
     assert abs(float(F @ u) - float(F @ u64)) <= 1e-9 * abs(float(F @ u64))
     assert np.linalg.norm(u - u64) <= 1e-7 * np.linalg.norm(u64)
+
+
+def test_c2_full_displacement_against_compiled_reference():
+    """1024x512 lognormal cantilever (1,051,650 DOF): the compiled
+    reference's own Jacobi-PCG (its stop test, tol 1e-8 ||F||, OpenMP on up
+    to 16 host threads, about 12 s) computes the whole displacement here, on
+    the GPU box, and every device solver meets it in full rel-L2 within 1e-7,
+    not on samples: FP64 PCG, the Chebyshev mixed solve, the multigrid-
+    preconditioned solve and FP32 CG in FP64 refinement, each to a true
+    relative residual of 1e-8."""
+    import os
+    import oracle
+    from nbots_amd import synth
+    from nbots_amd.fem import (Fem, SOLVE_FP64, SOLVE_MIXED, MIXED_CHEB_FCG, MIXED_AMG_FCG,
+                               MIXED_CG_RESTART, solve_opts)
+    if not oracle.ref_available():
+        pytest.skip("the compiled reference (oracle/_ref) is not built")
+    p = synth.cantilever_traction(1024, 512)
+    f = Fem.from_problem(p)
+    rp, col, val, F = f.system()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    print(f"[c2] compiled reference PCG on {threads} threads", flush=True)
+    x_ref, it, _, st = oracle.ref_pcg(rp, col, val, F, tol=1e-8 * float(np.linalg.norm(F)),
+                                      max_iter=10 * p.N, threads=threads)
+    assert st == 0 and 11000 < it < 12500, (st, it)
+    nref = np.linalg.norm(x_ref)
+    cases = (("fp64", SOLVE_FP64, solve_opts(true_residual=True), 200000),
+             ("cheb_fcg", SOLVE_MIXED, solve_opts(MIXED_CHEB_FCG, true_residual=True), 200000),
+             ("amg_fcg", SOLVE_MIXED, solve_opts(MIXED_AMG_FCG, true_residual=True), 2000),
+             ("fp32_ir", SOLVE_MIXED, solve_opts(MIXED_CG_RESTART, inner_rtol=1e-4,
+                                                  true_residual=True), 200000))
+    for name, mode, o, mi in cases:
+        print(f"[c2] {name}", flush=True)
+        info = f.solve(mode, tol=1e-8, max_iter=mi, opts=o)
+        assert info["status"] == 0, (name, info)
+        rel = np.linalg.norm(f.displacement() - x_ref) / nref
+        print(f"[c2] {name}: {info['iters']} iterations, rel-L2 to the reference {rel:.2e}",
+              flush=True)
+        assert rel <= 1e-7, (name, rel)
+    f.destroy()
 
 
 def test_c5_simp_history_mixed_vs_fp64():
