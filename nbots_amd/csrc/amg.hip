@@ -62,6 +62,7 @@ struct AmgDevLevel {
     float *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
     // Chebyshev smoothing on [lambda/30, 1.1 lambda]
     float inv_theta = 0.f, c1[AMG_DEGREE] = {}, c2[AMG_DEGREE] = {};
+    int degree = AMG_DEGREE;   // smoothing steps before and after (amg_level_degree)
 };
 
 struct AmgDev {
@@ -757,6 +758,7 @@ static int amg_symbolic(FemSys& S) {
         L.n = h.n;
         L.nc = h.nc;
         L.bs = l == 0 ? 2 : 3;
+        L.degree = amg_level_degree(l);
         L.a_nnz = h.a_col.size();
         NBX_TRY(up(&L.a_ptr, h.a_ptr, A->bytes));
         NBX_TRY(up(&L.a_col, h.a_col, A->bytes));
@@ -864,7 +866,7 @@ static int amg_numeric(FemSys& S) {
             const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
             double rho = 1.0 / sigma;
             L.inv_theta = (float)(1.0 / theta);
-            for (int k = 1; k < AMG_DEGREE; ++k) {
+            for (int k = 1; k < L.degree; ++k) {
                 const double rn = 1.0 / (2.0 * sigma - rho);
                 L.c1[k] = (float)(rn * rho);
                 L.c2[k] = (float)(2.0 * rn / delta);
@@ -1076,7 +1078,7 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
         with_bl(V.ab.lanes, [&](auto ln) {
             constexpr int LN = decltype(ln)::value;
             const unsigned g = grid_for((size_t)V.ab.rows * LN);
-            for (int k = 1; k < AMG_DEGREE; ++k) {
+            for (int k = 1; k < V.degree; ++k) {
                 hipLaunchKernelGGL((k_bsm_step<false, LN>), dim3(g), dim3(BLOCK), 0, s,
                                    bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],
                                    V.c2[k], st, V.ab.rows);
@@ -1103,7 +1105,7 @@ static void cycle_from(AmgDev* A, size_t l0, const float* b, float* x, const Pcg
             const unsigned g = grid_for((size_t)V.ab.rows * LN);
             hipLaunchKernelGGL(k_bsm_init<LN>, dim3(g), dim3(BLOCK), 0, s, bell<3, 3>(V.ab),
                                V.dinv32, bl, xl, V.r, V.d0, V.inv_theta, st, V.ab.rows);
-            for (int k = 1; k < AMG_DEGREE; ++k) {
+            for (int k = 1; k < V.degree; ++k) {
                 if (k == 1)
                     hipLaunchKernelGGL((k_bsm_step<true, LN>), dim3(g), dim3(BLOCK), 0, s,
                                        bell<3, 3>(V.ab), V.dinv32, din, dout, V.r, xl, V.c1[k],

@@ -15,9 +15,10 @@
 //   * A_c = P^T A P, recursively, until at most AMG_COARSEST unknowns remain,
 //     whose matrix is inverted (dense, FP64) and applied as a dense product;
 //   * one V-cycle: Chebyshev smoothing in D^-1 A on [lambda/30, 1.1 lambda]
-//     before and after the coarse correction, of degree AMG_DEGREE below the
-//     finest level and AMG_FINE_DEGREE on it (degree 1: one Jacobi step
-//     weighted 1/theta, theta the interval's centre).
+//     before and after the coarse correction, of degree AMG_FINE_DEGREE on
+//     the finest level (degree 1: one Jacobi step weighted 1/theta, theta
+//     the interval's centre), AMG_LEVEL1_DEGREE on the next and AMG_DEGREE
+//     below.
 // The host builds every pattern as scalar CSR, grouped by node (a node's
 // rows share one pattern of complete column groups).  The aggregation, T
 // and every pattern depend on the matrix's graph, its Dirichlet rows and the
@@ -27,6 +28,7 @@
 // copies).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>
@@ -40,10 +42,22 @@ constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and a
 // cantilever: 64 against 66; 1.05M-DOF SIMP beam: 94-95 against 90 per
 // solve) for a third of the finest products (C3 solve 0.055 -> 0.046 s)
 constexpr int AMG_FINE_DEGREE = 1;
+// ... and on level 1, the largest coarse level (a third of the finest one's
+// nonzeros with six products per cycle at degree 3): degree 2 takes as many
+// outer iterations (4.2M DOF: 64; 1.05M-DOF SIMP beam: 92 against 95) for
+// two products fewer (C3 solve 0.045 -> 0.042 s, SIMP 39.4 -> 36.8 ms)
+constexpr int AMG_LEVEL1_DEGREE = 2;
+// the smoothing degree of level l
+constexpr int amg_level_degree(size_t l) {
+    return l == 0 ? AMG_FINE_DEGREE : l == 1 ? AMG_LEVEL1_DEGREE : AMG_DEGREE;
+}
 constexpr uint32_t AMG_COARSEST = 200; // unknowns of the directly solved level
 constexpr int AMG_MAX_LEVELS = 12;
 static_assert(AMG_DEGREE <= 8 && AMG_FINE_DEGREE >= 1 && AMG_FINE_DEGREE <= 8,
               "AmgFine holds the coefficients of at most 8 steps");
+static_assert(AMG_LEVEL1_DEGREE >= 2 && AMG_LEVEL1_DEGREE <= AMG_DEGREE,
+              "a coarse level keeps its coefficients in arrays of AMG_DEGREE and adds its last "
+              "direction to x itself (degree >= 2)");
 
 // One level of the host-built symbolic hierarchy.
 struct AmgHostLevel {

@@ -74,10 +74,11 @@ Cheb cheb(double lmax, int degree) {
     return c;
 }
 
-// degree AMG_FINE_DEGREE on the finest level (l = 0), AMG_DEGREE below
+// degree amg_level_degree(l): AMG_FINE_DEGREE on the finest level, then
+// AMG_LEVEL1_DEGREE, then AMG_DEGREE
 void smooth(const AmgHostLevel& L, const Num& N, size_t l, const std::vector<double>& b,
             std::vector<double>& x, bool zero) {
-    const Cheb c = cheb(N.lmax, l == 0 ? AMG_FINE_DEGREE : AMG_DEGREE);
+    const Cheb c = cheb(N.lmax, amg_level_degree(l));
     std::vector<double> r(L.n), d(L.n), y(L.n);
     if (!zero) spmv(L, N.a, x, y);
     for (uint32_t i = 0; i < L.n; ++i) {
