@@ -84,8 +84,8 @@ struct AmgDev {
 
 namespace {
 
-template <class T>
-int up(T** d, const std::vector<T>& h, size_t& bytes) {
+template <class T, class Al>
+int up(T** d, const std::vector<T, Al>& h, size_t& bytes) {
     const size_t nb = (h.size() + 1) * sizeof(T);
     NBX_HIP(hipMalloc(d, nb));
     bytes += nb;
@@ -666,8 +666,8 @@ static int bell_upload(uint32_t rows, Len len_of, Col col_of, BellDev& B, size_t
 
 // Whether the R rows of every block row share one pattern of complete
 // C-column groups (what the block layout stores).
-static bool grouped(uint32_t rows, int R, int C, const std::vector<uint32_t>& ptr,
-                    const std::vector<uint32_t>& col) {
+static bool grouped(uint32_t rows, int R, int C, const HostVec<uint32_t>& ptr,
+                    const HostVec<uint32_t>& col) {
     for (uint32_t I = 0; I < rows; ++I) {
         const uint32_t b0 = ptr[(size_t)R * I], l = ptr[(size_t)R * I + 1] - b0;
         if (l % C) return false;

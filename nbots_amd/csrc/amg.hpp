@@ -31,9 +31,36 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <memory>
+#include <new>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace nbx {
+
+// An allocator whose value-initialisation leaves trivial values
+// uninitialised: the host setup's large arrays are written once by its
+// threads, so they are neither zeroed first nor first touched (page-faulted)
+// by one thread.
+template <class T, class A = std::allocator<T>>
+struct NoInitAlloc : A {
+    using A::A;
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U, typename std::allocator_traits<A>::template rebind_alloc<U>>;
+    };
+    template <class U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        std::allocator_traits<A>::construct(static_cast<A&>(*this), p, std::forward<Args>(args)...);
+    }
+};
+template <class T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 constexpr uint32_t AMG_NONE = 0xffffffffu;
 constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and after
@@ -63,15 +90,15 @@ static_assert(AMG_LEVEL1_DEGREE >= 2 && AMG_LEVEL1_DEGREE <= AMG_DEGREE,
 struct AmgHostLevel {
     uint32_t n = 0;                   // scalar unknowns
     uint32_t bs = 0;                  // unknowns per node (2 on the finest level, 3 below)
-    std::vector<uint32_t> a_ptr, a_col;   // this level's operator pattern
+    HostVec<uint32_t> a_ptr, a_col;   // this level's operator pattern
     // to the next coarser level (empty on the coarsest)
     uint32_t nc = 0;                  // coarse unknowns (3 per aggregate)
-    std::vector<double> t;            // [3 n] T's values of row i: columns agg(i)*3 + m
-    std::vector<uint32_t> self;       // [n] P row i's position of column agg(i)*3, or NONE
-    std::vector<uint32_t> p_ptr, p_col;   // P pattern (n x nc)
-    std::vector<uint32_t> ap_map;     // [nnz(A)] P row i's position of agg(col)*3, or NONE
-    std::vector<uint32_t> w_ptr, w_col;   // W = A P pattern (n x nc)
-    std::vector<uint32_t> pt_ptr, pt_row, pt_pos;   // P^T: coarse column c -> (row, position)
+    HostVec<double> t;                // [3 n] T's values of row i: columns agg(i)*3 + m
+    HostVec<uint32_t> self;           // [n] P row i's position of column agg(i)*3, or NONE
+    HostVec<uint32_t> p_ptr, p_col;   // P pattern (n x nc)
+    HostVec<uint32_t> ap_map;         // [nnz(A)] P row i's position of agg(col)*3, or NONE
+    HostVec<uint32_t> w_ptr, w_col;   // W = A P pattern (n x nc)
+    HostVec<uint32_t> pt_ptr, pt_row, pt_pos;   // P^T: coarse column c -> (row, position)
 };
 
 struct AmgHost {

@@ -17,7 +17,7 @@ namespace nbx {
 namespace {
 
 struct Graph {
-    std::vector<uint32_t> ptr, adj;
+    HostVec<uint32_t> ptr, adj;
 };
 
 // Host threads for the setup (at most 16: a one-GPU job's share of a host).
@@ -65,7 +65,7 @@ void par_for(uint32_t n, F f) {
 // Per-item lists built in parallel: f(i, out) appends item i's list to
 // out.  The result (ptr[n + 1], data) does not depend on the threads.
 template <class F>
-void build_lists(uint32_t n, F f, std::vector<uint32_t>& ptr, std::vector<uint32_t>& data) {
+void build_lists(uint32_t n, F f, HostVec<uint32_t>& ptr, HostVec<uint32_t>& data) {
     const unsigned nt = setup_threads(n);
     std::vector<std::vector<uint32_t>> part(nt);
     std::vector<uint32_t> len(n);
@@ -80,11 +80,13 @@ void build_lists(uint32_t n, F f, std::vector<uint32_t>& ptr, std::vector<uint32
         }
     };
     run_threads(nt, fill);
-    ptr.assign(n + 1, 0);
+    ptr.resize(n + 1);
+    ptr[0] = 0;
     for (uint32_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + len[i];
     data.resize(ptr[n]);
-    for (unsigned t = 0; t < nt; ++t)
+    run_threads(nt, [&](unsigned t) {
         std::copy(part[t].begin(), part[t].end(), data.begin() + ptr[lo[t]]);
+    });
 }
 
 
@@ -180,7 +182,10 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     }
     // tentative prolongator: modified Gram-Schmidt (twice) on each
     // aggregate's rows of B; a column that vanishes is a dead coarse unknown
-    L.t.assign(3 * (size_t)n, 0.0);
+    L.t.resize(3 * (size_t)n);
+    par_for(n, [&](uint32_t i0, uint32_t i1) {   // rows of no aggregate keep 0
+        std::fill(L.t.begin() + 3 * (size_t)i0, L.t.begin() + 3 * (size_t)i1, 0.0);
+    });
     Bc.assign(3 * (size_t)L.nc, 0.0);
     dead_c.assign(L.nc, 0);
     par_for(na, [&](uint32_t a0, uint32_t a1) {
@@ -229,7 +234,7 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     });
     // P pattern per node (shared by its rows): the aggregates of the nodes
     // its rows reach, three columns each
-    std::vector<uint32_t> anode_ptr, anode;   // the aggregates, ascending
+    HostVec<uint32_t> anode_ptr, anode;   // the aggregates, ascending
     build_lists(
         nodes,
         [&](uint32_t I, std::vector<uint32_t>& out) {
@@ -243,12 +248,13 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
             out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
         },
         anode_ptr, anode);
-    L.p_ptr.assign(n + 1, 0);
+    L.p_ptr.resize(n + 1);
+    L.p_ptr[0] = 0;
     for (uint32_t i = 0; i < n; ++i)
         L.p_ptr[i + 1] = L.p_ptr[i] + 3 * (anode_ptr[i / bs + 1] - anode_ptr[i / bs]);
     L.p_col.resize(L.p_ptr[n]);
-    L.self.assign(n, AMG_NONE);
-    L.ap_map.assign(L.a_ptr[n], AMG_NONE);
+    L.self.resize(n);
+    L.ap_map.resize(L.a_ptr[n]);
     par_for(nodes, [&](uint32_t I0, uint32_t I1) {
         for (uint32_t I = I0; I < I1; ++I) {
             const uint32_t* b = &anode[anode_ptr[I]];
@@ -257,17 +263,17 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
                 uint32_t q = L.p_ptr[i];
                 for (const uint32_t* A = b; A != e; ++A)
                     for (uint32_t c = 0; c < 3; ++c) L.p_col[q++] = 3 * *A + c;
-                if (agg[I] != AMG_NONE) L.self[i] = L.p_ptr[i] + 3 * find(b, e, agg[I]);
+                L.self[i] = agg[I] != AMG_NONE ? L.p_ptr[i] + 3 * find(b, e, agg[I]) : AMG_NONE;
                 for (uint32_t k = L.a_ptr[i]; k < L.a_ptr[i + 1]; ++k) {
                     const uint32_t A = agg[L.a_col[k] / bs];
-                    if (A != AMG_NONE) L.ap_map[k] = L.p_ptr[i] + 3 * find(b, e, A);
+                    L.ap_map[k] = A != AMG_NONE ? L.p_ptr[i] + 3 * find(b, e, A) : AMG_NONE;
                 }
             }
         }
     });
     // W = A P pattern per node: the union of the P rows its rows reach
     // (aggregates; three columns each)
-    std::vector<uint32_t> wnode_ptr, wnode;
+    HostVec<uint32_t> wnode_ptr, wnode;
     build_lists(
         nodes,
         [&](uint32_t I, std::vector<uint32_t>& out) {
@@ -285,7 +291,8 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
             out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
         },
         wnode_ptr, wnode);
-    L.w_ptr.assign(n + 1, 0);
+    L.w_ptr.resize(n + 1);
+    L.w_ptr[0] = 0;
     for (uint32_t i = 0; i < n; ++i)
         L.w_ptr[i + 1] = L.w_ptr[i] + 3 * (wnode_ptr[i / bs + 1] - wnode_ptr[i / bs]);
     L.w_col.resize(L.w_ptr[n]);
@@ -307,7 +314,8 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     }
     // P^T lists: coarse column c -> the rows i holding it (ascending) and
     // its position in P
-    L.pt_ptr.assign(L.nc + 1, 0);
+    L.pt_ptr.resize(L.nc + 1);
+    L.pt_ptr[0] = 0;
     for (uint32_t c = 0; c < L.nc; ++c)
         L.pt_ptr[c + 1] = L.pt_ptr[c] + bs * (hptr[c / 3 + 1] - hptr[c / 3]);
     L.pt_row.resize(L.p_ptr[n]);
@@ -330,7 +338,7 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     // whose P rows hold it
     C.n = L.nc;
     C.bs = 3;
-    std::vector<uint32_t> cnode_ptr, cnode;
+    HostVec<uint32_t> cnode_ptr, cnode;
     build_lists(
         na,
         [&](uint32_t a, std::vector<uint32_t>& out) {
@@ -343,7 +351,8 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
             out.erase(std::unique(out.begin() + (ptrdiff_t)b, out.end()), out.end());
         },
         cnode_ptr, cnode);
-    C.a_ptr.assign(C.n + 1, 0);
+    C.a_ptr.resize(C.n + 1);
+    C.a_ptr[0] = 0;
     for (uint32_t c = 0; c < C.n; ++c)
         C.a_ptr[c + 1] = C.a_ptr[c] + 3 * (cnode_ptr[c / 3 + 1] - cnode_ptr[c / 3]);
     C.a_col.resize(C.a_ptr[C.n]);
@@ -365,8 +374,14 @@ static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, c
     AmgHostLevel L0;
     L0.n = n;
     L0.bs = 2;
-    L0.a_ptr.assign(rowptr, rowptr + n + 1);
-    L0.a_col.assign(col, col + rowptr[n]);
+    L0.a_ptr.resize(n + 1);
+    L0.a_col.resize(rowptr[n]);
+    par_for(n + 1, [&](uint32_t i0, uint32_t i1) {
+        std::copy(rowptr + i0, rowptr + i1, L0.a_ptr.begin() + i0);
+    });
+    par_for(rowptr[n], [&](uint32_t e0, uint32_t e1) {
+        std::copy(col + e0, col + e1, L0.a_col.begin() + e0);
+    });
     // Dirichlet-eliminated unknowns: every off-diagonal value of the row is 0
     std::vector<uint8_t> dead(n, 0);
     for (uint32_t i = 0; i < n; ++i) {

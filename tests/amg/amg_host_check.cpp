@@ -32,7 +32,7 @@ bool rd(FILE* f, std::vector<T>& v, size_t n) {
     return fread(v.data(), sizeof(T), n, f) == n;
 }
 
-size_t pos_of(const std::vector<uint32_t>& col, size_t b, size_t e, uint32_t c) {
+size_t pos_of(const HostVec<uint32_t>& col, size_t b, size_t e, uint32_t c) {
     size_t lo = b, hi = e;
     while (lo < hi) {
         const size_t m = lo + (hi - lo) / 2;
