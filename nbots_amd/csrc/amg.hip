@@ -955,9 +955,8 @@ int amg_prepare(FemSys& S) {
         S.amg_symbolic_current = true;
     }
     if (!S.K.amg_current) {
-        // the finest level's smoothing reads the fp16 scaled copy when the
-        // matrix allows one (values in fp16 range), its own FP32 CSR copy
-        // otherwise
+        // the finest level's smoothing reads the fp16 Jacobi-scaled copy
+        // (a FEM matrix always has one: its scaled entries lie in [-1, 1])
         AmgDev* A = S.K.amg;
         A->scaled16 = false;
         if (A->lv.size() > 1) {
