@@ -27,9 +27,11 @@ reported apart; their product is the Chebyshev mixed solver's speedup.  A
 SIMP topology-optimisation loop on a 1024x512 MBB half-beam (1.05M DOF,
 50 iterations) gives ms per SIMP iteration with each solver; the headline
 ms_per_simp_iter is the multigrid-preconditioned loop's, setup included.
+As context for the roofline fraction, roofline.stream_context gives what
+a plain copy kernel (torch's) reaches on the same GPU.
 
     python bench.py [--steps K] [--warmup W] [--config c3|c2|c4] [--no-cpu]
-                    [--simp-iters 50] [--no-simp] [--no-decompose]
+                    [--simp-iters 50] [--no-simp] [--no-decompose] [--no-stream]
     python bench.py --cpu-full        # the CPU baseline plan at full size
     python bench.py --pmc-probe [--probe-mixed | --probe-amg]   # short runs for rocprofv3
     python bench.py --simp-only [--simp-iters N] [--simp-amg-only]  # the SIMP loop alone
@@ -50,12 +52,12 @@ sys.path.insert(0, ROOT)
 METRIC = ("DOF·CG-iter/s + time-to-1e-8 residual (FP32+IR vs FP64); "
           "ms per SIMP iteration")
 HBM_PEAK_GBS = 8000.0
+# iteration cap of the AMG-preconditioned solves (they take 25-100)
+AMG_MAX_ITER = 2000
 # HBM bytes per launch of k_pcg_A measured by two rocprofv3 --pmc passes
 # (FETCH_SIZE x 1.99995 + WRITE_SIZE x 1.000, the factors calibrated on a
 # 1 GiB elementwise kernel), summarised in profiles/k_pcg_A_c3_hbm_traffic.md;
 # keyed by (config, symmetric layout)
-# iteration cap of the AMG-preconditioned solves (they take 25-40)
-AMG_MAX_ITER = 2000
 PMC_TRAFFIC = {("c3", False): 599187989, ("c3", True): 411256645}
 CONFIGS = {
     "c3": dict(nx=2048, ny=1024, name="C3 mixed precision: 2048x1024 cantilever, lognormal(0,1) E, "
@@ -347,6 +349,49 @@ def pmc_probe(args):
           flush=True)
 
 
+def stream_rates(mib=1024, reps=20):
+    """What plain streaming kernels reach on this GPU, as context for the
+    SpMV's fraction of the 8 TB/s peak: torch's b.copy_(a) over `mib` MiB
+    (read + write), the median of `reps` launches timed with events on
+    torch's stream (torch's reductions stream slower still).  Runs in
+    a process of its own (`--stream-only`): torch brings its own HIP runtime,
+    which cannot start in a process where the library's has."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty(n, dtype=torch.float32, device="cuda")
+
+    def med(fn, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return round(nbytes / float(np.median(ts)) / 1e9, 1)
+
+    out = dict(copy_gbs=med(lambda: b.copy_(a), 8 * n),
+               note=f"torch b.copy_(a) of {mib} MiB (read and written), median of {reps} "
+                    "launches")
+    return out
+
+
+def stream_rates_child():
+    """stream_rates() in a child process; None when it fails."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-only"],
+                       capture_output=True, text=True, timeout=300)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        log(f"[bench] streaming context failed (exit {r.returncode}): {r.stderr[-400:]}")
+        return None
+    return json.loads(lines[-1])
+
+
 # ---- GPU solvers --------------------------------------------------------------------
 
 REPLAY = False   # --replay: the solvers' repeated steps as captured HIP graphs
@@ -590,6 +635,10 @@ def main():
     ap.add_argument("--probe-iters", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the plain streaming kernels timed as roofline context")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only those streaming kernels, as one JSON line")
     ap.add_argument("--no-decompose", action="store_true",
                     help="skip the FP64-preconditioner and FP32-CG + FP64-refinement runs")
     ap.add_argument("--mixed-max-iter", type=int, default=200000)
@@ -619,6 +668,9 @@ def main():
         return
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.stream_only:
+        print(json.dumps(stream_rates()), flush=True)
         return
     if args.simp_only:
         from nbots_amd.fem import MIXED_CHEB_FCG
@@ -700,6 +752,11 @@ def main():
                     csr_bytes_per_launch=csr_spmv_bytes(N, nnz),
                     csr_note="the same product on scalar CSR with 4-byte indices (the survey's "
                              "byte model); this layout moves fewer bytes for the same result")
+    st = None if args.no_stream else stream_rates_child()
+    if st:
+        st["spmv_frac_of_copy"] = round(achieved / st["copy_gbs"], 3)
+        roofline["stream_context"] = st
+        log(f"[bench] streaming context: torch copy {st['copy_gbs']} GB/s")
 
     extra = {}
     if not args.no_mixed:

@@ -49,6 +49,9 @@ def test_bench_json_line(monkeypatch):
     assert d["mixed_true_rel_res"] <= 1e-8
     assert rl["byte_model"] and rl["csr_bytes_per_launch"] > rl["bytes_per_launch"]
     assert cb["host"]["nproc"] >= 1 and cb["host"]["affinity"] >= 1
+    st = rl["stream_context"]
+    assert 0 < st["copy_gbs"] < 1.2 * rl["peak"]
+    assert abs(st["spmv_frac_of_copy"] - rl["achieved"] / st["copy_gbs"]) < 1e-2
     # the mixed-precision speedup split into the preconditioner's gain and
     # the precision's, and the FP32-CG + FP64-refinement scheme beside them
     dec = d["decomposition"]
