@@ -442,8 +442,8 @@ def amg_solve(f, t_fp64, u64, Fl):
                  vs_fp64_compliance_rel=abs(float(Fl @ u) - float(Fl @ u64)) / abs(float(Fl @ u64)),
                  solver=("FP64 flexible CG; preconditioner one FP32 V-cycle of smoothed-"
                          "aggregation multigrid (rigid-body modes; one weighted Jacobi step on "
-                         "the finest level's fp16 scaled copy, degree-3 Chebyshev-Jacobi "
-                         "smoothing below; dense coarsest level); setup = first solve - solve")))
+                         "the finest level's fp16 scaled copy, Chebyshev-Jacobi smoothing of "
+                         "degree 2, 4 and 5 below; dense coarsest level); setup = first solve - solve")))
 
 
 def decompose(f, args, t_fp64, iters64, u64, Fl):

@@ -61,7 +61,7 @@ struct AmgDevLevel {
     // finest level's come from the caller), residual, two directions
     float *b = nullptr, *x = nullptr, *r = nullptr, *d0 = nullptr, *d1 = nullptr;
     // Chebyshev smoothing on [lambda/30, 1.1 lambda]
-    float inv_theta = 0.f, c1[AMG_DEGREE] = {}, c2[AMG_DEGREE] = {};
+    float inv_theta = 0.f, c1[AMG_MAX_DEGREE] = {}, c2[AMG_MAX_DEGREE] = {};
     int degree = AMG_DEGREE;   // smoothing steps before and after (amg_level_degree)
 };
 

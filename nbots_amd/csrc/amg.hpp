@@ -17,8 +17,8 @@
 //   * one V-cycle: Chebyshev smoothing in D^-1 A on [lambda/30, 1.1 lambda]
 //     before and after the coarse correction, of degree AMG_FINE_DEGREE on
 //     the finest level (degree 1: one Jacobi step weighted 1/theta, theta
-//     the interval's centre), AMG_LEVEL1_DEGREE on the next and AMG_DEGREE
-//     below.
+//     the interval's centre), AMG_LEVEL1_DEGREE on the next, then
+//     AMG_LEVEL2_DEGREE, and AMG_DEGREE below.
 // The host builds every pattern as scalar CSR, grouped by node (a node's
 // rows share one pattern of complete column groups).  The aggregation, T
 // and every pattern depend on the matrix's graph, its Dirichlet rows and the
@@ -63,7 +63,15 @@ template <class T>
 using HostVec = std::vector<T, NoInitAlloc<T>>;
 
 constexpr uint32_t AMG_NONE = 0xffffffffu;
-constexpr int AMG_DEGREE = 3;          // Chebyshev smoothing steps before and after
+// Chebyshev smoothing steps before and after the coarse correction on the
+// levels below level 2, a few thousand unknowns and fewer, where a step
+// costs the latency of its launch and dependent gathers (3.5-5 us) rather
+// than its bytes: degree 5 there and 4 on level 2 take 11-20 % fewer outer
+// iterations than 3 (4.2M-DOF cantilever: 57 against 64; 1.05M-DOF SIMP
+// beam: 74 against 92 per solve) for 2-3 steps more per level and cycle
+// (C3 solve 42.4 -> 38.7 ms, SIMP solve 23.2 -> 21.3 ms per iteration)
+constexpr int AMG_DEGREE = 5;
+constexpr int AMG_LEVEL2_DEGREE = 4;
 // ... on the finest level, whose products are the cycle's largest: with
 // degree 1 the solves take as many outer iterations as with 3 (4.2M-DOF
 // cantilever: 64 against 66; 1.05M-DOF SIMP beam: 94-95 against 90 per
@@ -74,17 +82,21 @@ constexpr int AMG_FINE_DEGREE = 1;
 // outer iterations (4.2M DOF: 64; 1.05M-DOF SIMP beam: 92 against 95) for
 // two products fewer (C3 solve 0.045 -> 0.042 s, SIMP 39.4 -> 36.8 ms)
 constexpr int AMG_LEVEL1_DEGREE = 2;
+constexpr int AMG_MAX_DEGREE = 8;      // the coefficient arrays' length
 // the smoothing degree of level l
 constexpr int amg_level_degree(size_t l) {
-    return l == 0 ? AMG_FINE_DEGREE : l == 1 ? AMG_LEVEL1_DEGREE : AMG_DEGREE;
+    return l == 0 ? AMG_FINE_DEGREE : l == 1 ? AMG_LEVEL1_DEGREE
+                                    : l == 2 ? AMG_LEVEL2_DEGREE : AMG_DEGREE;
 }
 constexpr uint32_t AMG_COARSEST = 200; // unknowns of the directly solved level
 constexpr int AMG_MAX_LEVELS = 12;
-static_assert(AMG_DEGREE <= 8 && AMG_FINE_DEGREE >= 1 && AMG_FINE_DEGREE <= 8,
-              "AmgFine holds the coefficients of at most 8 steps");
-static_assert(AMG_LEVEL1_DEGREE >= 2 && AMG_LEVEL1_DEGREE <= AMG_DEGREE,
-              "a coarse level keeps its coefficients in arrays of AMG_DEGREE and adds its last "
-              "direction to x itself (degree >= 2)");
+static_assert(AMG_FINE_DEGREE >= 1 && AMG_FINE_DEGREE <= AMG_MAX_DEGREE,
+              "AmgFine holds the coefficients of at most AMG_MAX_DEGREE steps");
+static_assert(AMG_LEVEL1_DEGREE >= 2 && AMG_LEVEL2_DEGREE >= 2 && AMG_DEGREE >= 2 &&
+              AMG_LEVEL1_DEGREE <= AMG_MAX_DEGREE && AMG_LEVEL2_DEGREE <= AMG_MAX_DEGREE &&
+              AMG_DEGREE <= AMG_MAX_DEGREE,
+              "a coarse level keeps its coefficients in arrays of AMG_MAX_DEGREE and adds its "
+              "last direction to x itself (degree >= 2)");
 
 // One level of the host-built symbolic hierarchy.
 struct AmgHostLevel {

@@ -75,7 +75,7 @@ Cheb cheb(double lmax, int degree) {
 }
 
 // degree amg_level_degree(l): AMG_FINE_DEGREE on the finest level, then
-// AMG_LEVEL1_DEGREE, then AMG_DEGREE
+// AMG_LEVEL1_DEGREE, AMG_LEVEL2_DEGREE and AMG_DEGREE
 void smooth(const AmgHostLevel& L, const Num& N, size_t l, const std::vector<double>& b,
             std::vector<double>& x, bool zero) {
     const Cheb c = cheb(N.lmax, amg_level_degree(l));
