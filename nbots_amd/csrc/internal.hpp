@@ -93,6 +93,11 @@ struct Workspace {
     // Replay a solve's repeated steps as captured HIP graphs (one host call
     // per step instead of one per kernel; same kernels, same results).
     bool replay = false;
+    // Outer iterations of this system's last converged flexible-CG solve
+    // (first pass): the next solve launches most of that many before it
+    // first polls, then polls every few iterations.  Host cadence only: the
+    // iterates and results do not depend on it.
+    uint32_t fcg_hint = 0;
 };
 
 struct SolveStats {

@@ -1144,8 +1144,8 @@ static int solve_mixed_cg(const DevMatrix& m, const double* b, double* x, double
 
 // FP64 flexible CG with the FP32 Chebyshev-Jacobi preconditioner.
 static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double rtol,
-                         const MixedOpts& opt, uint32_t max_iter, Workspace& w, hipStream_t s,
-                         SolveStats* stats) {
+                         const MixedOpts& opt, uint32_t max_iter, uint32_t hint, Workspace& w,
+                         hipStream_t s, SolveStats* stats) {
     const uint32_t nu = m.s.n_nodes;
     int rc = NBX_OK;
     const double bb = device_norm2(m.n, b, w, s, &rc);
@@ -1321,7 +1321,20 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
     precondition(1);
     NBX_HIP(hipGetLastError());
     NBX_TRY(poll_state(w, s));
+    // Iterations launched between polls of the device state.  Kernels
+    // launched after the stop test fired return at once but still cost a
+    // launch each (about 2.5 us; some 50 per outer iteration with the
+    // multigrid preconditioner), so the batches end near the expected count:
+    // with a hint (the iterations of a similar earlier solve) the first
+    // batch runs to 7/8 of it and the next ones take 4, 4, 8, 16, 32; without
+    // one they take 4, 8, 16, 32.
     uint32_t batch = 4, nsamp = 0;
+    int fours = 0;   // batches of 4 still to come before the doubling
+    if (hint >= 16) {
+        batch = hint - hint / 8;
+        if (batch > max_iter) batch = max_iter > 4 ? max_iter : 4u;
+        fours = 2;
+    }
     uint64_t launched = 0;   // 64-bit: max_iter near UINT32_MAX cannot wrap the guard
     constexpr uint32_t every = 8;   // sample one outer iteration in eight
     // An outer iteration: the FP64 product, then the preconditioner and the
@@ -1356,7 +1369,12 @@ static int fcg_cheb_pass(const DevMatrix& m, const double* b, double* x, double 
             set_last_error("flexible CG did not report completion");
             return NBX_ERR_BREAKDOWN;
         }
-        batch = batch < 32 ? batch * 2 : 32;
+        if (fours > 0) {
+            batch = 4;
+            --fours;
+        } else {
+            batch = batch < 32 ? batch * 2 : 32;
+        }
     }
     NBX_HIP(hipEventRecord(w.ev[1], s));
     NBX_HIP(hipEventSynchronize(w.ev[1]));
@@ -1398,7 +1416,9 @@ static int solve_fcg_cheb(const DevMatrix& m, const double* b, double* x, double
     const int passes = opt.true_residual ? SOLVE_PASSES : 1;
     for (int pass = 0; pass < passes; ++pass) {
         SolveStats ps{};
-        rc = fcg_cheb_pass(m, b, x, rtol, opt, max_iter - used, w, s, &ps);
+        rc = fcg_cheb_pass(m, b, x, rtol, opt, max_iter - used, pass == 0 ? w.fcg_hint : 0u, w,
+                           s, &ps);
+        if (pass == 0) w.fcg_hint = rc == NBX_OK ? ps.iters : 0u;
         if (rc != NBX_OK && rc != NBX_NOT_CONVERGED) return rc;
         used += ps.iters;
         add_pass(tot, ps, pass);

@@ -11,7 +11,8 @@ Bars:
     Dirichlet conditions rebuild it, and the solves still converge;
   * the SIMP loop with AMG solves follows the FP64 loop's compliance history
     within 1e-6;
-  * a system without node coordinates (the sparse drop-in) refuses the variant.
+  * a system without node coordinates (the sparse drop-in) refuses the variant;
+  * the polling cadence taken from the previous solve changes no result.
 """
 import dataclasses
 
@@ -150,6 +151,37 @@ def test_simp_default_solver_is_amg():
         assert info["status"] == 0 and info["layout"] & 16, info
     s.destroy()
     f.destroy()
+
+
+def test_polling_hint_leaves_results_unchanged():
+    """A solve launches most of the previous solve's iteration count before it
+    first reads the device state (host cadence only): the same system solved
+    again from zero gives the same iterate bit for bit, as does a fresh system
+    without a hint; a solve that starts at the solution stops at once (the
+    hinted launches return immediately); a smaller max_iter than the hint still
+    stops at max_iter."""
+    from nbots_amd.fem import Fem, SOLVE_MIXED
+    p = problem_from_golden(load_golden("traction_128x64"))
+    f, g = Fem.from_problem(p), Fem.from_problem(p)
+    o = _amg_opts(true_residual=True)
+    i1 = f.solve(SOLVE_MIXED, tol=1e-8, opts=o)
+    u1 = f.displacement()
+    i2 = f.solve(SOLVE_MIXED, tol=1e-8, opts=o)
+    u2 = f.displacement()
+    i3 = g.solve(SOLVE_MIXED, tol=1e-8, opts=o)
+    u3 = g.displacement()
+    print(f"[amg] hint: {i1['iters']} {i2['iters']} {i3['iters']}", flush=True)
+    assert i1["status"] == i2["status"] == i3["status"] == 0
+    assert i1["iters"] == i2["iters"] == i3["iters"] >= 16, (i1, i2, i3)
+    assert np.array_equal(u1, u2) and np.array_equal(u1, u3)
+    i4 = f.solve(SOLVE_MIXED, tol=1e-8, u0=u1, opts=o)
+    assert i4["status"] == 0 and i4["iters"] <= 2, i4
+    assert np.linalg.norm(f.displacement() - u1) <= 1e-8 * np.linalg.norm(u1)
+    f.solve(SOLVE_MIXED, tol=1e-8, opts=o)
+    i5 = f.solve(SOLVE_MIXED, tol=1e-8, max_iter=10, opts=o)
+    assert i5["status"] == 1 and i5["iters"] == 10, i5
+    f.destroy()
+    g.destroy()
 
 
 def test_amg_refused_without_coordinates():
