@@ -194,28 +194,36 @@ __global__ __launch_bounds__(FIN) void k_amg_pow_fin(const double* partials, uns
     }
 }
 
-// P = T - omega D^-1 A T, row i
+// P = T - omega D^-1 A T: one thread per group of three P entries (row i,
+// the columns of one aggregate, positions q .. q + 2), summing the row's A
+// entries that map to q in entry order
 __global__ __launch_bounds__(BLOCK) void k_amg_smooth_p(
     const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_col,
     const double* __restrict__ a, const double* __restrict__ dinv, const double* __restrict__ t,
     const uint32_t* __restrict__ self, const uint32_t* __restrict__ p_ptr,
-    const uint32_t* __restrict__ ap_map, double* __restrict__ p, double omega, uint32_t n) {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t pb = p_ptr[i], pe = p_ptr[i + 1];
-    for (uint32_t q = pb; q < pe; ++q) p[q] = 0.0;
-    // positions index triples of the row (checked: the host built them so)
-    const uint32_t sq = self[i];
-    if (sq != AMG_NONE && sq >= pb && sq + 3 <= pe)
-        for (int m = 0; m < 3; ++m) p[sq + m] += t[3 * (size_t)i + m];
+    const uint32_t* __restrict__ ap_map, double* __restrict__ p, double omega, uint32_t n,
+    size_t groups) {
+    const size_t g = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (g >= groups) return;
+    const uint32_t q = (uint32_t)(3 * g);
+    // the row holding position q: the last i with p_ptr[i] <= q
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (p_ptr[mid] <= q) lo = mid; else hi = mid;
+    }
+    const uint32_t i = lo;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (self[i] == q)
+        for (int m = 0; m < 3; ++m) acc[m] += t[3 * (size_t)i + m];
     const double od = omega * dinv[i];
     for (uint32_t e = a_ptr[i]; e < a_ptr[i + 1]; ++e) {
-        const uint32_t q = ap_map[e];
-        if (q == AMG_NONE || q < pb || q + 3 > pe) continue;
+        if (ap_map[e] != q) continue;
         const double coef = od * a[e];
         const uint32_t j = a_col[e];
-        for (int m = 0; m < 3; ++m) p[q + m] -= coef * t[3 * (size_t)j + m];
+        for (int m = 0; m < 3; ++m) acc[m] -= coef * t[3 * (size_t)j + m];
     }
+    for (int m = 0; m < 3; ++m) p[q + m] = acc[m];
 }
 
 // Block values from a CSR whose R rows per block row share one pattern of
@@ -242,26 +250,33 @@ __global__ __launch_bounds__(BLOCK) void k_bell_fill(BellDev B, const uint32_t* 
 }
 // P^T's blocks from P's values through the P^T lists: block k of aggregate
 // a is node J = bcol; its value (c, r) is P(R J + r, 3 a + c), the
-// (R k + r)-th entry of column 3 a + c's list
+// (R k + r)-th entry of column 3 a + c's list.  One thread per slot of the
+// layout (padding slots return).
 template <int R>
-__global__ __launch_bounds__(BLOCK) void k_bell_fill_t(BellDev B, const uint32_t* __restrict__ pt_ptr,
+__global__ __launch_bounds__(BLOCK) void k_bell_fill_t(BellDev B, uint32_t slices,
+                                                       const uint32_t* __restrict__ pt_ptr,
                                                        const uint32_t* __restrict__ pt_pos,
                                                        const double* __restrict__ p,
                                                        const double* __restrict__ dis) {
-    const uint32_t a = blockIdx.x * BLOCK + threadIdx.x;
-    if (a >= B.rows) return;
-    const size_t base = (size_t)B.slice_ptr[a >> 6] + (a & 63);
-    const int len = B.len[a];
-    for (int k = 0; k < len; ++k) {
-        const size_t sl = base + (size_t)k * SLICE;
-        const uint32_t J = B.bcol[sl];
-        for (int c = 0; c < 3; ++c)
-            for (int r = 0; r < R; ++r) {
-                const uint32_t q = pt_pos[pt_ptr[3 * (size_t)a + c] + (size_t)R * k + r];
-                const double sc = dis ? 1.0 / dis[(size_t)R * J + r] : 1.0;
-                B.val[(size_t)(c * R + r) * B.slots + sl] = (float)(p[q] * sc);
-            }
+    const size_t sl = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (sl >= B.slots) return;
+    // the slice holding slot sl: the last s with slice_ptr[s] <= sl
+    uint32_t lo = 0, hi = slices;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (B.slice_ptr[mid] <= sl) lo = mid; else hi = mid;
     }
+    const size_t off = sl - B.slice_ptr[lo];
+    const uint32_t a = lo * SLICE + (uint32_t)(off % SLICE);
+    const uint32_t k = (uint32_t)(off / SLICE);
+    if (a >= B.rows || k >= B.len[a]) return;
+    const uint32_t J = B.bcol[sl];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < R; ++r) {
+            const uint32_t q = pt_pos[pt_ptr[3 * (size_t)a + c] + (size_t)R * k + r];
+            const double sc = dis ? 1.0 / dis[(size_t)R * J + r] : 1.0;
+            B.val[(size_t)(c * R + r) * B.slots + sl] = (float)(p[q] * sc);
+        }
 }
 
 // W = A P and A_c = P^T W, one thread per output block.  The host built
@@ -296,8 +311,11 @@ __device__ __forceinline__ uint32_t find_group(const uint32_t* __restrict__ col,
     return lo < len && col[b + 3 * lo] == b3 ? lo : len;
 }
 
-// W(I, b) = sum over neighbour blocks J of A(I, J) P(J, b)
-template <int R>
+// W(I, b) = sum over neighbour blocks J of A(I, J) P(J, b).  LANES
+// consecutive lanes take an output block (each sums every LANES-th
+// neighbour, then a butterfly adds the lanes' sums): more lanes on the small
+// levels, whose few output blocks each sum many products.
+template <int R, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_amg_w(const uint32_t* __restrict__ a_ptr,
                                                  const uint32_t* __restrict__ a_col,
                                                  const double* __restrict__ a,
@@ -308,8 +326,10 @@ __global__ __launch_bounds__(BLOCK) void k_amg_w(const uint32_t* __restrict__ a_
                                                  const uint32_t* __restrict__ w_col,
                                                  double* __restrict__ w, uint32_t nodes,
                                                  size_t blocks) {
-    const size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= blocks) return;
+    const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t t = tid / LANES;
+    const uint32_t lane = (uint32_t)(tid % LANES);
+    if (t >= blocks) return;   // (a whole group: blocks end at a group boundary)
     const uint32_t I = block_row(w_ptr, R, 3 * R, t, nodes);
     const uint32_t j = (uint32_t)(t - w_ptr[(size_t)R * I] / (3 * R));
     const uint32_t b3 = w_col[w_ptr[(size_t)R * I] + 3 * j];
@@ -318,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_w(const uint32_t* __restrict__ a_
     for (int r = 0; r < R; ++r) ar[r] = a_ptr[(size_t)R * I + r];
     const uint32_t al = (a_ptr[(size_t)R * I + 1] - ar[0]) / R;
     double acc[R][3] = {};
-    for (uint32_t k = 0; k < al; ++k) {
+    for (uint32_t k = lane; k < al; k += LANES) {
         const uint32_t J = a_col[ar[0] + R * k] / R;
         const uint32_t p0 = p_ptr[(size_t)R * J];
         const uint32_t kk = find_group(p_col, p0, (p_ptr[(size_t)R * J + 1] - p0) / 3, b3);
@@ -339,21 +359,31 @@ __global__ __launch_bounds__(BLOCK) void k_amg_w(const uint32_t* __restrict__ a_
             }
     }
 #pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) acc[r][m] += __shfl_xor(acc[r][m], off, LANES);
+    if (lane != 0) return;
+#pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int m = 0; m < 3; ++m) w[w_ptr[(size_t)R * I + r] + 3 * j + m] = acc[r][m];
 }
 
-// A_c(a, b) = sum over the nodes J holding aggregate a of P(J, a)^T W(J, b)
-template <int R>
+// A_c(a, b) = sum over the nodes J holding aggregate a of P(J, a)^T W(J, b),
+// LANES lanes per output block as in k_amg_w
+template <int R, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_amg_galerkin(
     const uint32_t* __restrict__ pt_ptr, const uint32_t* __restrict__ pt_row,
     const uint32_t* __restrict__ pt_pos, const double* __restrict__ p,
     const uint32_t* __restrict__ w_ptr, const uint32_t* __restrict__ w_col,
     const double* __restrict__ w, const uint32_t* __restrict__ c_ptr,
     const uint32_t* __restrict__ c_col, double* __restrict__ ac, uint32_t aggs, size_t blocks) {
-    const size_t t = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= blocks) return;
+    const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t t = tid / LANES;
+    const uint32_t lane = (uint32_t)(tid % LANES);
+    if (t >= blocks) return;   // (a whole group)
     const uint32_t A = block_row(c_ptr, 3, 9, t, aggs);
     const uint32_t j = (uint32_t)(t - c_ptr[3 * (size_t)A] / 9);
     const uint32_t b3 = c_col[c_ptr[3 * (size_t)A] + 3 * j];
@@ -362,7 +392,7 @@ __global__ __launch_bounds__(BLOCK) void k_amg_galerkin(
     for (int m = 0; m < 3; ++m) tr[m] = pt_ptr[3 * (size_t)A + m];
     const uint32_t hl = (pt_ptr[3 * (size_t)A + 1] - tr[0]) / R;
     double acc[3][3] = {};
-    for (uint32_t h = 0; h < hl; ++h) {
+    for (uint32_t h = lane; h < hl; h += LANES) {
         const uint32_t J = pt_row[tr[0] + R * h] / R;
         const uint32_t w0 = w_ptr[(size_t)R * J];
         const uint32_t wl = (w_ptr[(size_t)R * J + 1] - w0) / 3;
@@ -386,6 +416,13 @@ __global__ __launch_bounds__(BLOCK) void k_amg_galerkin(
                 acc[m][n] += s;
             }
     }
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) acc[m][n] += __shfl_xor(acc[m][n], off, LANES);
+    if (lane != 0) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
@@ -825,6 +862,38 @@ static int amg_symbolic(FemSys& S) {
     return NBX_OK;
 }
 
+// f(integral_constant LANES) for a block layout's lanes per row
+template <class F>
+static void with_bl(int lanes, F&& f) {
+    switch (lanes) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 16>{}); break;
+    }
+}
+// f(integral_constant R) for a level's rows per node
+template <class F>
+static void with_bs(uint32_t bs, F&& f) {
+    if (bs == 2)
+        f(std::integral_constant<int, 2>{});
+    else
+        f(std::integral_constant<int, 3>{});
+}
+
+// f(integral_constant LANES) for the numeric phase's products: lanes per
+// output block, more on the small levels
+template <class F>
+static void with_out_lanes(size_t blocks, F&& f) {
+    if (blocks >= (size_t)1 << 18)
+        f(std::integral_constant<int, 1>{});
+    else if (blocks >= (size_t)1 << 14)
+        f(std::integral_constant<int, 4>{});
+    else
+        f(std::integral_constant<int, 16>{});
+}
+
 // Numeric phase for the current values of K.
 static int amg_numeric(FemSys& S) {
     AmgDev* A = S.K.amg;
@@ -841,7 +910,9 @@ static int amg_numeric(FemSys& S) {
         // (the finest level's smoothing reads the fp16 scaled copy; the
         // coarser levels' operators are filled in block form by the level
         // above)
-        // lambda_max(D^-1 A): 15 power steps from a fixed start
+        // lambda_max(D^-1 A): 15 power steps from a fixed start (the
+        // coarsest level, solved directly, needs none)
+        if (l + 1 < A->lv.size()) {
         hipLaunchKernelGGL(k_amg_pow_init, dim3(g), dim3(BLOCK), 0, s, A->pow_x, L.n);
         hipLaunchKernelGGL(k_amg_sq, dim3(g), dim3(BLOCK), 0, s, A->pow_x, A->partials, L.n);
         hipLaunchKernelGGL(k_amg_pow_fin, dim3(1), dim3(FIN), 0, s, A->partials, g, A->scal);
@@ -873,6 +944,7 @@ static int amg_numeric(FemSys& S) {
                 rho = rn;
             }
         }
+        }
         if (l + 1 == A->lv.size()) {
             // coarsest: its inverse, on the host (at most AMG_COARSEST^2 entries)
             std::vector<uint32_t> ptr(L.n + 1), col(L.a_nnz);
@@ -899,39 +971,42 @@ static int amg_numeric(FemSys& S) {
         }
         AmgDevLevel& C = A->lv[l + 1];
         const double omega = 4.0 / (3.0 * L.lmax);
-        hipLaunchKernelGGL(k_amg_smooth_p, dim3(g), dim3(BLOCK), 0, s, L.a_ptr, L.a_col, L.a64,
-                           L.dinv64, L.t, L.self, L.p_ptr, L.ap_map, L.p64, omega, L.n);
+        hipLaunchKernelGGL(k_amg_smooth_p, dim3(grid_for(L.p_nnz / 3)), dim3(BLOCK), 0, s, L.a_ptr,
+                           L.a_col, L.a64, L.dinv64, L.t, L.self, L.p_ptr, L.ap_map, L.p64, omega,
+                           L.n, (size_t)(L.p_nnz / 3));
         // P and P^T in block form (D^1/2 P on the finest level when it is
         // smoothed in the scaled variables)
         const double* pdis = l == 0 && A->scaled16 ? S.K.dis : nullptr;
         if (L.bs == 2) {
             hipLaunchKernelGGL((k_bell_fill<2, 3>), dim3(grid_for(L.pb.rows)), dim3(BLOCK), 0, s,
                                L.pb, L.p_ptr, L.p64, pdis);
-            hipLaunchKernelGGL((k_bell_fill_t<2>), dim3(grid_for(L.tb.rows)), dim3(BLOCK), 0, s,
-                               L.tb, L.pt_ptr, L.pt_pos, L.p64, pdis);
+            hipLaunchKernelGGL((k_bell_fill_t<2>), dim3(grid_for(L.tb.slots)), dim3(BLOCK), 0, s,
+                               L.tb, (L.tb.rows + SLICE - 1) / SLICE, L.pt_ptr, L.pt_pos, L.p64,
+                               pdis);
         } else {
             hipLaunchKernelGGL((k_bell_fill<3, 3>), dim3(grid_for(L.pb.rows)), dim3(BLOCK), 0, s,
                                L.pb, L.p_ptr, L.p64, pdis);
-            hipLaunchKernelGGL((k_bell_fill_t<3>), dim3(grid_for(L.tb.rows)), dim3(BLOCK), 0, s,
-                               L.tb, L.pt_ptr, L.pt_pos, L.p64, pdis);
+            hipLaunchKernelGGL((k_bell_fill_t<3>), dim3(grid_for(L.tb.slots)), dim3(BLOCK), 0, s,
+                               L.tb, (L.tb.rows + SLICE - 1) / SLICE, L.pt_ptr, L.pt_pos, L.p64,
+                               pdis);
         }
         const uint32_t nodes = L.n / L.bs, aggs = L.nc / 3;
         const size_t wblocks = L.w_nnz / (3 * L.bs), cblocks = C.a_nnz / 9;
-        if (L.bs == 2) {
-            hipLaunchKernelGGL(k_amg_w<2>, dim3(grid_for(wblocks)), dim3(BLOCK), 0, s, L.a_ptr,
-                               L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64,
-                               nodes, wblocks);
-            hipLaunchKernelGGL(k_amg_galerkin<2>, dim3(grid_for(cblocks)), dim3(BLOCK), 0, s,
-                               L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64,
-                               C.a_ptr, C.a_col, C.a64, aggs, cblocks);
-        } else {
-            hipLaunchKernelGGL(k_amg_w<3>, dim3(grid_for(wblocks)), dim3(BLOCK), 0, s, L.a_ptr,
-                               L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr, L.w_col, L.w64,
-                               nodes, wblocks);
-            hipLaunchKernelGGL(k_amg_galerkin<3>, dim3(grid_for(cblocks)), dim3(BLOCK), 0, s,
-                               L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr, L.w_col, L.w64,
-                               C.a_ptr, C.a_col, C.a64, aggs, cblocks);
-        }
+        with_bs(L.bs, [&](auto rr) {
+            constexpr int R = decltype(rr)::value;
+            with_out_lanes(wblocks, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                hipLaunchKernelGGL((k_amg_w<R, LN>), dim3(grid_for(wblocks * LN)), dim3(BLOCK), 0,
+                                   s, L.a_ptr, L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr,
+                                   L.w_col, L.w64, nodes, wblocks);
+            });
+            with_out_lanes(cblocks, [&](auto ln) {
+                constexpr int LN = decltype(ln)::value;
+                hipLaunchKernelGGL((k_amg_galerkin<R, LN>), dim3(grid_for(cblocks * LN)),
+                                   dim3(BLOCK), 0, s, L.pt_ptr, L.pt_row, L.pt_pos, L.p64, L.w_ptr,
+                                   L.w_col, L.w64, C.a_ptr, C.a_col, C.a64, aggs, cblocks);
+            });
+        });
         if (C.ab.rows)
             hipLaunchKernelGGL((k_bell_fill<3, 3>), dim3(grid_for(C.ab.rows)), dim3(BLOCK), 0, s,
                                C.ab, C.a_ptr, C.a64, (const double*)nullptr);
@@ -997,26 +1072,6 @@ AmgFine amg_fine(const AmgDev* A) {
 const double* amg_ones(const AmgDev* A) { return A->ones; }
 uint32_t amg_levels(const AmgDev* A) { return (uint32_t)A->lv.size(); }
 
-
-// f(integral_constant LANES) for a block layout's lanes per row
-template <class F>
-static void with_bl(int lanes, F&& f) {
-    switch (lanes) {
-        case 1: f(std::integral_constant<int, 1>{}); break;
-        case 2: f(std::integral_constant<int, 2>{}); break;
-        case 4: f(std::integral_constant<int, 4>{}); break;
-        case 8: f(std::integral_constant<int, 8>{}); break;
-        default: f(std::integral_constant<int, 16>{}); break;
-    }
-}
-// f(integral_constant R) for a level's rows per node
-template <class F>
-static void with_bs(uint32_t bs, F&& f) {
-    if (bs == 2)
-        f(std::integral_constant<int, 2>{});
-    else
-        f(std::integral_constant<int, 3>{});
-}
 
 // b_{l+1} = P_l^T r_l, with the next level's smoothing start unless that
 // level is the coarsest
