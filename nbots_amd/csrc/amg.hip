@@ -194,36 +194,59 @@ __global__ __launch_bounds__(FIN) void k_amg_pow_fin(const double* partials, uns
     }
 }
 
-// P = T - omega D^-1 A T: one thread per group of three P entries (row i,
-// the columns of one aggregate, positions q .. q + 2), summing the row's A
-// entries that map to q in entry order
+// P = T - omega D^-1 A T: SMOOTH_SLOTS threads per node, each taking every
+// SMOOTH_SLOTS-th aggregate column group of the node's P rows (its R rows
+// share one A and one P pattern) and summing each row's A entries that map
+// to the group in entry order
+constexpr int SMOOTH_SLOTS = 8;
+template <int R>
 __global__ __launch_bounds__(BLOCK) void k_amg_smooth_p(
     const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_col,
     const double* __restrict__ a, const double* __restrict__ dinv, const double* __restrict__ t,
     const uint32_t* __restrict__ self, const uint32_t* __restrict__ p_ptr,
-    const uint32_t* __restrict__ ap_map, double* __restrict__ p, double omega, uint32_t n,
-    size_t groups) {
-    const size_t g = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (g >= groups) return;
-    const uint32_t q = (uint32_t)(3 * g);
-    // the row holding position q: the last i with p_ptr[i] <= q
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (p_ptr[mid] <= q) lo = mid; else hi = mid;
+    const uint32_t* __restrict__ ap_map, double* __restrict__ p, double omega, uint32_t nodes) {
+    const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t I = (uint32_t)(tid / SMOOTH_SLOTS);
+    if (I >= nodes) return;
+    const uint32_t i0 = R * I;
+    const uint32_t ng = (p_ptr[i0 + 1] - p_ptr[i0]) / 3;
+    double od[R];
+    uint32_t ao[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        od[r] = omega * dinv[i0 + r];
+        ao[r] = a_ptr[i0 + r] - a_ptr[i0];   // row r's entry e + ao[r] pairs with e
     }
-    const uint32_t i = lo;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (self[i] == q)
-        for (int m = 0; m < 3; ++m) acc[m] += t[3 * (size_t)i + m];
-    const double od = omega * dinv[i];
-    for (uint32_t e = a_ptr[i]; e < a_ptr[i + 1]; ++e) {
-        if (ap_map[e] != q) continue;
-        const double coef = od * a[e];
-        const uint32_t j = a_col[e];
-        for (int m = 0; m < 3; ++m) acc[m] -= coef * t[3 * (size_t)j + m];
+    for (uint32_t gi = (uint32_t)(tid % SMOOTH_SLOTS); gi < ng; gi += SMOOTH_SLOTS) {
+        const uint32_t qo = 3 * gi;          // the group's offset in each row
+        const uint32_t q = p_ptr[i0] + qo;
+        double acc[R][3];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) acc[r][m] = 0.0;
+            if (self[i0 + r] == p_ptr[i0 + r] + qo)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) acc[r][m] += t[3 * (size_t)(i0 + r) + m];
+        }
+        for (uint32_t e = a_ptr[i0]; e < a_ptr[i0 + 1]; ++e) {
+            if (ap_map[e] != q) continue;
+            const uint32_t j = a_col[e];
+            double tj[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) tj[m] = t[3 * (size_t)j + m];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double coef = od[r] * a[e + ao[r]];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) acc[r][m] -= coef * tj[m];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) p[p_ptr[i0 + r] + qo + m] = acc[r][m];
     }
-    for (int m = 0; m < 3; ++m) p[q + m] = acc[m];
 }
 
 // Block values from a CSR whose R rows per block row share one pattern of
@@ -883,10 +906,10 @@ static void with_bs(uint32_t bs, F&& f) {
 }
 
 // f(integral_constant LANES) for the numeric phase's products: lanes per
-// output block, more on the small levels
+// output block, more on the small levels (0 blocks: one lane)
 template <class F>
 static void with_out_lanes(size_t blocks, F&& f) {
-    if (blocks >= (size_t)1 << 18)
+    if (blocks == 0)
         f(std::integral_constant<int, 1>{});
     else if (blocks >= (size_t)1 << 14)
         f(std::integral_constant<int, 4>{});
@@ -971,9 +994,13 @@ static int amg_numeric(FemSys& S) {
         }
         AmgDevLevel& C = A->lv[l + 1];
         const double omega = 4.0 / (3.0 * L.lmax);
-        hipLaunchKernelGGL(k_amg_smooth_p, dim3(grid_for(L.p_nnz / 3)), dim3(BLOCK), 0, s, L.a_ptr,
-                           L.a_col, L.a64, L.dinv64, L.t, L.self, L.p_ptr, L.ap_map, L.p64, omega,
-                           L.n, (size_t)(L.p_nnz / 3));
+        with_bs(L.bs, [&](auto rr) {
+            constexpr int R = decltype(rr)::value;
+            hipLaunchKernelGGL(k_amg_smooth_p<R>,
+                               dim3(grid_for((size_t)(L.n / R) * SMOOTH_SLOTS)), dim3(BLOCK), 0,
+                               s, L.a_ptr, L.a_col, L.a64, L.dinv64, L.t, L.self, L.p_ptr,
+                               L.ap_map, L.p64, omega, L.n / R);
+        });
         // P and P^T in block form (D^1/2 P on the finest level when it is
         // smoothed in the scaled variables)
         const double* pdis = l == 0 && A->scaled16 ? S.K.dis : nullptr;
@@ -994,7 +1021,9 @@ static int amg_numeric(FemSys& S) {
         const size_t wblocks = L.w_nnz / (3 * L.bs), cblocks = C.a_nnz / 9;
         with_bs(L.bs, [&](auto rr) {
             constexpr int R = decltype(rr)::value;
-            with_out_lanes(wblocks, [&](auto ln) {
+            // (the finest level's W blocks sum about nine products each: one
+            // lane per block measured fastest there)
+            with_out_lanes(R == 2 ? 0 : wblocks, [&](auto ln) {
                 constexpr int LN = decltype(ln)::value;
                 hipLaunchKernelGGL((k_amg_w<R, LN>), dim3(grid_for(wblocks * LN)), dim3(BLOCK), 0,
                                    s, L.a_ptr, L.a_col, L.a64, L.p_ptr, L.p_col, L.p64, L.w_ptr,
