@@ -972,8 +972,8 @@ static int amg_numeric(FemSys& S) {
             // coarsest: its inverse, on the host (at most AMG_COARSEST^2 entries)
             std::vector<uint32_t> ptr(L.n + 1), col(L.a_nnz);
             std::vector<double> a(L.a_nnz);
-            NBX_HIP(copy_on(ptr.data(), L.a_ptr, ptr.size() * 4, hipMemcpyDeviceToHost, s));
-            NBX_HIP(copy_on(col.data(), L.a_col, col.size() * 4, hipMemcpyDeviceToHost, s));
+            NBX_HIP(hipMemcpyAsync(ptr.data(), L.a_ptr, ptr.size() * 4, hipMemcpyDeviceToHost, s));
+            NBX_HIP(hipMemcpyAsync(col.data(), L.a_col, col.size() * 4, hipMemcpyDeviceToHost, s));
             NBX_HIP(copy_on(a.data(), L.a64, a.size() * 8, hipMemcpyDeviceToHost, s));
             std::vector<double>& inv = A->host_inv;
             std::fill(inv.begin(), inv.end(), 0.0);

@@ -444,15 +444,30 @@ int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, cons
     }
 }
 
+// x . y over len contiguous entries, four partial sums in a fixed order
+static double dense_dot(const double* x, const double* y, uint32_t len) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    uint32_t p = 0;
+    for (; p + 4 <= len; p += 4) {
+        s0 += x[p] * y[p];
+        s1 += x[p + 1] * y[p + 1];
+        s2 += x[p + 2] * y[p + 2];
+        s3 += x[p + 3] * y[p + 3];
+    }
+    for (; p < len; ++p) s0 += x[p] * y[p];
+    return (s0 + s1) + (s2 + s3);
+}
+
 int amg_dense_inverse(uint32_t n, double* a) {
-    // Cholesky, lower triangle in place
+    // Cholesky, lower triangle in place (every inner product over
+    // contiguous rows)
     double dmax = 0.0;
     for (uint32_t i = 0; i < n; ++i) dmax = fmax(dmax, a[(size_t)i * n + i]);
     std::vector<uint8_t> drop(n, 0);
     int dropped = 0;
     for (uint32_t k = 0; k < n; ++k) {
-        double s = a[(size_t)k * n + k];
-        for (uint32_t p = 0; p < k; ++p) s -= a[(size_t)k * n + p] * a[(size_t)k * n + p];
+        double* rk = a + (size_t)k * n;
+        const double s = rk[k] - dense_dot(rk, rk, k);
         if (!(s > 1e-13 * dmax)) {
             drop[k] = 1;
             ++dropped;
@@ -460,29 +475,29 @@ int amg_dense_inverse(uint32_t n, double* a) {
             continue;
         }
         const double lkk = sqrt(s);
-        a[(size_t)k * n + k] = lkk;
+        rk[k] = lkk;
         for (uint32_t i = k + 1; i < n; ++i) {
-            double t = a[(size_t)i * n + k];
-            for (uint32_t p = 0; p < k; ++p) t -= a[(size_t)i * n + p] * a[(size_t)k * n + p];
-            a[(size_t)i * n + k] = t / lkk;
+            double* ri = a + (size_t)i * n;
+            ri[k] = (ri[k] - dense_dot(ri, rk, k)) / lkk;
         }
     }
-    // M = L^-1 (lower), rows of dropped unknowns 0
-    std::vector<double> M((size_t)n * n, 0.0);
-    for (uint32_t i = 0; i < n; ++i) {
-        if (drop[i]) continue;
-        const double lii = a[(size_t)i * n + i];
-        for (uint32_t j = 0; j <= i; ++j) {
-            double t = (i == j) ? 1.0 : 0.0;
-            for (uint32_t p = j; p < i; ++p) t -= a[(size_t)i * n + p] * M[(size_t)p * n + j];
-            M[(size_t)i * n + j] = t / lii;
+    // MT = (L^-1)^T: row j of MT is column j of L^-1 (entries j .. n-1);
+    // the rows of L^-1 of dropped unknowns are 0
+    std::vector<double> MT((size_t)n * n, 0.0);
+    for (uint32_t j = 0; j < n; ++j) {
+        double* mj = MT.data() + (size_t)j * n;
+        for (uint32_t i = j; i < n; ++i) {
+            if (drop[i]) continue;
+            const double* li = a + (size_t)i * n;
+            const double t = (i == j ? 1.0 : 0.0) - dense_dot(li + j, mj + j, i - j);
+            mj[i] = t / li[i];
         }
     }
-    // A^-1 = M^T M
+    // A^-1 = (L^-1)^T L^-1: entry (i, j), i >= j, sums over p >= i
     for (uint32_t i = 0; i < n; ++i)
         for (uint32_t j = 0; j <= i; ++j) {
-            double t = 0.0;
-            for (uint32_t p = i; p < n; ++p) t += M[(size_t)p * n + i] * M[(size_t)p * n + j];
+            const double t = dense_dot(MT.data() + (size_t)i * n + i, MT.data() + (size_t)j * n + i,
+                                       n - i);
             a[(size_t)i * n + j] = t;
             a[(size_t)j * n + i] = t;
         }
