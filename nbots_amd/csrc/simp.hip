@@ -81,25 +81,45 @@ __device__ __forceinline__ double oc_candidate(double xe, double dce, double dve
     return v;
 }
 
-// One bisection step: the filtered volume of the candidate design,
-// sum_e a_e (H x_new)_e / Hs_e, which is x_new . (H (a ./ Hs)) = x_new . dv
-// because H is symmetric; so a step reads x, dc and dv once and filters
-// nothing (per-workgroup partials).
-__global__ __launch_bounds__(BLOCK) void k_oc_step(const OcState* __restrict__ st,
-                                                   const double* __restrict__ x,
-                                                   const double* __restrict__ dc,
-                                                   const double* __restrict__ dv, double move,
-                                                   uint32_t ne, double* partials) {
-    if (st->done) return;   // uniform over the grid
-    const double lmid = st->lmid;
-    double acc[1] = {0.0};
+// The bisection's next three levels in one pass.  From the bracket (l1, l2)
+// and its midpoint c0 the next two steps can test only 0.5 (c0 + l1) or
+// 0.5 (l2 + c0), then one of four midpoints; the pass evaluates all seven
+// candidates, each multiplier formed exactly as the step that would test it
+// forms it (0.5 (l2 + l1) of that step's bracket), and k_oc_fin3 walks the
+// tree one step at a time with the sequential test.  Candidate k's volume is
+// the filtered volume of its design, sum_e a_e (H x_new)_e / Hs_e =
+// x_new . (H (a ./ Hs)) = x_new . dv because H is symmetric, so a pass reads
+// x, dc and dv once and filters nothing (per-workgroup partials, seven
+// sums each in the order a one-candidate pass would take).
+constexpr int OC_CAND = 7;   // c0; c1 = (L, R); c2 = (LL, LR, RL, RR)
+__device__ __forceinline__ void oc_candidates(const OcState& s, double (&c)[OC_CAND]) {
+    // L: the volume did not exceed the target (l2 = mid); R: it did (l1 = mid)
+    c[0] = s.lmid;
+    c[1] = 0.5 * (c[0] + s.l1);    // bracket (l1, c0)
+    c[2] = 0.5 * (s.l2 + c[0]);    // bracket (c0, l2)
+    c[3] = 0.5 * (c[1] + s.l1);    // (l1, c1L)
+    c[4] = 0.5 * (c[0] + c[1]);    // (c1L, c0)
+    c[5] = 0.5 * (c[2] + c[0]);    // (c0, c1R)
+    c[6] = 0.5 * (s.l2 + c[2]);    // (c1R, l2)
+}
+__global__ __launch_bounds__(BLOCK) void k_oc_step3(const OcState* __restrict__ st,
+                                                    const double* __restrict__ x,
+                                                    const double* __restrict__ dc,
+                                                    const double* __restrict__ dv, double move,
+                                                    uint32_t ne, double* partials) {
+    const OcState s = *st;
+    if (s.done) return;   // uniform over the grid
+    double c[OC_CAND];
+    oc_candidates(s, c);
+    double acc[OC_CAND] = {};
     for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < ne; e += gridDim.x * BLOCK) {
-        const double dve = dv[e];
-        acc[0] += dve * oc_candidate(x[e], dc[e], dve, lmid, move);
+        const double xe = x[e], dce = dc[e], dve = dv[e];
+#pragma unroll
+        for (int k = 0; k < OC_CAND; ++k) acc[k] += dve * oc_candidate(xe, dce, dve, c[k], move);
     }
-    __shared__ double red[1][BLOCK / WAVE];
-    block_sum<1>(acc, red);
-    put_partials<1>(acc, partials);
+    __shared__ double red[OC_CAND][BLOCK / WAVE];
+    block_sum<OC_CAND>(acc, red);
+    put_partials<OC_CAND>(acc, partials);
 }
 
 // the design of the last multiplier the bisection tried, and its filtered
@@ -118,22 +138,29 @@ __global__ void k_oc_phys(DevFilter H, const double* __restrict__ xnew,
     xphys[e] = filt_row<false>(H, e, xnew) / H.Hs[e];
 }
 
-// volume test, next bracket and multiplier
-__global__ __launch_bounds__(FIN) void k_oc_fin(OcState* st, const double* partials, unsigned np) {
+// up to three bisection steps on the pass's volumes: each the volume test,
+// the next bracket and multiplier (or the end of the bisection)
+__global__ __launch_bounds__(FIN) void k_oc_fin3(OcState* st, const double* partials,
+                                                 unsigned np) {
     if (st->done) return;
-    double tot[1];
-    fin_sum<1>(partials, np, tot);
+    double tot[OC_CAND];
+    fin_sum<OC_CAND>(partials, np, tot);
     if (threadIdx.x == 0) {
         OcState s = *st;
-        if (tot[0] > s.target)
-            s.l1 = s.lmid;
-        else
-            s.l2 = s.lmid;
-        if ((s.l2 - s.l1) / (s.l1 + s.l2) > 1e-3)
-            s.lmid = 0.5 * (s.l2 + s.l1);
-        else
-            s.done = 1;
-        s.steps += 1;
+        int k = 0;   // the candidate this step tests (s.lmid)
+        for (int level = 0; level < 3 && !s.done; ++level) {
+            const bool above = tot[k] > s.target;
+            if (above)
+                s.l1 = s.lmid;
+            else
+                s.l2 = s.lmid;
+            if ((s.l2 - s.l1) / (s.l1 + s.l2) > 1e-3)
+                s.lmid = 0.5 * (s.l2 + s.l1);
+            else
+                s.done = 1;
+            s.steps += 1;
+            k = 2 * k + (above ? 2 : 1);   // the child candidate
+        }
         *st = s;
     }
 }
@@ -370,20 +397,24 @@ int simp_update(Simp& o, FemSys& S, double* change) {
     hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(WAVE), 0, s, o.st, o.volfrac * o.atot);
     unsigned g = grid_for(ne);
     if (g > (unsigned)MAX_GRID) g = MAX_GRID;
-    // Bisection steps are launched in batches; kernels of a finished
-    // bisection return at once.  From (0, 1e9) the bracket reaches any
-    // multiplier above 1e-300 within about 1050 halvings.
-    constexpr int BATCH = 64, MAX_STEPS = 2048;
+    // Bisection passes (three steps each) are launched in batches, the
+    // first as long as the previous update's bisection; kernels of a
+    // finished bisection return at once.  From (0, 1e9) the bracket reaches
+    // any multiplier above 1e-300 within about 1050 halvings.
+    constexpr int MAX_STEPS = 2048;
+    int batch = o.last_oc_steps ? (int)(o.last_oc_steps + 2) / 3 : 16;
     OcState h{};
-    for (int done_steps = 0;; done_steps += BATCH) {
+    // (seven partials per workgroup in the workspace's 4 MAX_GRID)
+    const unsigned g3 = std::min(g, (unsigned)(4 * MAX_GRID / OC_CAND));
+    for (int done_steps = 0;; done_steps += 3 * batch, batch = 4) {
         if (done_steps >= MAX_STEPS) {
             set_last_error("simp: OC bisection did not close in %d steps", MAX_STEPS);
             return NBX_ERR_BREAKDOWN;
         }
-        for (int k = 0; k < BATCH; ++k) {
-            hipLaunchKernelGGL(k_oc_step, dim3(g), dim3(BLOCK), 0, s, o.st, o.x, o.dcf, o.dvf,
+        for (int k = 0; k < batch; ++k) {
+            hipLaunchKernelGGL(k_oc_step3, dim3(g3), dim3(BLOCK), 0, s, o.st, o.x, o.dcf, o.dvf,
                                o.move, ne, S.ws.partials);
-            hipLaunchKernelGGL(k_oc_fin, dim3(1), dim3(FIN), 0, s, o.st, S.ws.partials, g);
+            hipLaunchKernelGGL(k_oc_fin3, dim3(1), dim3(FIN), 0, s, o.st, S.ws.partials, g3);
         }
         NBX_HIP(hipGetLastError());
         NBX_HIP(hipMemcpyAsync(&h, o.st, sizeof h, hipMemcpyDeviceToHost, s));
