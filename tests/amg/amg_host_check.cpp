@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <string>
 #include <vector>
 
 #include "amg.hpp"
@@ -135,7 +136,58 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 
 }  // namespace
 
+// --dense: amg_dense_inverse on two matrices with known answers, one line
+//   spd_err=<max |A X - I|> lap_dropped=<d> lap_err=<max |A_KK X_KK - I|>
+// (an SPD matrix of order 200, and the path-graph Laplacian of order 150,
+// singular with the constants as its null space: the Cholesky pivot guard
+// drops its last unknown, and the inverse of the rest is exact)
+int dense_check() {
+    auto max_err = [](uint32_t n, const std::vector<double>& A, const std::vector<double>& X,
+                      const std::vector<uint8_t>& keep) {
+        double e = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!keep[i]) continue;
+            for (uint32_t j = 0; j < n; ++j) {
+                if (!keep[j]) continue;
+                double s = 0.0;
+                for (uint32_t k = 0; k < n; ++k)
+                    if (keep[k]) s += A[(size_t)i * n + k] * X[(size_t)k * n + j];
+                e = fmax(e, fabs(s - (i == j ? 1.0 : 0.0)));
+            }
+        }
+        return e;
+    };
+    uint32_t n = 200;
+    std::vector<double> A((size_t)n * n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = 0; j < n; ++j)
+            A[(size_t)i * n + j] = 1.0 / (1.0 + fabs((double)i - (double)j)) + (i == j ? 3.0 : 0.0);
+    std::vector<double> X = A;
+    const int d0 = amg_dense_inverse(n, X.data());
+    const double e0 = max_err(n, A, X, std::vector<uint8_t>(n, 1));
+    n = 150;
+    std::vector<double> Lp((size_t)n * n, 0.0);
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+        Lp[(size_t)i * n + i] += 1.0;
+        Lp[(size_t)(i + 1) * n + i + 1] += 1.0;
+        Lp[(size_t)i * n + i + 1] -= 1.0;
+        Lp[(size_t)(i + 1) * n + i] -= 1.0;
+    }
+    std::vector<double> Y = Lp;
+    const int d1 = amg_dense_inverse(n, Y.data());
+    std::vector<uint8_t> keep(n, 1);
+    keep[n - 1] = 0;
+    double zero = 0.0;   // the dropped row and column of the inverse
+    for (uint32_t k = 0; k < n; ++k)
+        zero = fmax(zero, fmax(fabs(Y[(size_t)(n - 1) * n + k]), fabs(Y[(size_t)k * n + n - 1])));
+    const double e1 = max_err(n, Lp, Y, keep);
+    printf("spd_dropped=%d spd_err=%.3e lap_dropped=%d lap_err=%.3e lap_dropped_entries=%.3e\n", d0,
+           e0, d1, e1, zero);
+    return d0 == 0 && e0 < 1e-12 && d1 == 1 && e1 < 1e-9 && zero == 0.0 ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 2 && std::string(argv[1]) == "--dense") return dense_check();
     if (argc < 2) {
         fprintf(stderr, "usage: amg_host_check <system.bin> [max_iter] [omega_scale]\n");
         return 2;

@@ -12,7 +12,10 @@ conditions); the bars:
     against 65-91 outer iterations of degree-40 Chebyshev ones;
   * the hierarchy coarsens to at most 200 unknowns;
   * without the prolongator's smoothing (plain aggregation) the same solve
-    takes clearly more iterations, so the check sees the smoothing.
+    takes clearly more iterations, so the check sees the smoothing;
+  * the coarsest level's dense inverse (Cholesky with a pivot guard) inverts
+    an SPD matrix of order 200 and, on a singular Laplacian, drops exactly the
+    unknown its null space leaves and inverts the rest.
 """
 import os
 import shutil
@@ -109,3 +112,11 @@ def test_prolongator_smoothing_matters(host_check, tmp_path):
     r0, kv0 = _run(host_check, path, 500, 0.0)
     assert r1.returncode == 0 and r0.returncode == 0, r1.stdout + r0.stdout
     assert int(kv0["iters"]) >= 1.5 * int(kv1["iters"]), (kv0, kv1)
+
+
+def test_dense_inverse(host_check):
+    r = subprocess.run([host_check, "--dense"], capture_output=True, text=True, timeout=120)
+    kv = dict(tok.split("=", 1) for tok in r.stdout.split() if "=" in tok)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert int(kv["spd_dropped"]) == 0 and float(kv["spd_err"]) < 1e-12, r.stdout
+    assert int(kv["lap_dropped"]) == 1 and float(kv["lap_err"]) < 1e-9, r.stdout
