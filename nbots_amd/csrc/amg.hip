@@ -67,7 +67,6 @@ struct AmgDevLevel {
 
 struct AmgDev {
     std::vector<AmgDevLevel> lv;
-    uint32_t* fine_map = nullptr;   // [nnz of the finest level] index into the 2x2-block values
     float* cinv = nullptr;          // coarsest inverse, n x n
     double* ones = nullptr;         // [finest n + 2] the flexible CG's unit scaling
     double* pow_x = nullptr;        // power iteration: [finest n + 2] x, y
@@ -102,11 +101,43 @@ int alloc(T** d, size_t n, size_t& bytes) {
 
 // ---- numeric phase ----------------------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_amg_fine_vals(const double* __restrict__ kval,
-                                                         const uint32_t* __restrict__ map,
-                                                         double* __restrict__ a, size_t nnz) {
-    const size_t e = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (e < nnz) a[e] = kval[map[e]];
+// the slice holding slot sl of a sliced layout: the last s < slices with
+// slice_ptr[s] <= sl
+__device__ __forceinline__ uint32_t slice_of(const uint32_t* __restrict__ slice_ptr,
+                                             uint32_t slices, size_t sl) {
+    uint32_t lo = 0, hi = slices;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (slice_ptr[mid] <= sl) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The finest level's CSR values from the 2x2-block layout, one thread per
+// slot: block k of node v holds entries (i, j) of rows 2v + i at positions
+// a_ptr[2v + i] + 2k + j (the CSR the host built from the same layout)
+__global__ __launch_bounds__(BLOCK) void k_amg_fine_vals(Sell2 K, const uint32_t* __restrict__ a_ptr,
+                                                         double* __restrict__ a) {
+    const size_t sl = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (sl >= K.n_blocks) return;
+    const uint32_t s = slice_of(K.slice_ptr, K.n_slices, sl);
+    const size_t off = sl - K.slice_ptr[s];
+    const uint32_t v = s * SLICE + (uint32_t)(off % SLICE);
+    const uint32_t k = (uint32_t)(off / SLICE);
+    if (v >= K.n_nodes || k >= K.rowlen[v]) return;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            a[a_ptr[2 * (size_t)v + i] + 2 * (size_t)k + j] = K.val[ent(K.n_blocks, sl, i, j)];
+}
+// bit 2 i + j of a slot: entry (i, j) of its block is nonzero
+__global__ __launch_bounds__(BLOCK) void k_block_nzbits(Sell2 K, uint8_t* __restrict__ bits) {
+    const size_t sl = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (sl >= K.n_blocks) return;
+    uint32_t b = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            if (K.val[ent(K.n_blocks, sl, i, j)] != 0.0) b |= 1u << (2 * i + j);
+    bits[sl] = (uint8_t)b;
 }
 
 // D^-1 (0 where the diagonal is not positive: an unknown with no coarse or
@@ -673,7 +704,7 @@ void amg_free(AmgDev* A) {
                 if (q) (void)hipFree(q);
         }
     }
-    void* ps[] = {A->fine_map, A->cinv, A->ones, A->pow_x, A->pow_y, A->scal, A->partials};
+    void* ps[] = {A->cinv, A->ones, A->pow_x, A->pow_y, A->scal, A->partials};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete A;
@@ -757,12 +788,40 @@ static int amg_symbolic(FemSys& S) {
                        (unsigned)(MAX_GRID * BLOCK), nnz);
         return NBX_ERR_ARG;
     }
-    std::vector<uint32_t> rp(n + 1), col(nnz);
-    std::vector<double> val(nnz), xy(2 * (size_t)S.nn);
-    NBX_TRY(fem_download_csr(S, rp.data(), col.data(), val.data()));
+    // the finest pattern and its nonzero entries, from the block layout's
+    // columns and one flag byte per block (the values stay on the device)
+    const Sell2& K = S.K.s;
+    std::vector<uint32_t> sp(K.n_slices + 1);
+    std::vector<uint8_t> len(S.nn + 1);
+    HostVec<uint32_t> bc(K.n_blocks + 1);
+    HostVec<uint8_t> bits(K.n_blocks + 1);
+    std::vector<double> xy(2 * (size_t)S.nn);
+    {
+        uint8_t* dbits = nullptr;
+        NBX_HIP(hipMalloc(&dbits, K.n_blocks + 1));
+        hipLaunchKernelGGL(k_block_nzbits, dim3(grid_for(K.n_blocks)), dim3(BLOCK), 0, S.s, K,
+                           dbits);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(bits.data(), dbits, K.n_blocks, hipMemcpyDeviceToHost, S.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(S.s);
+        (void)hipFree(dbits);
+        NBX_HIP(e);
+    }
+    NBX_HIP(hipMemcpyAsync(sp.data(), K.slice_ptr, (K.n_slices + 1) * 4, hipMemcpyDeviceToHost,
+                           S.s));
+    if (S.nn) NBX_HIP(hipMemcpyAsync(len.data(), K.rowlen, S.nn, hipMemcpyDeviceToHost, S.s));
+    NBX_HIP(hipMemcpyAsync(bc.data(), K.bcol, K.n_blocks * 4, hipMemcpyDeviceToHost, S.s));
     NBX_HIP(copy_on(xy.data(), S.nod, xy.size() * sizeof(double), hipMemcpyDeviceToHost, S.s));
+    HostVec<uint32_t> rp, col;
+    HostVec<uint8_t> nzf;
+    amg_expand_blocks(S.nn, sp.data(), len.data(), bc.data(), bits.data(), rp, col, nzf);
+    if (rp.size() != (size_t)n + 1 || col.size() != nnz) {
+        set_last_error("AMG: the block layout holds %zu entries, the pattern %zu", col.size(), nnz);
+        return NBX_ERR_ARG;
+    }
     AmgHost H;
-    const int hb = amg_build_host(n, rp.data(), col.data(), val.data(), xy.data(), H);
+    const int hb = amg_build_host(n, rp.data(), col.data(), nzf.data(), xy.data(), H);
     if (hb == -2) {
         set_last_error("AMG: host memory ran out building the hierarchy");
         return NBX_ERR_ARG;
@@ -776,28 +835,8 @@ static int amg_symbolic(FemSys& S) {
                        H.lv.back().n, AMG_COARSEST);
         return NBX_ERR_ARG;
     }
-    // index of every CSR entry's value in the 2x2-block layout: row 2v+k,
-    // block idx (the k-th block of node v) -> ent(idx, k, l)
-    std::vector<uint32_t> map(nnz);
-    {
-        const Sell2& K = S.K.s;
-        std::vector<uint32_t> sp(K.n_slices + 1);
-        std::vector<uint8_t> len(S.nn + 1);
-        NBX_HIP(copy_on(sp.data(), K.slice_ptr, (K.n_slices + 1) * 4, hipMemcpyDeviceToHost, S.s));
-        if (S.nn) NBX_HIP(copy_on(len.data(), K.rowlen, S.nn, hipMemcpyDeviceToHost, S.s));
-        for (uint32_t v = 0; v < S.nn; ++v) {
-            size_t idx = (size_t)sp[v >> 6] + (v & 63);
-            for (uint32_t k = 0; k < len[v]; ++k, idx += SLICE)
-                for (int i = 0; i < 2; ++i) {
-                    const uint32_t o = rp[2 * v + i] + 2 * k;
-                    map[o] = (uint32_t)ent(K.n_blocks, idx, i, 0);
-                    map[o + 1] = (uint32_t)ent(K.n_blocks, idx, i, 1);
-                }
-        }
-    }
     AmgDev* A = new AmgDev();
     S.K.amg = A;
-    NBX_TRY(up(&A->fine_map, map, A->bytes));
     A->lv.resize(H.lv.size());
     const size_t nl = H.lv.size();
     for (size_t l = 0; l < nl; ++l) {
@@ -922,8 +961,8 @@ static int amg_numeric(FemSys& S) {
     AmgDev* A = S.K.amg;
     const hipStream_t s = S.s;
     AmgDevLevel& L0 = A->lv[0];
-    hipLaunchKernelGGL(k_amg_fine_vals, dim3(grid_for(L0.a_nnz)), dim3(BLOCK), 0, s, S.K.s.val,
-                       A->fine_map, L0.a64, L0.a_nnz);
+    hipLaunchKernelGGL(k_amg_fine_vals, dim3(grid_for(S.K.s.n_blocks)), dim3(BLOCK), 0, s, S.K.s,
+                       L0.a_ptr, L0.a64);
     NBX_HIP(hipGetLastError());
     for (size_t l = 0; l < A->lv.size(); ++l) {
         AmgDevLevel& L = A->lv[l];

@@ -118,13 +118,24 @@ struct AmgHost {
 };
 
 // Build the symbolic hierarchy from the finest operator: CSR pattern
-// (rowptr[n+1], col ascending per row) with its values (to find the
-// Dirichlet-eliminated unknowns: rows whose off-diagonal values are all 0)
-// and the coordinates of the n/2 nodes (x, y interleaved).  Returns 0, -1
-// when the pattern is not a 2-dof node pattern, or -2 when host memory ran
-// out (the setup runs on up to 16 threads).
-int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+// (rowptr[n+1], col ascending per row), which of its entries hold a nonzero
+// value (nz[e] = 1: the Dirichlet-eliminated unknowns are the rows whose
+// off-diagonal entries are all 0, and the elimination's zeros are no graph
+// edges) and the coordinates of the n/2 nodes (x, y interleaved).  Returns
+// 0, -1 when the pattern is not a 2-dof node pattern, or -2 when host memory
+// ran out (the setup runs on up to 16 threads).
+int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint8_t* nz,
                    const double* xy, AmgHost& out);
+
+// The scalar CSR pattern of a 2x2-block sliced-ELL matrix of `nodes` block
+// rows (slices of 64 rows: block k of row v sits in slot slice_ptr[v / 64] +
+// v % 64 + 64 k, rowlen[v] blocks, column node bcol[slot]): row 2v + i holds
+// columns 2 bcol + 0, 2 bcol + 1 of each block in slot order, and nz[e] is bit
+// (2 i + j) of nzbits[slot], 1 where entry (i, j) of the block is nonzero.
+// Threaded like the setup.
+void amg_expand_blocks(uint32_t nodes, const uint32_t* slice_ptr, const uint8_t* rowlen,
+                       const uint32_t* bcol, const uint8_t* nzbits, HostVec<uint32_t>& rowptr,
+                       HostVec<uint32_t>& col, HostVec<uint8_t>& nz);
 
 // The power iteration's start vector: +-1 from a hash of the index, so that
 // every frequency is present (a smooth start underestimates lambda_max, and

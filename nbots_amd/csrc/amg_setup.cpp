@@ -95,7 +95,7 @@ void build_lists(uint32_t n, F f, HostVec<uint32_t>& ptr, HostVec<uint32_t>& dat
 // on the finest level, a rank-deficient coarse mode below) has no edges, and
 // on the finest level an edge needs a nonzero value (the elimination zeroes
 // the entries it decouples).
-Graph node_graph(const AmgHostLevel& L, const std::vector<uint8_t>& dead, const double* val) {
+Graph node_graph(const AmgHostLevel& L, const std::vector<uint8_t>& dead, const uint8_t* nz) {
     const uint32_t bs = L.bs, nodes = L.n / bs;
     Graph g;
     build_lists(
@@ -107,7 +107,7 @@ Graph node_graph(const AmgHostLevel& L, const std::vector<uint8_t>& dead, const 
                 for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
                     const uint32_t j = L.a_col[e], J = j / bs;
                     if (J == I || dead[j]) continue;
-                    if (val && val[e] == 0.0) continue;
+                    if (nz && !nz[e]) continue;
                     out.push_back(J);
                 }
             }
@@ -163,10 +163,10 @@ uint32_t find(const uint32_t* b, const uint32_t* e, uint32_t c) {
 // One coarsening step: fills L's prolongator fields and returns the coarse
 // level's pattern, modes B (3 per coarse unknown) and dead unknowns.
 void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<uint8_t>& dead,
-             const double* val, AmgHostLevel& C, std::vector<double>& Bc,
+             const uint8_t* nz, AmgHostLevel& C, std::vector<double>& Bc,
              std::vector<uint8_t>& dead_c) {
     const uint32_t bs = L.bs, n = L.n, nodes = n / bs;
-    const Graph g = node_graph(L, dead, val);
+    const Graph g = node_graph(L, dead, nz);
     std::vector<uint32_t> agg;
     const uint32_t na = aggregate(g, agg);
     L.nc = 3 * na;
@@ -367,7 +367,7 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
 
 }  // namespace
 
-static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint8_t* nz,
                       const double* xy, AmgHost& out) {
     out.lv.clear();
     if (n == 0 || n % 2) return -1;
@@ -387,7 +387,7 @@ static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, c
     for (uint32_t i = 0; i < n; ++i) {
         bool any = false;
         for (uint32_t e = rowptr[i]; e < rowptr[i + 1] && !any; ++e)
-            any = col[e] != i && val[e] != 0.0;
+            any = col[e] != i && nz[e];
         dead[i] = !any;
     }
     // rigid-body modes, centred on the mesh's centroid
@@ -412,7 +412,7 @@ static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, c
         by[2] = x;
     }
     out.lv.push_back(std::move(L0));
-    const double* v = val;
+    const uint8_t* v = nz;
     while (out.lv.back().n > AMG_COARSEST && (int)out.lv.size() < AMG_MAX_LEVELS) {
         AmgHostLevel C;
         std::vector<double> Bc;
@@ -434,10 +434,10 @@ static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, c
     return 0;
 }
 
-int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const double* val,
+int amg_build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, const uint8_t* nz,
                    const double* xy, AmgHost& out) {
     try {
-        return build_host(n, rowptr, col, val, xy, out);
+        return build_host(n, rowptr, col, nz, xy, out);
     } catch (const std::bad_alloc&) {
         out.lv.clear();
         return -2;
@@ -456,6 +456,35 @@ static double dense_dot(const double* x, const double* y, uint32_t len) {
     }
     for (; p < len; ++p) s0 += x[p] * y[p];
     return (s0 + s1) + (s2 + s3);
+}
+
+void amg_expand_blocks(uint32_t nodes, const uint32_t* slice_ptr, const uint8_t* rowlen,
+                       const uint32_t* bcol, const uint8_t* nzbits, HostVec<uint32_t>& rowptr,
+                       HostVec<uint32_t>& col, HostVec<uint8_t>& nz) {
+    rowptr.resize(2 * (size_t)nodes + 1);
+    rowptr[0] = 0;
+    for (uint32_t v = 0; v < nodes; ++v) {
+        rowptr[2 * (size_t)v + 1] = rowptr[2 * (size_t)v] + 2u * rowlen[v];
+        rowptr[2 * (size_t)v + 2] = rowptr[2 * (size_t)v + 1] + 2u * rowlen[v];
+    }
+    col.resize(rowptr[2 * (size_t)nodes]);
+    nz.resize(col.size());
+    par_for(nodes, [&](uint32_t v0, uint32_t v1) {
+        for (uint32_t v = v0; v < v1; ++v) {
+            size_t slot = (size_t)slice_ptr[v / 64] + v % 64;
+            for (uint32_t k = 0; k < rowlen[v]; ++k, slot += 64) {
+                const uint32_t c = bcol[slot];
+                const uint8_t bits = nzbits[slot];
+                for (uint32_t i = 0; i < 2; ++i) {
+                    const size_t o = rowptr[2 * (size_t)v + i] + 2 * (size_t)k;
+                    col[o] = 2 * c;
+                    col[o + 1] = 2 * c + 1;
+                    nz[o] = (bits >> (2 * i)) & 1u;
+                    nz[o + 1] = (bits >> (2 * i + 1)) & 1u;
+                }
+            }
+        }
+    });
 }
 
 int amg_dense_inverse(uint32_t n, double* a) {

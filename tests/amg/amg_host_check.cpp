@@ -207,7 +207,9 @@ int main(int argc, char** argv) {
     // method; 0 = plain aggregation, for the check that smoothing matters)
     const double omega_scale = argc > 3 ? atof(argv[3]) : 1.0;
     AmgHost H;
-    if (amg_build_host(n, rp.data(), col.data(), val.data(), xy.data(), H)) return 4;
+    std::vector<uint8_t> nz(nnz);
+    for (size_t e = 0; e < nnz; ++e) nz[e] = val[e] != 0.0;
+    if (amg_build_host(n, rp.data(), col.data(), nz.data(), xy.data(), H)) return 4;
     // numeric phase, with the device kernels' loops
     std::vector<Num> N(H.lv.size());
     N[0].a = val;
