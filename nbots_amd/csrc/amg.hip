@@ -49,10 +49,9 @@ struct AmgDevLevel {
     float* dinv32 = nullptr;
     double lmax = 0.0;
     double* t = nullptr;
-    uint32_t* self = nullptr;
+    uint32_t* agg = nullptr;   // [n / bs] each node's aggregate, or AMG_NONE
     uint32_t *p_ptr = nullptr, *p_col = nullptr;
     double* p64 = nullptr;
-    uint32_t* ap_map = nullptr;
     uint32_t *w_ptr = nullptr, *w_col = nullptr;
     double* w64 = nullptr;
     uint32_t *pt_ptr = nullptr, *pt_row = nullptr, *pt_pos = nullptr;
@@ -227,15 +226,16 @@ __global__ __launch_bounds__(FIN) void k_amg_pow_fin(const double* partials, uns
 
 // P = T - omega D^-1 A T: SMOOTH_SLOTS threads per node, each taking every
 // SMOOTH_SLOTS-th aggregate column group of the node's P rows (its R rows
-// share one A and one P pattern) and summing each row's A entries that map
-// to the group in entry order
+// share one A and one P pattern; a group is the three columns of one
+// aggregate) and summing, in entry order, each row's A entries whose column
+// node belongs to that aggregate
 constexpr int SMOOTH_SLOTS = 8;
 template <int R>
 __global__ __launch_bounds__(BLOCK) void k_amg_smooth_p(
     const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_col,
     const double* __restrict__ a, const double* __restrict__ dinv, const double* __restrict__ t,
-    const uint32_t* __restrict__ self, const uint32_t* __restrict__ p_ptr,
-    const uint32_t* __restrict__ ap_map, double* __restrict__ p, double omega, uint32_t nodes) {
+    const uint32_t* __restrict__ agg, const uint32_t* __restrict__ p_ptr,
+    const uint32_t* __restrict__ p_col, double* __restrict__ p, double omega, uint32_t nodes) {
     const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t I = (uint32_t)(tid / SMOOTH_SLOTS);
     if (I >= nodes) return;
@@ -250,18 +250,19 @@ __global__ __launch_bounds__(BLOCK) void k_amg_smooth_p(
     }
     for (uint32_t gi = (uint32_t)(tid % SMOOTH_SLOTS); gi < ng; gi += SMOOTH_SLOTS) {
         const uint32_t qo = 3 * gi;          // the group's offset in each row
-        const uint32_t q = p_ptr[i0] + qo;
+        const uint32_t ga = p_col[p_ptr[i0] + qo] / 3;   // its aggregate
+        const bool own = agg[I] == ga;
         double acc[R][3];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) acc[r][m] = 0.0;
-            if (self[i0 + r] == p_ptr[i0 + r] + qo)
+            if (own)
 #pragma unroll
                 for (int m = 0; m < 3; ++m) acc[r][m] += t[3 * (size_t)(i0 + r) + m];
         }
         for (uint32_t e = a_ptr[i0]; e < a_ptr[i0 + 1]; ++e) {
-            if (ap_map[e] != q) continue;
+            if (agg[a_col[e] / R] != ga) continue;
             const uint32_t j = a_col[e];
             double tj[3];
 #pragma unroll
@@ -693,8 +694,8 @@ __global__ __launch_bounds__(DENSE_BLOCK) void k_amg_dense(const float* __restri
 void amg_free(AmgDev* A) {
     if (!A) return;
     for (AmgDevLevel& L : A->lv) {
-        void* ps[] = {L.a_ptr, L.a_col, L.a64, L.dinv64, L.dinv32, L.t, L.self, L.p_ptr,
-                      L.p_col, L.p64, L.ap_map, L.w_ptr, L.w_col, L.w64, L.pt_ptr,
+        void* ps[] = {L.a_ptr, L.a_col, L.a64, L.dinv64, L.dinv32, L.t, L.agg, L.p_ptr,
+                      L.p_col, L.p64, L.w_ptr, L.w_col, L.w64, L.pt_ptr,
                       L.pt_row, L.pt_pos, L.b, L.x, L.r, L.d0, L.d1};
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -898,11 +899,10 @@ static int amg_symbolic(FemSys& S) {
                 NBX_TRY((bell_upload<3, 3>(L.nc / 3, tlen, tcol, L.tb, A->bytes)));
             }
             NBX_TRY(up(&L.t, h.t, A->bytes));
-            NBX_TRY(up(&L.self, h.self, A->bytes));
+            NBX_TRY(up(&L.agg, h.agg, A->bytes));
             NBX_TRY(up(&L.p_ptr, h.p_ptr, A->bytes));
             NBX_TRY(up(&L.p_col, h.p_col, A->bytes));
             NBX_TRY(alloc(&L.p64, L.p_nnz, A->bytes));
-            NBX_TRY(up(&L.ap_map, h.ap_map, A->bytes));
             NBX_TRY(up(&L.w_ptr, h.w_ptr, A->bytes));
             NBX_TRY(up(&L.w_col, h.w_col, A->bytes));
             NBX_TRY(alloc(&L.w64, L.w_nnz, A->bytes));
@@ -1037,8 +1037,8 @@ static int amg_numeric(FemSys& S) {
             constexpr int R = decltype(rr)::value;
             hipLaunchKernelGGL(k_amg_smooth_p<R>,
                                dim3(grid_for((size_t)(L.n / R) * SMOOTH_SLOTS)), dim3(BLOCK), 0,
-                               s, L.a_ptr, L.a_col, L.a64, L.dinv64, L.t, L.self, L.p_ptr,
-                               L.ap_map, L.p64, omega, L.n / R);
+                               s, L.a_ptr, L.a_col, L.a64, L.dinv64, L.t, L.agg, L.p_ptr,
+                               L.p_col, L.p64, omega, L.n / R);
         });
         // P and P^T in block form (D^1/2 P on the finest level when it is
         // smoothed in the scaled variables)

@@ -106,9 +106,8 @@ struct AmgHostLevel {
     // to the next coarser level (empty on the coarsest)
     uint32_t nc = 0;                  // coarse unknowns (3 per aggregate)
     HostVec<double> t;                // [3 n] T's values of row i: columns agg(i)*3 + m
-    HostVec<uint32_t> self;           // [n] P row i's position of column agg(i)*3, or NONE
+    HostVec<uint32_t> agg;            // [n / bs] each node's aggregate, or NONE
     HostVec<uint32_t> p_ptr, p_col;   // P pattern (n x nc)
-    HostVec<uint32_t> ap_map;         // [nnz(A)] P row i's position of agg(col)*3, or NONE
     HostVec<uint32_t> w_ptr, w_col;   // W = A P pattern (n x nc)
     HostVec<uint32_t> pt_ptr, pt_row, pt_pos;   // P^T: coarse column c -> (row, position)
 };

@@ -253,8 +253,6 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
     for (uint32_t i = 0; i < n; ++i)
         L.p_ptr[i + 1] = L.p_ptr[i] + 3 * (anode_ptr[i / bs + 1] - anode_ptr[i / bs]);
     L.p_col.resize(L.p_ptr[n]);
-    L.self.resize(n);
-    L.ap_map.resize(L.a_ptr[n]);
     par_for(nodes, [&](uint32_t I0, uint32_t I1) {
         for (uint32_t I = I0; I < I1; ++I) {
             const uint32_t* b = &anode[anode_ptr[I]];
@@ -263,14 +261,10 @@ void coarsen(AmgHostLevel& L, const std::vector<double>& B, const std::vector<ui
                 uint32_t q = L.p_ptr[i];
                 for (const uint32_t* A = b; A != e; ++A)
                     for (uint32_t c = 0; c < 3; ++c) L.p_col[q++] = 3 * *A + c;
-                L.self[i] = agg[I] != AMG_NONE ? L.p_ptr[i] + 3 * find(b, e, agg[I]) : AMG_NONE;
-                for (uint32_t k = L.a_ptr[i]; k < L.a_ptr[i + 1]; ++k) {
-                    const uint32_t A = agg[L.a_col[k] / bs];
-                    L.ap_map[k] = A != AMG_NONE ? L.p_ptr[i] + 3 * find(b, e, A) : AMG_NONE;
-                }
             }
         }
     });
+    L.agg.assign(agg.begin(), agg.end());
     // W = A P pattern per node: the union of the P rows its rows reach
     // (aggregates; three columns each)
     HostVec<uint32_t> wnode_ptr, wnode;
@@ -422,7 +416,7 @@ static int build_host(uint32_t n, const uint32_t* rowptr, const uint32_t* col, c
             // nothing (or no reduction) to coarsen: this level is the coarsest
             AmgHostLevel& L = out.lv.back();
             L.nc = 0;
-            L.t.clear(); L.self.clear(); L.p_ptr.clear(); L.p_col.clear(); L.ap_map.clear();
+            L.t.clear(); L.agg.clear(); L.p_ptr.clear(); L.p_col.clear();
             L.w_ptr.clear(); L.w_col.clear(); L.pt_ptr.clear(); L.pt_row.clear(); L.pt_pos.clear();
             break;
         }

@@ -243,15 +243,26 @@ int main(int argc, char** argv) {
             break;
         }
         const double omega = omega_scale * 4.0 / (3.0 * M.lmax);
+        // P row i's position of aggregate a's columns (its groups ascend)
+        auto pos_in_row = [&](uint32_t i, uint32_t a) {
+            for (uint32_t q = L.p_ptr[i]; q < L.p_ptr[i + 1]; q += 3)
+                if (L.p_col[q] == 3 * a) return q;
+            fprintf(stderr, "aggregate %u not in P row %u\n", a, i);
+            exit(3);
+        };
         M.p.assign(L.p_ptr[L.n], 0.0);
         for (uint32_t i = 0; i < L.n; ++i) {
-            if (L.self[i] != AMG_NONE)
-                for (int m = 0; m < 3; ++m) M.p[L.self[i] + m] += L.t[3 * (size_t)i + m];
+            const uint32_t own = L.agg[i / L.bs];
+            if (own != AMG_NONE) {
+                const uint32_t q = pos_in_row(i, own);
+                for (int m = 0; m < 3; ++m) M.p[q + m] += L.t[3 * (size_t)i + m];
+            }
             for (uint32_t e = L.a_ptr[i]; e < L.a_ptr[i + 1]; ++e) {
-                if (L.ap_map[e] == AMG_NONE) continue;
+                const uint32_t j = L.a_col[e], a = L.agg[j / L.bs];
+                if (a == AMG_NONE) continue;
                 const double coef = omega * M.dinv[i] * M.a[e];
-                const uint32_t j = L.a_col[e];
-                for (int m = 0; m < 3; ++m) M.p[L.ap_map[e] + m] -= coef * L.t[3 * (size_t)j + m];
+                const uint32_t q = pos_in_row(i, a);
+                for (int m = 0; m < 3; ++m) M.p[q + m] -= coef * L.t[3 * (size_t)j + m];
             }
         }
         M.w.assign(L.w_ptr[L.n], 0.0);
