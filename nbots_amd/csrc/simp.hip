@@ -70,15 +70,19 @@ __global__ void k_oc_init(OcState* st, double target) {
     *st = s;
 }
 
-// the OC candidate of one element for the multiplier lmid
-__device__ __forceinline__ double oc_candidate(double xe, double dce, double dve, double lmid,
-                                               double move) {
-    double v = xe * sqrt(-dce / dve / lmid);
+// the OC candidate of one element for the multiplier lmid, from
+// q = -dc / dv (-dc / dv / lmid evaluates as q / lmid)
+__device__ __forceinline__ double oc_candidate_q(double xe, double q, double lmid, double move) {
+    double v = xe * sqrt(q / lmid);
     if (v > xe + move) v = xe + move;
     if (v > 1.0) v = 1.0;
     if (v < xe - move) v = xe - move;
     if (v < 0.0) v = 0.0;
     return v;
+}
+__device__ __forceinline__ double oc_candidate(double xe, double dce, double dve, double lmid,
+                                               double move) {
+    return oc_candidate_q(xe, -dce / dve, lmid, move);
 }
 
 // The bisection's next three levels in one pass.  From the bracket (l1, l2)
@@ -113,9 +117,9 @@ __global__ __launch_bounds__(BLOCK) void k_oc_step3(const OcState* __restrict__ 
     oc_candidates(s, c);
     double acc[OC_CAND] = {};
     for (uint32_t e = blockIdx.x * BLOCK + threadIdx.x; e < ne; e += gridDim.x * BLOCK) {
-        const double xe = x[e], dce = dc[e], dve = dv[e];
+        const double xe = x[e], dve = dv[e], q = -dc[e] / dve;
 #pragma unroll
-        for (int k = 0; k < OC_CAND; ++k) acc[k] += dve * oc_candidate(xe, dce, dve, c[k], move);
+        for (int k = 0; k < OC_CAND; ++k) acc[k] += dve * oc_candidate_q(xe, q, c[k], move);
     }
     __shared__ double red[OC_CAND][BLOCK / WAVE];
     block_sum<OC_CAND>(acc, red);
